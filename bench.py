@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 Radiance path tracer on one MI355X.
+
+Workload (default): the triangle-mesh BVH scene -- a displaced geodesic
+icosphere of 100,820 triangles (+2 ground, +2 light), 1024x512, 64 spp,
+maxBounce 8, Beckmann 0.3 mesh, Lambert ground, one quad area light, FP32
+traversal/shading, FP64 per-pixel accumulation. One step = one full frame
+(33,554,432 camera paths). Scene data (geometry, BVH, light tables) is
+resident in HBM before the timed region; the accumulators stay on the device.
+
+Prints one JSON line. `roofline` is for the closest-hit traversal kernel
+(k_trace): algorithmic bytes per query = 44 B of queue I/O + node visits x
+node bytes + triangle tests x 48 B, with the visit counts measured by the
+same kernel on a 4-spp sample of the frame, divided by the kernel time from
+HIP events on the render's stream over the timed steps. `cpu_baseline` is
+the oracle's FP32 restatement of the reference CPU path (CPUMulticore thread
+scheme, reference BVH and traversal, release flags) on a bounded sample of
+the same frame on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cuda-raytracer_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "paths/s + wall-clock to fixed RMSE vs FP64 oracle (plain vs cached) on 1 MI355X"
+HBM_PEAK_GBS = 8000.0
+QUEUE_BYTES_PER_QUERY = 44  # o, d in (24 B); t, u, v, hit code out (16 B); bucket entry (4 B)
+TRI_BYTES = 48
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, target_seconds):
+    import numpy as np
+    import oracle_lib as ol
+    orc = ol.OracleScene(scene, "f32", traversal=0)
+    p = scene.params(seed=1)
+    w, h = p.width, p.height
+    nthreads = max(1, (os.cpu_count() or 2) - 1)
+    # probe on 8 rows, then size the sample to ~target_seconds
+    rows = np.arange(0, h, max(1, h // 8))[:8]
+    pix = (rows[:, None] * w + np.arange(w)[None, :]).ravel()
+    _, _, st = orc.render(p, 0, 1, nthreads, pix)
+    rate = pix.size / max(st["seconds"], 1e-6)
+    nrows = int(min(h, max(8, target_seconds * rate / w)))
+    rows = np.linspace(0, h - 1, nrows).astype(np.int64)
+    pix = (rows[:, None] * w + np.arange(w)[None, :]).ravel()
+    _, _, st = orc.render(p, 0, 1, nthreads, pix)
+    orc.free()
+    return {
+        "value": pix.size / st["seconds"],
+        "unit": "paths/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": f"{nrows} of {h} rows x {w} px x 1 spp ({pix.size} paths, {st['seconds']:.1f} s) of the "
+                  f"same frame; FP32 restatement of CPUMulticoreTracer + RadianceKernel, reference BVH and "
+                  f"traversal, -O3 -ffast-math -fno-finite-math-only -funroll-loops; "
+                  f"{os.cpu_count()} logical CPUs, {nthreads} tracer threads",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mesh", choices=["mesh", "spheres", "large"])
+    ap.add_argument("--bvh-width", type=int, default=0)
+    ap.add_argument("--chunk-paths", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-torch", action="store_true", help="skip torch.cuda.synchronize (profiling runs)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: this build has no multi-GPU path (paths are independent; "
+              "sharding by pixel tiles or sample ranges is described in DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from koishi_amd import kr, scenegen as sg
+
+    torch = None
+    if not args.no_torch:
+        import torch as _t
+        torch = _t
+
+    def barrier():
+        if torch is not None:
+            torch.cuda.synchronize()
+
+    log(f"[bench] building workload {args.workload}")
+    scene = sg.WORKLOADS[args.workload]()
+    dev = kr.Device(0)
+    gs = kr.GpuScene(dev, scene.to_desc(bvh_width=args.bvh_width))
+    info = gs.info()
+    log(f"[bench] scene: {info.num_triangles} tris, {info.num_spheres} spheres, {info.num_lights} lights, "
+        f"BVH{info.bvh_width} {info.bvh_nodes} nodes depth {info.bvh_depth}, build {info.build_seconds:.2f}s")
+    p = scene.params(seed=1, chunk_paths=args.chunk_paths)
+    rend = kr.Render(gs, scene.camera_struct(), p)
+    paths_per_step = p.width * p.height * p.spp
+
+    # traversal work per query, counted by the same kernel on a 4-spp sample
+    rend.set_profiling(kr.KR_PROFILE_COUNTERS)
+    rend.run(0, min(4, p.spp))
+    rend.sync()
+    st = rend.stats()
+    nodes_per_ray = st.node_visits / max(1, st.closest_rays)
+    tris_per_ray = st.tri_tests / max(1, st.closest_rays)
+    bytes_per_query = QUEUE_BYTES_PER_QUERY + nodes_per_ray * info.node_bytes + tris_per_ray * TRI_BYTES
+    log(f"[bench] per closest-hit query: {nodes_per_ray:.1f} nodes, {tris_per_ray:.1f} tri tests, "
+        f"{bytes_per_query:.0f} algorithmic bytes")
+
+    rend.set_profiling(0)
+    for i in range(args.warmup):
+        rend.run(0, p.spp)
+        rend.sync()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done")
+    rend.reset()
+    rend.set_profiling(kr.KR_PROFILE_EVENTS)
+    barrier()
+    rend.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        rend.run(0, p.spp)
+    rend.sync()
+    barrier()
+    elapsed = time.perf_counter() - t0
+    st = rend.stats()
+    log(f"[bench] {args.steps} steps in {elapsed:.3f}s")
+    rgb, valid = rend.read()
+    value = paths_per_step * args.steps / elapsed
+    trace_bytes = st.closest_rays * bytes_per_query
+    achieved = trace_bytes / (st.trace_ms * 1e-3) / 1e9 if st.trace_ms > 0 else 0.0
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_trace_mesh_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(scene, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu['value']:.0f} paths/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": {"mesh": "triangle-mesh BVH scene: displaced icosphere 100,820 tris + ground + quad "
+                                 "light, 1024x512, 64 spp, maxBounce 8, plain wavefront path",
+                         "spheres": "sphere scene: 8 spheres + ground sphere + quad light, 256x256, 16 spp, "
+                                    "maxBounce 5",
+                         "large": "large scene: 40 displaced icosphere instances 1,036,800 tris, 4 quad "
+                                  "lights, 1920x1080, 64 spp, maxBounce 8"}[args.workload],
+            "width": p.width, "height": p.height, "spp": p.spp, "max_bounce": p.max_bounce,
+            "triangles": info.num_triangles, "bvh_width": info.bvh_width, "paths_per_step": paths_per_step,
+            "mrays_per_s": (st.closest_rays + st.shadow_rays) / elapsed / 1e6,
+            "kernel_ms_per_step": {"trace": st.trace_ms / args.steps, "shade": st.shade_ms / args.steps,
+                                   "shadow": st.shadow_ms / args.steps, "other": st.other_ms / args.steps},
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_trace (closest-hit wide-BVH traversal)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_query": bytes_per_query,
+            "queries_per_launch": st.closest_rays / max(1, st.trace_launches),
+            "avg_launch_ms": st.trace_ms / max(1, st.trace_launches),
+        },
+        "cpu_baseline": cpu,
+    }
+    rend.free()
+    gs.free()
+    dev.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,477 @@
+// kr_device.h -- gfx950 device code: vector math, Philox, wide-BVH traversal,
+// watertight triangles, spheres and the Radiance shading rules.
+//
+// Semantics follow the reference Radiance path (cited per function); the
+// arithmetic is FP32 except the sphere quadratic, which is FP64 (the
+// reference's r = 1000 ground sphere loses ~4 digits of t in FP32; gfx950
+// runs FP64 VALU at half the FP32 rate and there are only a few spheres).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kr_api.h"
+#include "kr_layout.h"
+
+namespace kr {
+
+// ---------------------------------------------------------------- vectors --
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, f3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// normalize = a * (1 / sqrt(|a|^2)) (src/vec/vmath.hpp:648-652)
+__device__ __forceinline__ f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+__device__ __forceinline__ float length(f3 a) { return sqrtf(dot(a, a)); }
+// `a > b ? a : b` as in __func::max (src/vec/vmath.hpp:28-31)
+__device__ __forceinline__ float smax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float smin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ bool finite3(f3 a) { return isfinite(a.x) && isfinite(a.y) && isfinite(a.z); }
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInvPi = float(1.0 / 3.14159265358979323846);
+constexpr float kSpawnEps = 4e-3f;  // src/core/meta/varyings.hpp:97,108
+
+// ------------------------------------------------------------------- RNG --
+// Philox4x32-10 (Salmon et al., SC'11). Same counter layout as the oracle:
+// ctr = {pixel, sample, 2*bounce + dim/4, stream}, key = seed.
+struct Philox4 {
+    uint32_t v[4];
+};
+__device__ __forceinline__ Philox4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                          uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0;
+        c1 = lo1;
+        c2 = n2;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return {{c0, c1, c2, c3}};
+}
+__device__ __forceinline__ float u01(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
+
+// ------------------------------------------------------------- scene view --
+struct DevScene {
+    const float4 *nodes;
+    const float4 *tris;       // 3 float4 per triangle, leaf order
+    const float *tri_nrm;     // 9 floats per global triangle
+    const uint32_t *tri_prim; // primitive of each global triangle
+    const double *spheres;    // 4 doubles per sphere: centre, r2 (r2 = float(r)*float(r))
+    const uint32_t *sphere_prim;
+    const PrimRec *prims;
+    const ShaderRec *shaders;
+    const LightRec *lights;
+    const float *light_table; // 6 floats per entry
+    uint32_t nspheres, nlights, ntris, stack_cap;
+};
+
+// --------------------------------------------------------- ray intersection --
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1) 2013) on a
+// per-ray shear. Two-sided; accepts 0 < t < tmax. Barycentrics b1, b2 are
+// the weights of v1 and v2, the (u, v) of the reference's Moller-Trumbore
+// test (src/core/basic/ray.hpp:37-52).
+struct ShearRay {
+    f3 o;
+    f3 mx, my, mz;  // rows of the shear: A' = M (v - o)
+};
+
+__device__ __forceinline__ ShearRay make_shear(f3 o, f3 d) {
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    auto comp = [&](int k) { return k == 0 ? d.x : (k == 1 ? d.y : d.z); };
+    float dz = comp(kz);
+    if (dz < 0.f) {
+        int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    float Sx = comp(kx) / dz, Sy = comp(ky) / dz, Sz = 1.0f / dz;
+    auto e = [](int k) { return f3{k == 0 ? 1.f : 0.f, k == 1 ? 1.f : 0.f, k == 2 ? 1.f : 0.f}; };
+    ShearRay r;
+    r.o = o;
+    r.mx = e(kx) - e(kz) * Sx;
+    r.my = e(ky) - e(kz) * Sy;
+    r.mz = e(kz) * Sz;
+    return r;
+}
+
+__device__ __forceinline__ bool tri_hit(const ShearRay &r, f3 v0, f3 v1, f3 v2, float tmax, float &t, float &b1,
+                                        float &b2) {
+    f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Ax = dot(r.mx, A), Ay = dot(r.my, A);
+    float Bx = dot(r.mx, B), By = dot(r.my, B);
+    float Cx = dot(r.mx, C), Cy = dot(r.my, C);
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+        V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+        W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    float det = U + V + W;
+    if (det == 0.f) return false;
+    float T = U * dot(r.mz, A) + V * dot(r.mz, B) + W * dot(r.mz, C);
+    float inv = 1.0f / det;
+    float tt = T * inv;
+    if (!(tt > 0.f && tt < tmax)) return false;
+    t = tt;
+    b1 = V * inv;
+    b2 = W * inv;
+    return true;
+}
+
+// Sphere::intersect (src/core/primitive/sphere.hpp:71-94) in FP64: near root,
+// else far root, hit iff t >= 0.
+__device__ __forceinline__ bool sphere_hit(const double *s, f3 o, f3 d, double &t) {
+    double ocx = s[0] - double(o.x), ocy = s[1] - double(o.y), ocz = s[2] - double(o.z);
+    double dx = d.x, dy = d.y, dz = d.z;
+    double poc = dx * ocx + dy * ocy + dz * ocz;
+    double oc2 = ocx * ocx + ocy * ocy + ocz * ocz;
+    double det = s[3] - (oc2 - poc * poc);
+    if (det < 0) return false;
+    det = sqrt(det);
+    t = poc - det;
+    if (t < 0) t = poc + det;
+    return t >= 0;
+}
+
+// ------------------------------------------------------------- BVH walk --
+// Stack: the top LDS_DEPTH entries of each lane live in LDS ([depth][lane],
+// conflict free), deeper ones spill to a per-lane global area. The spill
+// capacity comes from the tree depth, so the stack cannot overflow.
+template <int LDS_DEPTH>
+struct Stack {
+    uint32_t *lds_ref;  // [LDS_DEPTH][blockDim]
+    float *lds_t;
+    uint32_t *spill_ref;  // per lane, (cap - LDS_DEPTH) entries, stride = lanes
+    float *spill_t;
+    uint32_t lane, lanes, gstride;
+    int sp = 0;
+    __device__ __forceinline__ void push(uint32_t ref, float t) {
+        if (sp < LDS_DEPTH) {
+            lds_ref[sp * lanes + lane] = ref;
+            lds_t[sp * lanes + lane] = t;
+        } else {
+            size_t k = size_t(sp - LDS_DEPTH) * gstride;
+            spill_ref[k] = ref;
+            spill_t[k] = t;
+        }
+        ++sp;
+    }
+    __device__ __forceinline__ void pop(uint32_t &ref, float &t) {
+        --sp;
+        if (sp < LDS_DEPTH) {
+            ref = lds_ref[sp * lanes + lane];
+            t = lds_t[sp * lanes + lane];
+        } else {
+            size_t k = size_t(sp - LDS_DEPTH) * gstride;
+            ref = spill_ref[k];
+            t = spill_t[k];
+        }
+    }
+};
+
+struct RayBox {
+    f3 o, inv, oinv;
+};
+__device__ __forceinline__ RayBox make_raybox(f3 o, f3 d) {
+    auto fix = [](float v) { return fabsf(v) < 1e-30f ? copysignf(1e-30f, v) : v; };
+    RayBox rb;
+    rb.o = o;
+    rb.inv = {1.0f / fix(d.x), 1.0f / fix(d.y), 1.0f / fix(d.z)};
+    rb.oinv = o * rb.inv;
+    return rb;
+}
+
+// Slab test of one child with the ray interval [0, tmax]; conservative by
+// (1 + 2 gamma_3) on the far distance (Ize, JCGT 2(2) 2013).
+__device__ __forceinline__ float box_enter(const RayBox &rb, float lx, float hx, float ly, float hy, float lz, float hz,
+                                           float tmax) {
+    float tx0 = fmaf(lx, rb.inv.x, -rb.oinv.x), tx1 = fmaf(hx, rb.inv.x, -rb.oinv.x);
+    float ty0 = fmaf(ly, rb.inv.y, -rb.oinv.y), ty1 = fmaf(hy, rb.inv.y, -rb.oinv.y);
+    float tz0 = fmaf(lz, rb.inv.z, -rb.oinv.z), tz1 = fmaf(hz, rb.inv.z, -rb.oinv.z);
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), 0.f));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tmax));
+    tf *= 1.00000024f;
+    return tn <= tf ? tn : __builtin_inff();
+}
+
+struct HitRec {
+    float t, u, v;
+    uint32_t code;  // global triangle index, kSphereBit|sphere, or kMissCode
+};
+
+template <int W>
+struct NodeView {
+    float lo[3][W], hi[3][W];
+    uint32_t ref[W];
+};
+
+template <int W>
+__device__ __forceinline__ void load_node(const float4 *nodes, uint32_t idx, NodeView<W> &nv) {
+    constexpr int NF4 = int(node_floats(W)) / 4;
+    const float4 *p = nodes + size_t(idx) * NF4;
+    float f[NF4 * 4];
+#pragma unroll
+    for (int i = 0; i < NF4; ++i) {
+        float4 q = p[i];
+        f[4 * i + 0] = q.x;
+        f[4 * i + 1] = q.y;
+        f[4 * i + 2] = q.z;
+        f[4 * i + 3] = q.w;
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            nv.lo[a][c] = f[(2 * a) * W + c];
+            nv.hi[a][c] = f[(2 * a + 1) * W + c];
+        }
+        nv.ref[c] = __float_as_uint(f[6 * W + c]);
+    }
+}
+
+// Closest hit over the wide BVH and the sphere list: ordered traversal,
+// nearest child first, far children pushed with their entry distance and
+// culled on pop. Equivalent to the reference's primitive loop + per-mesh DFS
+// (src/core/meta/scene.hpp:35-47, src/core/primitive/mesh.cu:200-267) up to
+// exact ties.
+template <int W, int LDS_DEPTH>
+__device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, Stack<LDS_DEPTH> &st,
+                                                uint64_t *counts = nullptr) {
+    HitRec h;
+    h.t = __builtin_inff();
+    h.u = h.v = 0.f;
+    h.code = kMissCode;
+    // spheres (linear, FP64)
+    for (uint32_t s = 0; s < S.nspheres; ++s) {
+        double ts;
+        if (sphere_hit(S.spheres + 4 * s, o, d, ts)) {
+            float tf = float(ts);
+            if (tf < h.t) {
+                h.t = tf;
+                h.code = kSphereBit | s;
+            }
+        }
+    }
+    if (S.ntris == 0) return h;
+    RayBox rb = make_raybox(o, d);
+    ShearRay sr = make_shear(o, d);
+    uint32_t cur = 0;  // root is always an inner node
+    uint64_t nvis = 0, ntri = 0;
+    st.sp = 0;
+    while (true) {
+        if (!(cur & kLeafBit)) {
+            NodeView<W> nv;
+            load_node<W>(S.nodes, cur, nv);
+            ++nvis;
+            float key[W];
+            uint32_t ref[W];
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                float tn = box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
+                                     h.t);
+                bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
+                key[c] = ok ? tn : __builtin_inff();
+                ref[c] = ok ? nv.ref[c] : kEmpty;
+            }
+            // sort (key, ref) ascending: insertion network over W entries
+#pragma unroll
+            for (int i = 1; i < W; ++i) {
+#pragma unroll
+                for (int j = i; j > 0; --j) {
+                    bool sw = key[j] < key[j - 1];
+                    float tk = sw ? key[j] : key[j - 1];
+                    float tk2 = sw ? key[j - 1] : key[j];
+                    uint32_t tr = sw ? ref[j] : ref[j - 1];
+                    uint32_t tr2 = sw ? ref[j - 1] : ref[j];
+                    key[j - 1] = tk;
+                    key[j] = tk2;
+                    ref[j - 1] = tr;
+                    ref[j] = tr2;
+                }
+            }
+#pragma unroll
+            for (int c = W - 1; c >= 1; --c)
+                if (ref[c] != kEmpty) st.push(ref[c], key[c]);
+            if (ref[0] != kEmpty) {
+                cur = ref[0];
+                continue;
+            }
+        } else {
+            uint32_t first = cur & kLeafFirstMask;
+            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 *tp = S.tris + 3 * size_t(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                float tt, b1, b2;
+                ++ntri;
+                if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, h.t, tt, b1, b2)) {
+                    h.t = tt;
+                    h.u = b1;
+                    h.v = b2;
+                    h.code = __float_as_uint(b.w);
+                }
+            }
+        }
+        // pop the next subtree still in front of the closest hit
+        bool found = false;
+        while (st.sp > 0) {
+            float tn;
+            st.pop(cur, tn);
+            if (tn <= h.t) {
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    if (counts) {
+        counts[0] = nvis;
+        counts[1] = ntri;
+    }
+    return h;
+}
+
+// Any hit on the segment (o, d, tmax): Scene::intersect(Seg)
+// (src/core/meta/scene.hpp:57-67). Triangles block for 0 < t <= tmax,
+// spheres for a ray hit with t <= tmax (src/core/primitive/primitive.hpp:16-21).
+template <int W, int LDS_DEPTH>
+__device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, float tmax, Stack<LDS_DEPTH> &st) {
+    for (uint32_t s = 0; s < S.nspheres; ++s) {
+        double ts;
+        if (sphere_hit(S.spheres + 4 * s, o, d, ts) && ts <= double(tmax)) return true;
+    }
+    if (S.ntris == 0) return false;
+    RayBox rb = make_raybox(o, d);
+    ShearRay sr = make_shear(o, d);
+    // tri_hit accepts t < limit; the segment includes its end point.
+    float limit = tmax * 1.00000012f;
+    uint32_t cur = 0;
+    st.sp = 0;
+    while (true) {
+        if (!(cur & kLeafBit)) {
+            NodeView<W> nv;
+            load_node<W>(S.nodes, cur, nv);
+            uint32_t next = kEmpty;
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                float tn = box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
+                                     limit);
+                if (nv.ref[c] != kEmpty && tn != __builtin_inff()) {
+                    if (next == kEmpty)
+                        next = nv.ref[c];
+                    else
+                        st.push(nv.ref[c], tn);
+                }
+            }
+            if (next != kEmpty) {
+                cur = next;
+                continue;
+            }
+        } else {
+            uint32_t first = cur & kLeafFirstMask;
+            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 *tp = S.tris + 3 * size_t(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                float tt, b1, b2;
+                if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, limit, tt, b1, b2) &&
+                    tt <= tmax)
+                    return true;
+            }
+        }
+        if (st.sp == 0) break;
+        float tn;
+        st.pop(cur, tn);
+    }
+    return false;
+}
+
+// ------------------------------------------------------------- shading --
+// H::fromEular (src/util/hemisphere.hpp:66-69)
+__device__ __forceinline__ f3 from_euler(float sinth, float costh, float phi) {
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    return {sinth * sp, sinth * cp, costh};
+}
+
+// H::sampleCos (src/util/hemisphere.hpp:76-95)
+__device__ __forceinline__ f3 sample_cos(float r0, float r1) {
+    float ux = 2.f * r0 - 1.f, uy = 2.f * r1 - 1.f;
+    if (ux == 0.f && uy == 0.f) return {0.f, 0.f, 1.f};
+    float r, phi;
+    if (ux * ux > uy * uy) {
+        r = ux;
+        phi = (kPi / 4.f) * (uy / ux);
+    } else {
+        r = uy;
+        phi = (kPi / 2.f) - ((kPi / 4.f) * (ux / uy));
+    }
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    return {r * cp, r * sp, sqrtf(smax(0.f, 1.f - r * r))};
+}
+
+// BeckmannDistribution::sample / f (src/ext/distribution/beckmann.hpp:194-213)
+__device__ __forceinline__ f3 beckmann_sample(float u0, float u1, float alpha2) {
+    float tg2th = -alpha2 * logf(u0);
+    float phi = u1 * 2.f * kPi;
+    float costh = 1.f / sqrtf(1.f + tg2th);
+    float sinth = sqrtf(smax(0.f, 1.f - costh * costh));
+    return from_euler(sinth, costh, phi);
+}
+__device__ __forceinline__ float beckmann_f(f3 w, float alpha2) {
+    float cos2 = w.z * w.z;
+    float tg2 = smax(0.f, 1.f - cos2) / cos2;
+    return expf(-tg2 / alpha2) / (kPi * alpha2 * (cos2 * cos2)) * fabsf(w.z);
+}
+__device__ __forceinline__ float cos_f(f3 w) { return w.z >= 0.f ? kInvPi : 0.f; }
+
+// Shader::execute(compute_f_by_wi_wo) (lambert.hpp:31-33, glossy.hpp:77-79,
+// emission.hpp:117-122).
+__device__ __forceinline__ f3 shader_eval(const ShaderRec &sh, f3 wo, f3 wi) {
+    f3 c{sh.cr[0], sh.cr[1], sh.cr[2]};
+    if (sh.type == KR_SHADER_GLOSSY) {
+        f3 h = normalize(wo + wi);
+        float D = sh.dist == KR_DIST_COSINE ? cos_f(h) : beckmann_f(h, sh.alpha2);
+        return c * D;
+    }
+    return c * cos_f(wi);
+}
+
+// Shader::execute(sample_wi_f_by_wo) (lambert.hpp:26-30, glossy.hpp:73-76).
+__device__ __forceinline__ void shader_sample(const ShaderRec &sh, f3 wo, float u0, float u1, f3 &wi, f3 &f) {
+    f3 c{sh.cr[0], sh.cr[1], sh.cr[2]};
+    if (sh.type == KR_SHADER_GLOSSY) {
+        f3 h = sh.dist == KR_DIST_COSINE ? sample_cos(u0, u1) : beckmann_sample(u0, u1, sh.alpha2);
+        wi = h * (dot(h, wo) * 2.f) - wo;  // reflect (src/vec/vmath.hpp:470-473)
+        f = wo.z * wi.z > 0.f ? c : f3{0.f, 0.f, 0.f};
+    } else {
+        wi = sample_cos(u0, u1);
+        f = c * cos_f(wi) / kInvPi;
+    }
+}
+
+}  // namespace kr

@@ -1,0 +1,62 @@
+// kr_layout.h -- data layout shared by the host builder and the gfx950 kernels.
+//
+// Everything the kernels read lives in flat arrays in HBM:
+//
+//  Wide BVH node (W = 2, 4 or 8 children), 16-byte aligned, NODE_FLOATS(W)
+//  floats each:  lo.x[W] hi.x[W] lo.y[W] hi.y[W] lo.z[W] hi.z[W] ref[W] pad
+//  A child ref is an inner-node index, a leaf (LEAF bit | (count-1) << 27 |
+//  first triangle) or EMPTY.
+//
+//  Triangles in leaf order, 3 float4 each (48 B):
+//    {v0.xyz, prim index bits}, {v1.xyz, global triangle index bits}, {v2.xyz, 0}
+//
+//  Per input triangle (global index, gathered only on a hit): 9 floats of
+//  vertex normals and the owning primitive.
+#pragma once
+#include <cstdint>
+
+namespace kr {
+
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kLeafCountShift = 27;
+constexpr uint32_t kLeafFirstMask = (1u << kLeafCountShift) - 1u;
+constexpr uint32_t kMaxLeafTris = 16;
+constexpr uint32_t kSphereBit = 0x80000000u;  // hit code: sphere index
+constexpr uint32_t kMissCode = 0xFFFFFFFFu;
+
+constexpr uint32_t node_floats(uint32_t w) { return (7 * w + 3) / 4 * 4; }
+
+inline uint32_t make_leaf(uint32_t first, uint32_t count) {
+    return kLeafBit | ((count - 1) << kLeafCountShift) | first;
+}
+
+// Device-side primitive / shader / light records.
+struct PrimRec {
+    uint32_t type;       // KR_PRIM_*
+    uint32_t shader;
+    uint32_t tri_begin;  // mesh: first global triangle
+    uint32_t sphere;     // sphere: index into sphere arrays
+};
+
+struct ShaderRec {
+    uint32_t type;   // KR_SHADER_*
+    uint32_t dist;   // KR_DIST_*
+    float alpha2;    // roughness^2
+    float pad;
+    float cr[4];     // Lambert color*R; Glossy color; Emission (1,1,1)
+    float emission[4];
+};
+
+struct LightRec {
+    uint32_t prim;
+    uint32_t type;      // KR_PRIM_*
+    uint32_t table;     // mesh: first entry in the light table (1024 entries)
+    float pdf;          // mesh: 1 / total padded area
+    float emission[4];  // E = color * strength
+    float center[4];    // sphere: centre.xyz, r2
+};
+
+constexpr uint32_t kLightTable = 1024;
+
+}  // namespace kr

@@ -1,0 +1,1345 @@
+// kr_lib.hip -- C ABI (include/kr_api.h) and the wavefront Radiance kernels
+// for gfx950.
+//
+// Per chunk of P camera paths (one slot per path), each bounce b runs:
+//   k_trace   closest hit of every queued ray; hits go to per-material
+//             buckets (Lambert / GlossyBSDF / Emission) by wave-aggregated
+//             appends, misses end their path
+//   k_shade   one pass over the buckets: frame, area-light pick and sample,
+//             BSDF eval (NEE), emission, BSDF sample, throughput, Russian
+//             roulette; appends shadow segments and continuation rays
+//   k_shadow  any-hit test of the shadow segments; adds unoccluded NEE
+// and after the last bounce k_accumulate adds each path's radiance to its
+// pixel in FP64 (NaN/Inf samples rejected, src/core/tracer/cpuMulticore.cu:58-64).
+// Every kernel is a grid-stride loop over a queue length held in device
+// memory, so no bounce needs a host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bvh_build.h"
+#include "kr_api.h"
+#include "kr_device.h"
+#include "kr_layout.h"
+
+using namespace kr;
+
+// ------------------------------------------------------------- errors --
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define KR_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(e_ == hipErrorOutOfMemory ? KR_E_OOM : KR_E_HIP,                       \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                    \
+    } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kLdsDepth = 12;
+
+// Host Philox4x32-10 for the light tables (same function as the device one).
+void philox_host(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; ++r) {
+        uint64_t p0 = uint64_t(0xD2511F53u) * c0;
+        uint64_t p1 = uint64_t(0xCD9E8D57u) * c2;
+        uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0;
+        c1 = uint32_t(p1);
+        c2 = n2;
+        c3 = uint32_t(p0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+inline float u01h(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
+constexpr uint32_t kStreamLight = 0x4C495445u;
+
+struct H3 {
+    float x, y, z;
+};
+inline H3 hsub(H3 a, H3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline H3 hadd(H3 a, H3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline H3 hmul(H3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float hdot(H3 a, H3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline H3 hcross(H3 a, H3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline H3 hnorm(H3 a) { return hmul(a, 1.0f / std::sqrt(hdot(a, a))); }
+inline H3 hload(const float *p) { return {p[0], p[1], p[2]}; }
+
+template <class T>
+int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(KR_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return KR_OK;
+}
+}  // namespace
+
+// ---------------------------------------------------------- handle types --
+struct kr_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    std::string name;
+};
+
+struct kr_scene {
+    kr_device *dev = nullptr;
+    DevScene view{};
+    uint32_t width = 4;
+    std::vector<void *> allocs;
+    // host copies needed to answer queries
+    std::vector<kr_prim_desc> prims;
+    std::vector<uint32_t> tri_prim;
+    std::vector<uint32_t> sphere_prim;
+    std::vector<LightRec> lights;
+    std::vector<float> light_table;
+    kr_scene_info info{};
+    // spill stack for the ray-query entry points
+    uint32_t *spill = nullptr;
+    size_t spill_lanes = 0;
+};
+
+struct RayQ {
+    uint32_t *slot;
+    float *ox, *oy, *oz, *dx, *dy, *dz, *bx, *by, *bz;
+};
+struct ShadowQ {
+    uint32_t *slot;
+    float *ox, *oy, *oz, *dx, *dy, *dz, *tmax, *cx, *cy, *cz;
+};
+struct HitQ {
+    float *t, *u, *v;
+    uint32_t *code;
+};
+struct LensDev {
+    float n[3], u[3], v[3], du[3], dv[3], o[3];
+    uint32_t lens, pw, w, h;
+};
+
+// counters layout
+enum : int {
+    C_RAY0 = 0,
+    C_RAY1 = 1,
+    C_BUCKET = 2,  // 3 entries
+    C_SHADOW = 5,
+    C_NUM = 8
+};
+enum : int { S_CLOSEST = 0, S_SHADOW = 1, S_INVALID = 2, S_PATHS = 3, S_NODES = 4, S_TRIS = 5, S_NUM = 8 };
+enum : int { K_TRACE = 0, K_SHADE = 1, K_SHADOW = 2, K_OTHER = 3, K_KINDS = 4 };
+
+struct kr_render {
+    kr_scene *scene = nullptr;
+    kr_camera cam{};
+    kr_params prm{};
+    LensDev lens{};
+    uint32_t P = 0;  // paths per chunk
+    uint32_t npix = 0;
+    RayQ q[2]{};
+    HitQ hit{};
+    uint32_t *bucket[3]{};
+    ShadowQ sq{};
+    float *L = nullptr;  // 3 * P
+    uint32_t *counters = nullptr;
+    unsigned long long *stats = nullptr;
+    double *acc = nullptr;
+    uint32_t *valid = nullptr;
+    uint32_t *spill = nullptr;
+    size_t spill_lanes = 0;
+    int grid = 0;
+    std::vector<void *> allocs;
+    double last_seconds = 0;
+    uint64_t paths = 0;
+    // profiling (KR_PROFILE_*): pending event pairs per kernel kind
+    uint32_t profile = 0;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    double kind_ms[K_KINDS] = {0, 0, 0, 0};
+    uint64_t kind_launches[K_KINDS] = {0, 0, 0, 0};
+};
+
+namespace {
+void resolve_events(kr_render *r);
+}  // namespace
+
+// ------------------------------------------------------------- kernels --
+namespace {
+
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (mask) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if (int(lane) == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+        base = __shfl(base, leader);
+    }
+    return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ void wave_count(unsigned long long *counter, bool pred) {
+    unsigned long long mask = __ballot(pred);
+    if (mask && __lane_id() == uint32_t(__ffsll((unsigned long long)mask) - 1))
+        atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
+template <int LDS>
+__device__ __forceinline__ Stack<LDS> make_stack(uint32_t *lds_ref, float *lds_t, uint32_t *spill, size_t spill_lanes) {
+    Stack<LDS> st;
+    st.lds_ref = lds_ref;
+    st.lds_t = lds_t;
+    st.lane = threadIdx.x;
+    st.lanes = blockDim.x;
+    size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    st.gstride = uint32_t(spill_lanes);
+    st.spill_ref = spill + g;
+    st.spill_t = reinterpret_cast<float *>(spill + spill_lanes * 64) + g;
+    return st;
+}
+
+// Ray-query entry point: closest hit of n rays (parity hook).
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_query_closest(DevScene S, const float *o, const float *d, uint32_t n,
+                                                          float *t, uint32_t *prim, uint32_t *tri, float *u, float *v,
+                                                          uint32_t *spill, size_t spill_lanes, uint64_t *visits) {
+    __shared__ uint32_t sref[kLdsDepth * kBlock];
+    __shared__ float st_[kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        f3 ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
+        f3 rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+        uint64_t cnt[2] = {0, 0};
+        HitRec h = trace_closest<W, kLdsDepth>(S, ro, rd, st, visits ? cnt : nullptr);
+        if (visits) {
+            visits[2 * i] = cnt[0];
+            visits[2 * i + 1] = cnt[1];
+        }
+        t[i] = h.t;
+        if (h.code == kMissCode) {
+            prim[i] = KR_MISS;
+            tri[i] = 0;
+            u[i] = v[i] = 0.f;
+        } else if (h.code & kSphereBit) {
+            prim[i] = S.sphere_prim[h.code & ~kSphereBit];
+            tri[i] = 0;
+            u[i] = v[i] = 0.f;
+        } else {
+            uint32_t p = S.tri_prim[h.code];
+            prim[i] = p;
+            tri[i] = h.code - S.prims[p].tri_begin;
+            u[i] = h.u;
+            v[i] = h.v;
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_query_shadow(DevScene S, const float *o, const float *d, const float *tmax,
+                                                         uint32_t n, uint8_t *occ, uint32_t *spill,
+                                                         size_t spill_lanes) {
+    __shared__ uint32_t sref[kLdsDepth * kBlock];
+    __shared__ float st_[kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        f3 ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
+        f3 rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
+        occ[i] = trace_occluded<W, kLdsDepth>(S, ro, rd, tmax[i], st) ? 1 : 0;
+    }
+}
+
+__global__ void k_reset(uint32_t *counters, uint32_t keep_ray_slot) {
+    int i = threadIdx.x;
+    if (i < C_NUM && i != int(keep_ray_slot)) counters[i] = 0;
+}
+
+// Camera rays of chunk [g0, g0 + P): slot j is path g = g0 + j, sample
+// g / npix of pixel g % npix. PinholeLens/OrthographicLens::sample
+// (src/core/misc/lens.hpp:145-164).
+__global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, uint64_t g0, uint32_t P, uint32_t npix, RayQ q,
+                                                   float *L, uint32_t *counters, int qi) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
+        uint64_t g = g0 + j;
+        uint32_t s = uint32_t(g / npix), px = uint32_t(g % npix);
+        uint32_t x = px % lens.w, y = px / lens.w;
+        float fx = float(x) - float(lens.w) * 0.5f, fy = float(y) - float(lens.h) * 0.5f;
+        float kx = float(s % lens.pw), ky = float(s / lens.pw);
+        f3 n{lens.n[0], lens.n[1], lens.n[2]}, u{lens.u[0], lens.u[1], lens.u[2]}, v{lens.v[0], lens.v[1], lens.v[2]};
+        f3 du{lens.du[0], lens.du[1], lens.du[2]}, dv{lens.dv[0], lens.dv[1], lens.dv[2]};
+        f3 o{lens.o[0], lens.o[1], lens.o[2]};
+        f3 d;
+        if (lens.lens == KR_LENS_ORTHOGRAPHIC) {
+            f3 dd = u * fx + v * fy;
+            dd = dd + (du * kx + dv * ky);
+            o = o + dd * 4.f / float(lens.w);
+            d = normalize(n);
+        } else {
+            f3 t = n + u * fx + v * fy;
+            d = normalize(t + du * kx + dv * ky);
+        }
+        q.slot[j] = j;
+        q.ox[j] = o.x;
+        q.oy[j] = o.y;
+        q.oz[j] = o.z;
+        q.dx[j] = d.x;
+        q.dy[j] = d.y;
+        q.dz[j] = d.z;
+        q.bx[j] = 1.f;
+        q.by[j] = 1.f;
+        q.bz[j] = 1.f;
+        L[3 * j] = 0.f;
+        L[3 * j + 1] = 0.f;
+        L[3 * j + 2] = 0.f;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) counters[qi] = P;
+}
+
+template <int W, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, RayQ q, int qi, HitQ hit, uint32_t *b0, uint32_t *b1,
+                                                  uint32_t *b2, uint32_t *counters, unsigned long long *stats,
+                                                  int sort, uint32_t *spill, size_t spill_lanes) {
+    __shared__ uint32_t sref[kLdsDepth * kBlock];
+    __shared__ float st_[kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    uint32_t n = counters[qi];
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < n;
+        HitRec h;
+        h.code = kMissCode;
+        if (active) {
+            f3 o{q.ox[i], q.oy[i], q.oz[i]}, d{q.dx[i], q.dy[i], q.dz[i]};
+            if (COUNT) {
+                uint64_t cnt[2] = {0, 0};
+                h = trace_closest<W, kLdsDepth>(S, o, d, st, cnt);
+                atomicAdd(&stats[S_NODES], (unsigned long long)cnt[0]);
+                atomicAdd(&stats[S_TRIS], (unsigned long long)cnt[1]);
+            } else {
+                h = trace_closest<W, kLdsDepth>(S, o, d, st);
+            }
+            hit.t[i] = h.t;
+            hit.u[i] = h.u;
+            hit.v[i] = h.v;
+            hit.code[i] = h.code;
+        }
+        bool hitp = active && h.code != kMissCode;
+        uint32_t mat = 0;
+        if (hitp && sort) {
+            uint32_t prim = (h.code & kSphereBit) ? S.sphere_prim[h.code & ~kSphereBit] : S.tri_prim[h.code];
+            mat = S.shaders[S.prims[prim].shader].type;
+        }
+        wave_count(&stats[S_CLOSEST], active);
+        uint32_t p0 = wave_append(&counters[C_BUCKET + 0], hitp && mat == 0);
+        if (hitp && mat == 0) b0[p0] = i;
+        if (sort) {
+            uint32_t p1 = wave_append(&counters[C_BUCKET + 1], hitp && mat == 1);
+            if (hitp && mat == 1) b1[p1] = i;
+            uint32_t p2 = wave_append(&counters[C_BUCKET + 2], hitp && mat == 2);
+            if (hitp && mat == 2) b2[p2] = i;
+        }
+    }
+}
+
+struct ShadeArgs {
+    DevScene S;
+    RayQ q, qn;
+    int qi, qni;
+    HitQ hit;
+    uint32_t *b0, *b1, *b2;
+    ShadowQ sq;
+    float *L;
+    uint32_t *counters;
+    unsigned long long *stats;
+    uint64_t g0;
+    uint32_t npix, bounce, max_bounce, seed_lo, seed_hi;
+};
+
+// RadianceKernel::execute loop body (src/core/kernel/radiance.cu:26-71) for
+// one path vertex.
+__global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
+    const DevScene &S = a.S;
+    uint32_t n0 = a.counters[C_BUCKET + 0], n1 = a.counters[C_BUCKET + 1], n2 = a.counters[C_BUCKET + 2];
+    uint32_t total = n0 + n1 + n2;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint32_t k = base + threadIdx.x;
+        bool active = k < total;
+        bool push_shadow = false, push_ray = false;
+        uint32_t slot = 0;
+        f3 so{0, 0, 0}, sd{0, 0, 0}, contrib{0, 0, 0}, no{0, 0, 0}, nd{0, 0, 0}, nb{0, 0, 0};
+        float stmax = 0.f;
+        if (active) {
+            uint32_t i = k < n0 ? a.b0[k] : (k < n0 + n1 ? a.b1[k - n0] : a.b2[k - n0 - n1]);
+            slot = a.q.slot[i];
+            f3 o{a.q.ox[i], a.q.oy[i], a.q.oz[i]}, d{a.q.dx[i], a.q.dy[i], a.q.dz[i]};
+            f3 beta{a.q.bx[i], a.q.by[i], a.q.bz[i]};
+            float t = a.hit.t[i];
+            uint32_t code = a.hit.code[i];
+            uint64_t g = a.g0 + slot;
+            uint32_t sample = uint32_t(g / a.npix), pixel = uint32_t(g % a.npix);
+            // Scene::intersect frame (src/core/meta/scene.hpp:48-53)
+            f3 nrm;
+            uint32_t prim;
+            if (code & kSphereBit) {
+                uint32_t s = code & ~kSphereBit;
+                const double *sp = S.spheres + 4 * s;
+                prim = S.sphere_prim[s];
+                // hit.data = normalize(o + t d - c) (src/core/primitive/sphere.hpp:92)
+                double hx = double(o.x) + double(t) * double(d.x) - sp[0];
+                double hy = double(o.y) + double(t) * double(d.y) - sp[1];
+                double hz = double(o.z) + double(t) * double(d.z) - sp[2];
+                double inv = 1.0 / sqrt(hx * hx + hy * hy + hz * hz);
+                nrm = f3{float(hx * inv), float(hy * inv), float(hz * inv)};
+            } else {
+                prim = S.tri_prim[code];
+                const float *nn = S.tri_nrm + 9 * size_t(code);
+                float bu = a.hit.u[i], bv = a.hit.v[i], b0w = 1.f - bu - bv;
+                f3 n0{nn[0], nn[1], nn[2]}, n1{nn[3], nn[4], nn[5]}, n2{nn[6], nn[7], nn[8]};
+                nrm = normalize(n0 * b0w + n1 * bu + n2 * bv);
+            }
+            f3 fu = normalize(cross(nrm, f3{0.f, 1.f, 0.f}));
+            f3 fv = normalize(cross(nrm, fu));
+            f3 p = o + d * t;
+            f3 md = -d;
+            f3 wo = normalize(f3{dot(md, fu), dot(md, fv), dot(md, nrm)});
+            const ShaderRec sh = S.shaders[S.prims[prim].shader];
+            f3 L{0.f, 0.f, 0.f};
+            bool touch_L = false;
+            Philox4 r0 = philox(pixel, sample, 2 * a.bounce, 0u, a.seed_lo, a.seed_hi);
+            Philox4 r1 = philox(pixel, sample, 2 * a.bounce + 1, 0u, a.seed_lo, a.seed_hi);
+            if (S.nlights) {
+                // light pick (radiance.cu:32-34) and AreaLight::sample (areaLight.hpp:19-31)
+                float xi = u01(r0.v[0]);
+                uint32_t li_idx = min(uint32_t(floorf(xi * float(S.nlights))), S.nlights - 1u);
+                float lpdf = 1.f / float(S.nlights);
+                const LightRec lr = S.lights[li_idx];
+                float u0 = u01(r0.v[1]), u1 = u01(r0.v[2]);
+                f3 lp, ln;
+                float pdf;
+                if (lr.type == KR_PRIM_MESH) {
+                    uint32_t e = min(uint32_t(floorf(u0 * 1024.f)), 1023u);
+                    const float *te = S.light_table + 6 * (size_t(lr.table) + e);
+                    lp = f3{te[0], te[1], te[2]};
+                    ln = f3{te[3], te[4], te[5]};
+                    pdf = lr.pdf;
+                } else {
+                    // Sphere::sample (sphere.hpp:62-70)
+                    float st0, ct0;
+                    sincosf(u0 * kPi, &st0, &ct0);
+                    f3 w = from_euler(st0, ct0, u1 * kPi * 2.f);
+                    lp = f3{lr.center[0], lr.center[1], lr.center[2]} + w * sqrtf(lr.center[3]);
+                    ln = w;
+                    pdf = 1.f / (4.f * kPi * lr.center[3]);
+                }
+                f3 wl = p - lp;
+                float d2 = dot(wl, wl);
+                pdf *= d2 / fabsf(dot(ln, normalize(wl)));
+                // LocalVaryings::emitSeg (varyings.hpp:27-37)
+                f3 sdv = lp - p;
+                f3 sdir = normalize(sdv);
+                f3 diff = normalize(sdir) * kSpawnEps;
+                f3 sorig = p + diff;
+                float slen = length(sdv - diff * 2.f);
+                f3 wi = normalize(lp - p);
+                wi = normalize(f3{dot(wi, fu), dot(wi, fv), dot(wi, nrm)});
+                f3 f = shader_eval(sh, wo, wi);
+                f3 E{lr.emission[0], lr.emission[1], lr.emission[2]};
+                f3 li = E / pdf;
+                f3 c = beta * f * li * fabsf(wi.z) / lpdf;
+                if (!finite3(c)) {
+                    // The sample is rejected whether or not the segment is
+                    // occluded (an occluded li is 0/pdf), so no query is needed.
+                    L = L + c;
+                    touch_L = true;
+                } else if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
+                    push_shadow = true;
+                    so = sorig;
+                    sd = sdir;
+                    stmax = slen;
+                    contrib = c;
+                }
+            }
+            if (sh.type == KR_SHADER_EMISSION && S.nlights) {
+                L = L + beta * f3{sh.emission[0], sh.emission[1], sh.emission[2]};
+                touch_L = true;
+            }
+            if (touch_L) {
+                a.L[3 * slot] += L.x;
+                a.L[3 * slot + 1] += L.y;
+                a.L[3 * slot + 2] += L.z;
+            }
+            // BSDF sample (radiance.cu:56-60)
+            f3 wi, f;
+            shader_sample(sh, wo, u01(r0.v[3]), u01(r1.v[0]), wi, f);
+            beta = beta * (f * fabsf(wi.z));
+            f3 gd = normalize(fu * wi.x + fv * wi.y + nrm * wi.z);
+            f3 go = p + gd * kSpawnEps;
+            // Russian roulette (radiance.cu:62-68)
+            bool alive = a.bounce < a.max_bounce;
+            float rr = smax(beta.x, smax(beta.y, beta.z));
+            if (rr < 1.f && a.bounce > 3) {
+                float qq = smax(0.05f, 1.f - rr);
+                if (u01(r1.v[2]) < qq)
+                    alive = false;
+                else
+                    beta = beta / (1.f - qq);
+            }
+            if (alive) {
+                push_ray = true;
+                no = go;
+                nd = gd;
+                nb = beta;
+            }
+        }
+        uint32_t ps = wave_append(&a.counters[C_SHADOW], push_shadow);
+        if (push_shadow) {
+            a.sq.slot[ps] = slot;
+            a.sq.ox[ps] = so.x;
+            a.sq.oy[ps] = so.y;
+            a.sq.oz[ps] = so.z;
+            a.sq.dx[ps] = sd.x;
+            a.sq.dy[ps] = sd.y;
+            a.sq.dz[ps] = sd.z;
+            a.sq.tmax[ps] = stmax;
+            a.sq.cx[ps] = contrib.x;
+            a.sq.cy[ps] = contrib.y;
+            a.sq.cz[ps] = contrib.z;
+        }
+        uint32_t pr = wave_append(&a.counters[a.qni], push_ray);
+        if (push_ray) {
+            a.qn.slot[pr] = slot;
+            a.qn.ox[pr] = no.x;
+            a.qn.oy[pr] = no.y;
+            a.qn.oz[pr] = no.z;
+            a.qn.dx[pr] = nd.x;
+            a.qn.dy[pr] = nd.y;
+            a.qn.dz[pr] = nd.z;
+            a.qn.bx[pr] = nb.x;
+            a.qn.by[pr] = nb.y;
+            a.qn.bz[pr] = nb.z;
+        }
+    }
+}
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, ShadowQ sq, uint32_t *counters, float *L,
+                                                   unsigned long long *stats, uint32_t *spill, size_t spill_lanes) {
+    __shared__ uint32_t sref[kLdsDepth * kBlock];
+    __shared__ float st_[kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    uint32_t n = counters[C_SHADOW];
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        bool active = i < n;
+        if (active) {
+            f3 o{sq.ox[i], sq.oy[i], sq.oz[i]}, d{sq.dx[i], sq.dy[i], sq.dz[i]};
+            if (!trace_occluded<W, kLdsDepth>(S, o, d, sq.tmax[i], st)) {
+                uint32_t slot = sq.slot[i];
+                L[3 * slot] += sq.cx[i];
+                L[3 * slot + 1] += sq.cy[i];
+                L[3 * slot + 2] += sq.cz[i];
+            }
+        }
+        wave_count(&stats[S_SHADOW], active);
+    }
+}
+
+// Per-pixel FP64 accumulation of the chunk's finished paths; no atomics:
+// one lane owns a pixel and visits its slots j = j0, j0 + npix, ... < P.
+__global__ __launch_bounds__(kBlock) void k_accumulate(const float *L, uint64_t g0, uint32_t P, uint32_t npix,
+                                                       double *acc, uint32_t *valid, unsigned long long *stats) {
+    uint32_t off = uint32_t(g0 % npix);
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+        uint32_t j0 = (px + npix - off) % npix;
+        double sx = 0, sy = 0, sz = 0;
+        uint32_t nv = 0, bad = 0;
+        for (uint32_t j = j0; j < P; j += npix) {
+            float lx = L[3 * j], ly = L[3 * j + 1], lz = L[3 * j + 2];
+            if (isfinite(lx) && isfinite(ly) && isfinite(lz)) {
+                sx += lx;
+                sy += ly;
+                sz += lz;
+                ++nv;
+            } else {
+                ++bad;
+            }
+        }
+        if (j0 < P) {
+            acc[3 * size_t(px)] += sx;
+            acc[3 * size_t(px) + 1] += sy;
+            acc[3 * size_t(px) + 2] += sz;
+            valid[px] += nv;
+            if (bad) atomicAdd(&stats[S_INVALID], (unsigned long long)bad);
+        }
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ C ABI --
+#define KR_EXPORT extern "C" __attribute__((visibility("default")))
+
+
+KR_EXPORT const char *kr_version(void) { return "koishi-mi355x kr 1 (gfx950 wavefront)"; }
+
+KR_EXPORT int kr_last_error(char *buf, size_t len) {
+    if (!buf || len == 0) return KR_E_INVALID;
+    std::snprintf(buf, len, "%s", g_err.c_str());
+    return KR_OK;
+}
+
+KR_EXPORT int kr_device_count(int *count) {
+    if (!count) return fail(KR_E_INVALID, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *count = n;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_device_open(int ordinal, kr_device **out) {
+    if (!out) return fail(KR_E_INVALID, "out is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(KR_E_NO_DEVICE, "no HIP device visible");
+    if (ordinal < 0 || ordinal >= n) return fail(KR_E_INVALID, "device ordinal out of range");
+    KR_HIP(hipSetDevice(ordinal));
+    hipDeviceProp_t prop;
+    KR_HIP(hipGetDeviceProperties(&prop, ordinal));
+    auto *d = new kr_device();
+    d->ordinal = ordinal;
+    d->num_cu = prop.multiProcessorCount;
+    d->name = prop.gcnArchName;
+    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete d;
+        return fail(KR_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    *out = d;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_device_close(kr_device *dev) {
+    if (!dev) return KR_OK;
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipStreamSynchronize(dev->stream);
+    (void)hipStreamDestroy(dev->stream);
+    delete dev;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_scene_free(kr_scene *s) {
+    if (!s) return KR_OK;
+    (void)hipSetDevice(s->dev->ordinal);
+    (void)hipStreamSynchronize(s->dev->stream);
+    for (void *p : s->allocs) (void)hipFree(p);
+    delete s;
+    return KR_OK;
+}
+
+namespace {
+template <class T>
+int upload(kr_scene *s, const T *host, size_t n, const T **dst) {
+    T *p = nullptr;
+    int rc = dalloc(&p, n);
+    if (rc) return rc;
+    s->allocs.push_back(p);
+    if (n) KR_HIP(hipMemcpyAsync(p, host, n * sizeof(T), hipMemcpyHostToDevice, s->dev->stream));
+    *dst = p;
+    return KR_OK;
+}
+
+size_t grid_lanes(const kr_device *dev) { return size_t(std::max(dev->num_cu, 1)) * 8 * kBlock; }
+
+int ensure_spill(uint32_t *&spill, size_t &lanes, std::vector<void *> &allocs, const kr_device *dev) {
+    if (spill) return KR_OK;
+    lanes = grid_lanes(dev);
+    // 64 stack entries (ref + t) per lane beyond the LDS part
+    int rc = dalloc(&spill, lanes * 64 * 2);
+    if (rc) return rc;
+    allocs.push_back(spill);
+    return KR_OK;
+}
+}  // namespace
+
+KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene **out) {
+    if (!dev || !d || !out) return fail(KR_E_INVALID, "NULL argument");
+    *out = nullptr;
+    auto t0 = std::chrono::steady_clock::now();
+    if (d->num_prims == 0) return fail(KR_E_INVALID, "scene has no primitives");
+    if (d->num_triangles && (!d->positions || !d->normals || !d->indices))
+        return fail(KR_E_INVALID, "mesh arrays missing");
+    uint32_t width = d->bvh_width ? d->bvh_width : 4;
+    if (width != 2 && width != 4 && width != 8) return fail(KR_E_INVALID, "bvh_width must be 2, 4 or 8");
+    for (uint32_t t = 0; t < 3 * d->num_triangles; ++t)
+        if (d->indices[t] >= d->num_vertices) return fail(KR_E_INVALID, "triangle index out of range");
+    std::vector<uint32_t> tri_prim(d->num_triangles, KR_MISS);
+    std::vector<PrimRec> prec(d->num_prims);
+    std::vector<double> sph;
+    std::vector<uint32_t> sph_prim;
+    for (uint32_t i = 0; i < d->num_prims; ++i) {
+        const kr_prim_desc &p = d->prims[i];
+        if (p.shader >= d->num_shaders) return fail(KR_E_INVALID, "primitive shader index out of range");
+        prec[i] = {p.type, p.shader, p.tri_begin, 0};
+        if (p.type == KR_PRIM_MESH) {
+            if (p.tri_count == 0 || uint64_t(p.tri_begin) + p.tri_count > d->num_triangles)
+                return fail(KR_E_INVALID, "mesh triangle range out of bounds");
+            for (uint32_t t = p.tri_begin; t < p.tri_begin + p.tri_count; ++t) {
+                if (tri_prim[t] != KR_MISS) return fail(KR_E_INVALID, "triangle owned by two primitives");
+                tri_prim[t] = i;
+            }
+        } else if (p.type == KR_PRIM_SPHERE) {
+            prec[i].sphere = uint32_t(sph_prim.size());
+            sph_prim.push_back(i);
+            sph.push_back(p.center[0]);
+            sph.push_back(p.center[1]);
+            sph.push_back(p.center[2]);
+            sph.push_back(double(p.radius) * double(p.radius));
+        } else {
+            return fail(KR_E_INVALID, "unknown primitive type");
+        }
+    }
+    for (uint32_t t = 0; t < d->num_triangles; ++t)
+        if (tri_prim[t] == KR_MISS) return fail(KR_E_INVALID, "triangle not owned by any primitive");
+    std::vector<ShaderRec> shd(d->num_shaders);
+    for (uint32_t i = 0; i < d->num_shaders; ++i) {
+        const kr_shader_desc &s = d->shaders[i];
+        ShaderRec r{};
+        r.type = s.type;
+        r.dist = s.distribution;
+        r.alpha2 = s.roughness * s.roughness;
+        if (s.type == KR_SHADER_LAMBERT) {
+            for (int c = 0; c < 3; ++c) r.cr[c] = s.color[c] * s.R[c];
+        } else if (s.type == KR_SHADER_GLOSSY) {
+            for (int c = 0; c < 3; ++c) r.cr[c] = s.color[c];
+        } else if (s.type == KR_SHADER_EMISSION) {
+            for (int c = 0; c < 3; ++c) {
+                r.cr[c] = 1.f;
+                r.emission[c] = s.color[c] * s.strength;
+            }
+        } else {
+            return fail(KR_E_INVALID, "unknown shader type");
+        }
+        if (s.type == KR_SHADER_GLOSSY && s.distribution != KR_DIST_BECKMANN && s.distribution != KR_DIST_COSINE)
+            return fail(KR_E_INVALID, "unknown distribution type");
+        shd[i] = r;
+    }
+    auto *s = new kr_scene();
+    s->dev = dev;
+    s->width = width;
+    s->prims.assign(d->prims, d->prims + d->num_prims);
+    s->tri_prim = tri_prim;
+    s->sphere_prim = sph_prim;
+    // lights: every Emission primitive, in primitive order (scene.cu:200-216)
+    for (uint32_t i = 0; i < d->num_prims; ++i) {
+        const kr_prim_desc &p = d->prims[i];
+        const kr_shader_desc &sh = d->shaders[p.shader];
+        if (sh.type != KR_SHADER_EMISSION) continue;
+        LightRec lr{};
+        lr.prim = i;
+        lr.type = p.type;
+        for (int c = 0; c < 3; ++c) lr.emission[c] = sh.color[c] * sh.strength;
+        if (p.type == KR_PRIM_SPHERE) {
+            for (int c = 0; c < 3; ++c) lr.center[c] = p.center[c];
+            lr.center[3] = p.radius * p.radius;
+        } else {
+            // Mesh::generateSamples (mesh.hpp:61-85) over the face list padded to
+            // a multiple of 4 with copies of the last triangle (mesh.cu:135-143).
+            uint32_t light = uint32_t(s->lights.size());
+            lr.table = uint32_t(s->light_table.size() / 6);
+            uint32_t n = p.tri_count, F = (n + 3) / 4 * 4;
+            auto face = [&](uint32_t j, H3 &o, H3 &d1, H3 &d2, H3 &n0, H3 &n1, H3 &n2) {
+                uint32_t t = p.tri_begin + (j < n ? j : n - 1);
+                const uint32_t *ix = d->indices + 3 * size_t(t);
+                o = hload(d->positions + 3 * size_t(ix[0]));
+                d1 = hsub(hload(d->positions + 3 * size_t(ix[1])), o);
+                d2 = hsub(hload(d->positions + 3 * size_t(ix[2])), o);
+                n0 = hload(d->normals + 3 * size_t(ix[0]));
+                n1 = hload(d->normals + 3 * size_t(ix[1]));
+                n2 = hload(d->normals + 3 * size_t(ix[2]));
+            };
+            uint32_t key[2] = {uint32_t(d->light_seed), uint32_t(d->light_seed >> 32)};
+            for (uint32_t e = 0; e < kLightTable; ++e) {
+                uint32_t ctr[4] = {light, e, 0, kStreamLight}, r[4];
+                philox_host(ctr, key, r);
+                float xi = u01h(r[0]);
+                uint32_t idx = std::min(uint32_t(std::floor(xi * float(F))), F - 1);
+                float a = u01h(r[1]), b = u01h(r[2]);
+                float ux = a < b ? a : b, uy = a > b ? a : b;
+                H3 o, d1, d2, n0, n1, n2;
+                face(idx, o, d1, d2, n0, n1, n2);
+                H3 pp = hadd(hadd(o, hmul(d1, ux)), hmul(d2, uy));
+                H3 nn = hnorm(hadd(hadd(hmul(n0, 1.f - ux - uy), hmul(n1, ux)), hmul(n2, uy)));
+                float ent[6] = {pp.x, pp.y, pp.z, nn.x, nn.y, nn.z};
+                s->light_table.insert(s->light_table.end(), ent, ent + 6);
+            }
+            float area = 0.f;
+            for (uint32_t j = 0; j < F; ++j) {
+                H3 o, d1, d2, n0, n1, n2;
+                face(j, o, d1, d2, n0, n1, n2);
+                H3 c = hcross(d1, d2);
+                area = float(double(area) + double(std::sqrt(hdot(c, c))) * .5);
+            }
+            lr.pdf = 1.f / area;
+        }
+        s->lights.push_back(lr);
+    }
+    // BVH over all triangles
+    WideBvh bvh;
+    std::vector<float> tv(9 * size_t(d->num_triangles));
+    for (uint32_t t = 0; t < d->num_triangles; ++t)
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) tv[9 * size_t(t) + 3 * k + c] = d->positions[3 * size_t(d->indices[3 * t + k]) + c];
+    if (d->num_triangles) {
+        std::string err;
+        if (!build_wide_bvh(tv.data(), d->num_triangles, width, d->leaf_size, bvh, err)) {
+            delete s;
+            return fail(KR_E_INVALID, "bvh: " + err);
+        }
+    }
+    std::vector<float4> tris(3 * size_t(d->num_triangles));
+    std::vector<float> nrm(9 * size_t(d->num_triangles));
+    for (uint32_t k = 0; k < d->num_triangles; ++k) {
+        uint32_t t = bvh.tri_order[k];
+        const float *v = &tv[9 * size_t(t)];
+        float w0, w1;
+        uint32_t pb = tri_prim[t];
+        std::memcpy(&w0, &pb, 4);
+        std::memcpy(&w1, &t, 4);
+        tris[3 * size_t(k) + 0] = make_float4(v[0], v[1], v[2], w0);
+        tris[3 * size_t(k) + 1] = make_float4(v[3], v[4], v[5], w1);
+        tris[3 * size_t(k) + 2] = make_float4(v[6], v[7], v[8], 0.f);
+    }
+    for (uint32_t t = 0; t < d->num_triangles; ++t)
+        for (int k = 0; k < 3; ++k)
+            for (int c = 0; c < 3; ++c) nrm[9 * size_t(t) + 3 * k + c] = d->normals[3 * size_t(d->indices[3 * t + k]) + c];
+    (void)hipSetDevice(dev->ordinal);
+    DevScene &v = s->view;
+    int rc = KR_OK;
+    const float *nodes_f = nullptr;
+    rc = rc ? rc : upload(s, bvh.nodes.data(), bvh.nodes.size(), &nodes_f);
+    v.nodes = reinterpret_cast<const float4 *>(nodes_f);
+    rc = rc ? rc : upload(s, tris.data(), tris.size(), &v.tris);
+    rc = rc ? rc : upload(s, nrm.data(), nrm.size(), &v.tri_nrm);
+    rc = rc ? rc : upload(s, tri_prim.data(), tri_prim.size(), &v.tri_prim);
+    rc = rc ? rc : upload(s, sph.data(), sph.size(), &v.spheres);
+    rc = rc ? rc : upload(s, sph_prim.data(), sph_prim.size(), &v.sphere_prim);
+    rc = rc ? rc : upload(s, prec.data(), prec.size(), &v.prims);
+    rc = rc ? rc : upload(s, shd.data(), shd.size(), &v.shaders);
+    rc = rc ? rc : upload(s, s->lights.data(), s->lights.size(), &v.lights);
+    rc = rc ? rc : upload(s, s->light_table.data(), s->light_table.size(), &v.light_table);
+    if (rc) {
+        kr_scene_free(s);
+        return rc;
+    }
+    v.nspheres = uint32_t(sph_prim.size());
+    v.nlights = uint32_t(s->lights.size());
+    v.ntris = d->num_triangles;
+    uint32_t cap = (width - 1) * (bvh.depth + 1) + 1;
+    v.stack_cap = cap;
+    if (cap > kLdsDepth + 64) {
+        kr_scene_free(s);
+        return fail(KR_E_OVERFLOW, "BVH too deep for the traversal stack");
+    }
+    hipError_t e = hipStreamSynchronize(dev->stream);
+    if (e != hipSuccess) {
+        kr_scene_free(s);
+        return fail(KR_E_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    kr_scene_info &in = s->info;
+    in.num_prims = d->num_prims;
+    in.num_triangles = d->num_triangles;
+    in.num_spheres = v.nspheres;
+    in.num_lights = v.nlights;
+    in.bvh_width = width;
+    in.bvh_nodes = bvh.num_nodes;
+    in.bvh_leaves = bvh.num_leaves;
+    in.bvh_depth = bvh.depth;
+    in.node_bytes = node_floats(width) * 4;
+    in.tri_bytes = 48;
+    in.device_bytes = bvh.nodes.size() * 4 + tris.size() * 16 + nrm.size() * 4 + tri_prim.size() * 4 +
+                      sph.size() * 8 + s->light_table.size() * 4;
+    in.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    *out = s;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_scene_get_info(const kr_scene *s, kr_scene_info *info) {
+    if (!s || !info) return fail(KR_E_INVALID, "NULL argument");
+    *info = s->info;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_scene_light_samples(const kr_scene *s, uint32_t light, float *p_n, float *pdf) {
+    if (!s || !p_n || !pdf) return fail(KR_E_INVALID, "NULL argument");
+    if (light >= s->lights.size()) return fail(KR_E_INVALID, "light index out of range");
+    const LightRec &lr = s->lights[light];
+    if (lr.type != KR_PRIM_MESH) return fail(KR_E_INVALID, "light is not a mesh light");
+    std::memcpy(p_n, &s->light_table[6 * size_t(lr.table)], sizeof(float) * 6 * kLightTable);
+    *pdf = lr.pdf;
+    return KR_OK;
+}
+
+namespace {
+template <class F2, class F4, class F8>
+int dispatch_width(uint32_t w, F2 f2, F4 f4, F8 f8) {
+    if (w == 2) return f2();
+    if (w == 4) return f4();
+    if (w == 8) return f8();
+    return fail(KR_E_INVALID, "bad bvh width");
+}
+
+struct DevBufs {
+    std::vector<void *> ptrs;
+    ~DevBufs() {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <class T>
+    int get(T **p, size_t n) {
+        int rc = dalloc(p, n);
+        if (!rc) ptrs.push_back(*p);
+        return rc;
+    }
+};
+
+int query_grid(const kr_device *dev, uint32_t n) {
+    size_t blocks = (size_t(n) + kBlock - 1) / kBlock;
+    size_t cap = grid_lanes(dev) / kBlock;
+    return int(std::max<size_t>(1, std::min(blocks, cap)));
+}
+}  // namespace
+
+KR_EXPORT int kr_trace_closest(kr_scene *s, const kr_rays *rays, uint32_t n, kr_hits *hits, uint32_t mem) {
+    if (!s || !rays || !hits || !rays->origin || !rays->direction || !hits->t || !hits->prim || !hits->tri ||
+        !hits->u || !hits->v)
+        return fail(KR_E_INVALID, "NULL argument");
+    if (n == 0) return KR_OK;
+    (void)hipSetDevice(s->dev->ordinal);
+    hipStream_t stream = s->dev->stream;
+    int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev);
+    if (rc) return rc;
+    DevBufs b;
+    const float *o = rays->origin, *d = rays->direction;
+    float *t = hits->t, *u = hits->u, *v = hits->v;
+    uint32_t *prim = hits->prim, *tri = hits->tri;
+    if (mem == KR_MEM_HOST) {
+        float *o_, *d_;
+        if ((rc = b.get(&o_, 3 * size_t(n))) || (rc = b.get(&d_, 3 * size_t(n))) || (rc = b.get(&t, n)) ||
+            (rc = b.get(&u, n)) || (rc = b.get(&v, n)) || (rc = b.get(&prim, n)) || (rc = b.get(&tri, n)))
+            return rc;
+        KR_HIP(hipMemcpyAsync(o_, rays->origin, 12 * size_t(n), hipMemcpyHostToDevice, stream));
+        KR_HIP(hipMemcpyAsync(d_, rays->direction, 12 * size_t(n), hipMemcpyHostToDevice, stream));
+        o = o_;
+        d = d_;
+    } else if (mem != KR_MEM_DEVICE) {
+        return fail(KR_E_INVALID, "mem must be KR_MEM_HOST or KR_MEM_DEVICE");
+    }
+    int grid = query_grid(s->dev, n);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, s->view, o, d, n, t, prim, tri, u, v, s->spill,
+                           s->spill_lanes, (uint64_t *)nullptr);
+        return KR_OK;
+    };
+    rc = dispatch_width(
+        s->width, [&] { return launch(k_query_closest<2>); }, [&] { return launch(k_query_closest<4>); },
+        [&] { return launch(k_query_closest<8>); });
+    if (rc) return rc;
+    KR_HIP(hipGetLastError());
+    if (mem == KR_MEM_HOST) {
+        KR_HIP(hipMemcpyAsync(hits->t, t, 4 * size_t(n), hipMemcpyDeviceToHost, stream));
+        KR_HIP(hipMemcpyAsync(hits->u, u, 4 * size_t(n), hipMemcpyDeviceToHost, stream));
+        KR_HIP(hipMemcpyAsync(hits->v, v, 4 * size_t(n), hipMemcpyDeviceToHost, stream));
+        KR_HIP(hipMemcpyAsync(hits->prim, prim, 4 * size_t(n), hipMemcpyDeviceToHost, stream));
+        KR_HIP(hipMemcpyAsync(hits->tri, tri, 4 * size_t(n), hipMemcpyDeviceToHost, stream));
+    }
+    KR_HIP(hipStreamSynchronize(stream));
+    return KR_OK;
+}
+
+KR_EXPORT int kr_trace_shadow(kr_scene *s, const kr_rays *rays, uint32_t n, uint8_t *occluded, uint32_t mem) {
+    if (!s || !rays || !occluded || !rays->origin || !rays->direction || !rays->tmax)
+        return fail(KR_E_INVALID, "NULL argument");
+    if (n == 0) return KR_OK;
+    (void)hipSetDevice(s->dev->ordinal);
+    hipStream_t stream = s->dev->stream;
+    int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev);
+    if (rc) return rc;
+    DevBufs b;
+    const float *o = rays->origin, *d = rays->direction, *tm = rays->tmax;
+    uint8_t *occ = occluded;
+    if (mem == KR_MEM_HOST) {
+        float *o_, *d_, *t_;
+        if ((rc = b.get(&o_, 3 * size_t(n))) || (rc = b.get(&d_, 3 * size_t(n))) || (rc = b.get(&t_, n)) ||
+            (rc = b.get(&occ, n)))
+            return rc;
+        KR_HIP(hipMemcpyAsync(o_, rays->origin, 12 * size_t(n), hipMemcpyHostToDevice, stream));
+        KR_HIP(hipMemcpyAsync(d_, rays->direction, 12 * size_t(n), hipMemcpyHostToDevice, stream));
+        KR_HIP(hipMemcpyAsync(t_, rays->tmax, 4 * size_t(n), hipMemcpyHostToDevice, stream));
+        o = o_;
+        d = d_;
+        tm = t_;
+    } else if (mem != KR_MEM_DEVICE) {
+        return fail(KR_E_INVALID, "mem must be KR_MEM_HOST or KR_MEM_DEVICE");
+    }
+    int grid = query_grid(s->dev, n);
+    auto launch = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, s->view, o, d, tm, n, occ, s->spill,
+                           s->spill_lanes);
+        return KR_OK;
+    };
+    rc = dispatch_width(
+        s->width, [&] { return launch(k_query_shadow<2>); }, [&] { return launch(k_query_shadow<4>); },
+        [&] { return launch(k_query_shadow<8>); });
+    if (rc) return rc;
+    KR_HIP(hipGetLastError());
+    if (mem == KR_MEM_HOST) KR_HIP(hipMemcpyAsync(occluded, occ, n, hipMemcpyDeviceToHost, stream));
+    KR_HIP(hipStreamSynchronize(stream));
+    return KR_OK;
+}
+
+// ----------------------------------------------------------------- render --
+KR_EXPORT int kr_render_free(kr_render *r) {
+    if (!r) return KR_OK;
+    (void)hipSetDevice(r->scene->dev->ordinal);
+    (void)hipStreamSynchronize(r->scene->dev->stream);
+    for (void *p : r->allocs) (void)hipFree(p);
+    for (auto &pe : r->pending) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
+    delete r;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_params *prm, kr_render **out) {
+    if (!s || !cam || !prm || !out) return fail(KR_E_INVALID, "NULL argument");
+    *out = nullptr;
+    if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(KR_E_INVALID, "empty render");
+    if (uint64_t(prm->width) * prm->height > (1ull << 31)) return fail(KR_E_INVALID, "resolution too large");
+    if (cam->lens != KR_LENS_PINHOLE && cam->lens != KR_LENS_ORTHOGRAPHIC) return fail(KR_E_INVALID, "unknown lens");
+    (void)hipSetDevice(s->dev->ordinal);
+    auto *r = new kr_render();
+    r->scene = s;
+    r->cam = *cam;
+    r->prm = *prm;
+    r->npix = prm->width * prm->height;
+    // Lens ctor (src/core/misc/lens.hpp:113-125), FP32 as in the reference.
+    LensDev &L = r->lens;
+    L.lens = cam->lens;
+    L.w = prm->width;
+    L.h = prm->height;
+    L.pw = uint32_t(std::ceil(std::sqrt(double(prm->spp))));
+    H3 tgt = hnorm(hload(cam->target));
+    float den = float(2 * std::tan(3.14159265358979323846 * (double(cam->fovx) * .5) / 180.));
+    H3 n = hmul(tgt, float(prm->width));
+    n = {n.x / den, n.y / den, n.z / den};
+    H3 u = hnorm(hcross(n, hload(cam->upaxis)));
+    H3 v = hnorm(hcross(n, u));
+    float pw = float(L.pw);
+    H3 du{u.x / pw, u.y / pw, u.z / pw}, dv{v.x / pw, v.y / pw, v.z / pw};
+    auto st3 = [](float *dst, H3 a) {
+        dst[0] = a.x;
+        dst[1] = a.y;
+        dst[2] = a.z;
+    };
+    st3(L.n, n);
+    st3(L.u, u);
+    st3(L.v, v);
+    st3(L.du, du);
+    st3(L.dv, dv);
+    st3(L.o, hload(cam->position));
+    uint64_t total = uint64_t(r->npix) * prm->spp;
+    uint64_t P = prm->chunk_paths ? prm->chunk_paths : (4u << 20);
+    P = std::min<uint64_t>(P, total);
+    P = std::max<uint64_t>(P, 1);
+    r->P = uint32_t(P);
+    auto get = [&](auto **p, size_t cnt) {
+        int rc = dalloc(p, cnt);
+        if (!rc) r->allocs.push_back(*p);
+        return rc;
+    };
+    int rc = KR_OK;
+    for (int k = 0; k < 2 && !rc; ++k) {
+        RayQ &q = r->q[k];
+        rc = get(&q.slot, P);
+        float **fs[9] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.bx, &q.by, &q.bz};
+        for (auto f : fs)
+            if (!rc) rc = get(f, P);
+    }
+    if (!rc) rc = get(&r->hit.t, P);
+    if (!rc) rc = get(&r->hit.u, P);
+    if (!rc) rc = get(&r->hit.v, P);
+    if (!rc) rc = get(&r->hit.code, P);
+    for (int m = 0; m < 3 && !rc; ++m) rc = get(&r->bucket[m], P);
+    if (!rc) rc = get(&r->sq.slot, P);
+    {
+        float **fs[10] = {&r->sq.ox, &r->sq.oy, &r->sq.oz, &r->sq.dx, &r->sq.dy,
+                          &r->sq.dz, &r->sq.tmax, &r->sq.cx, &r->sq.cy, &r->sq.cz};
+        for (auto f : fs)
+            if (!rc) rc = get(f, P);
+    }
+    if (!rc) rc = get(&r->L, 3 * P);
+    if (!rc) rc = get(&r->counters, C_NUM);
+    if (!rc) rc = get(&r->stats, S_NUM);
+    if (!rc) rc = get(&r->acc, 3 * size_t(r->npix));
+    if (!rc) rc = get(&r->valid, r->npix);
+    if (!rc) rc = ensure_spill(r->spill, r->spill_lanes, r->allocs, s->dev);
+    if (rc) {
+        kr_render_free(r);
+        return rc;
+    }
+    r->grid = int(r->spill_lanes / kBlock);
+    rc = kr_render_reset(r);
+    if (rc) {
+        kr_render_free(r);
+        return rc;
+    }
+    *out = r;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_reset(kr_render *r) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    hipStream_t st = r->scene->dev->stream;
+    KR_HIP(hipMemsetAsync(r->acc, 0, sizeof(double) * 3 * size_t(r->npix), st));
+    KR_HIP(hipMemsetAsync(r->valid, 0, sizeof(uint32_t) * size_t(r->npix), st));
+    KR_HIP(hipMemsetAsync(r->stats, 0, sizeof(unsigned long long) * S_NUM, st));
+    KR_HIP(hipMemsetAsync(r->counters, 0, sizeof(uint32_t) * C_NUM, st));
+    KR_HIP(hipStreamSynchronize(st));
+    resolve_events(r);
+    for (int k = 0; k < K_KINDS; ++k) {
+        r->kind_ms[k] = 0;
+        r->kind_launches[k] = 0;
+    }
+    r->paths = 0;
+    return KR_OK;
+}
+
+namespace {
+// Brackets one launch with HIP events on the render's stream when
+// KR_PROFILE_EVENTS is on; the pairs are resolved at the next sync.
+struct Timed {
+    kr_render *r;
+    int kind;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timed(kr_render *r_, int k) : r(r_), kind(k) {
+        if (!(r->profile & KR_PROFILE_EVENTS)) return;
+        a = take();
+        b = take();
+        if (a && b) (void)hipEventRecord(a, r->scene->dev->stream);
+    }
+    hipEvent_t take() {
+        hipEvent_t e = nullptr;
+        if (!r->event_pool.empty()) {
+            e = r->event_pool.back();
+            r->event_pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            e = nullptr;
+        }
+        return e;
+    }
+    ~Timed() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, r->scene->dev->stream);
+        r->pending.push_back({kind, {a, b}});
+    }
+};
+
+void resolve_events(kr_render *r) {
+    for (auto &pe : r->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+            r->kind_ms[pe.first] += ms;
+            r->kind_launches[pe.first] += 1;
+        }
+        r->event_pool.push_back(pe.second.first);
+        r->event_pool.push_back(pe.second.second);
+    }
+    r->pending.clear();
+}
+
+template <int W>
+int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
+    kr_scene *s = r->scene;
+    hipStream_t st = s->dev->stream;
+    const DevScene &S = s->view;
+    int grid = r->grid;
+    int cam_grid = int(std::min<size_t>((P + kBlock - 1) / kBlock, size_t(grid)));
+    int cur = 0;
+    bool count = (r->profile & KR_PROFILE_COUNTERS) != 0;
+    {
+        Timed tm(r, K_OTHER);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, uint32_t(0xFFFFFFFFu));
+        hipLaunchKernelGGL(k_camera, dim3(cam_grid), dim3(kBlock), 0, st, r->lens, g0, P, r->npix, r->q[0], r->L,
+                           r->counters, C_RAY0);
+    }
+    int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
+    for (uint32_t b = 0; b <= r->prm.max_bounce; ++b) {
+        int qi = cur == 0 ? C_RAY0 : C_RAY1;
+        int qni = cur == 0 ? C_RAY1 : C_RAY0;
+        {
+            // clear buckets, shadow count and the next ray queue's count
+            Timed tm(r, K_OTHER);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, uint32_t(qi));
+        }
+        {
+            Timed tm(r, K_TRACE);
+            if (count)
+                hipLaunchKernelGGL((k_trace<W, true>), dim3(grid), dim3(kBlock), 0, st, S, r->q[cur], qi, r->hit,
+                                   r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort, r->spill,
+                                   r->spill_lanes);
+            else
+                hipLaunchKernelGGL((k_trace<W, false>), dim3(grid), dim3(kBlock), 0, st, S, r->q[cur], qi, r->hit,
+                                   r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort, r->spill,
+                                   r->spill_lanes);
+        }
+        ShadeArgs a;
+        a.S = S;
+        a.q = r->q[cur];
+        a.qn = r->q[1 - cur];
+        a.qi = qi;
+        a.qni = qni;
+        a.hit = r->hit;
+        a.b0 = r->bucket[0];
+        a.b1 = r->bucket[1];
+        a.b2 = r->bucket[2];
+        a.sq = r->sq;
+        a.L = r->L;
+        a.counters = r->counters;
+        a.stats = r->stats;
+        a.g0 = g0;
+        a.npix = r->npix;
+        a.bounce = b;
+        a.max_bounce = r->prm.max_bounce;
+        a.seed_lo = uint32_t(r->prm.seed);
+        a.seed_hi = uint32_t(r->prm.seed >> 32);
+        {
+            Timed tm(r, K_SHADE);
+            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, a);
+        }
+        {
+            Timed tm(r, K_SHADOW);
+            hipLaunchKernelGGL((k_shadow<W>), dim3(grid), dim3(kBlock), 0, st, S, r->sq, r->counters, r->L,
+                               r->stats, r->spill, r->spill_lanes);
+        }
+        cur = 1 - cur;
+    }
+    {
+        Timed tm(r, K_OTHER);
+        int acc_grid = int(std::min<size_t>((r->npix + kBlock - 1) / kBlock, size_t(grid)));
+        hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, r->L, g0, P, r->npix, r->acc,
+                           r->valid, r->stats);
+    }
+    KR_HIP(hipGetLastError());
+    return KR_OK;
+}
+}  // namespace
+
+KR_EXPORT int kr_render_run(kr_render *r, uint32_t spp_begin, uint32_t spp_count) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    if (spp_count == 0) return KR_OK;
+    if (uint64_t(spp_begin) + spp_count > 0xFFFFFFFFull) return fail(KR_E_INVALID, "sample range overflows");
+    if (uint64_t(spp_begin) + spp_count > r->prm.spp)
+        return fail(KR_E_INVALID, "sample range exceeds the render's spp (the lens strata depend on it)");
+    (void)hipSetDevice(r->scene->dev->ordinal);
+    auto t0 = std::chrono::steady_clock::now();
+    uint64_t g = uint64_t(spp_begin) * r->npix, gend = uint64_t(spp_begin + spp_count) * r->npix;
+    while (g < gend) {
+        uint32_t P = uint32_t(std::min<uint64_t>(r->P, gend - g));
+        int rc = dispatch_width(
+            r->scene->width, [&] { return run_chunk<2>(r, g, P); }, [&] { return run_chunk<4>(r, g, P); },
+            [&] { return run_chunk<8>(r, g, P); });
+        if (rc) return rc;
+        g += P;
+        // keep the number of outstanding event pairs bounded
+        if (r->pending.size() > 4096) {
+            KR_HIP(hipStreamSynchronize(r->scene->dev->stream));
+            resolve_events(r);
+        }
+    }
+    r->paths += gend - uint64_t(spp_begin) * r->npix;
+    r->last_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_sync(kr_render *r) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    auto t0 = std::chrono::steady_clock::now();
+    KR_HIP(hipStreamSynchronize(r->scene->dev->stream));
+    r->last_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    resolve_events(r);
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_read(kr_render *r, double *rgb, uint32_t *valid) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    hipStream_t st = r->scene->dev->stream;
+    if (rgb) KR_HIP(hipMemcpyAsync(rgb, r->acc, sizeof(double) * 3 * size_t(r->npix), hipMemcpyDeviceToHost, st));
+    if (valid) KR_HIP(hipMemcpyAsync(valid, r->valid, sizeof(uint32_t) * r->npix, hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
+    resolve_events(r);
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_get_stats(kr_render *r, kr_render_stats *out) {
+    if (!r || !out) return fail(KR_E_INVALID, "NULL argument");
+    unsigned long long h[S_NUM];
+    KR_HIP(hipMemcpyAsync(h, r->stats, sizeof(h), hipMemcpyDeviceToHost, r->scene->dev->stream));
+    KR_HIP(hipStreamSynchronize(r->scene->dev->stream));
+    resolve_events(r);
+    std::memset(out, 0, sizeof(*out));
+    out->closest_rays = h[S_CLOSEST];
+    out->shadow_rays = h[S_SHADOW];
+    out->invalid_samples = h[S_INVALID];
+    out->node_visits = h[S_NODES];
+    out->tri_tests = h[S_TRIS];
+    out->paths = r->paths;
+    out->seconds = r->last_seconds;
+    out->trace_ms = r->kind_ms[K_TRACE];
+    out->shade_ms = r->kind_ms[K_SHADE];
+    out->shadow_ms = r->kind_ms[K_SHADOW];
+    out->other_ms = r->kind_ms[K_OTHER];
+    out->trace_launches = r->kind_launches[K_TRACE];
+    out->shade_launches = r->kind_launches[K_SHADE];
+    out->shadow_launches = r->kind_launches[K_SHADOW];
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_set_profiling(kr_render *r, uint32_t flags) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    r->profile = flags;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_radiance(kr_scene *s, const kr_camera *cam, const kr_params *prm, double *rgb, uint32_t *valid) {
+    kr_render *r = nullptr;
+    int rc = kr_render_create(s, cam, prm, &r);
+    if (rc) return rc;
+    rc = kr_render_run(r, 0, prm->spp);
+    if (!rc) rc = kr_render_sync(r);
+    if (!rc) rc = kr_render_read(r, rgb, valid);
+    kr_render_free(r);
+    return rc;
+}
+

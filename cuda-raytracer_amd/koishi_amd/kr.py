@@ -1,0 +1,265 @@
+"""ctypes binding of the C ABI in include/kr_api.h (libkr.so, gfx950).
+
+The Python side is a thin caller: scenes, rays and images cross as numpy
+arrays into the library, which does all work in HIP kernels. Importing this
+module never falls back to anything else: if libkr.so is missing or has no
+usable device, the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_ROOT, "libkr.so")
+
+KR_OK = 0
+KR_MISS = 0xFFFFFFFF
+KR_MEM_HOST = 0
+KR_MEM_DEVICE = 1
+KR_RENDER_NO_MATERIAL_SORT = 0x1
+
+SHADER_LAMBERT, SHADER_GLOSSY, SHADER_EMISSION = 0, 1, 2
+DIST_BECKMANN, DIST_COSINE = 0, 1
+PRIM_MESH, PRIM_SPHERE = 0, 1
+LENS_PINHOLE, LENS_ORTHOGRAPHIC = 0, 1
+
+u32, f32, u64 = C.c_uint32, C.c_float, C.c_uint64
+
+
+class ShaderDesc(C.Structure):
+    _fields_ = [("type", u32), ("distribution", u32), ("color", f32 * 3), ("R", f32 * 3),
+                ("roughness", f32), ("strength", f32)]
+
+
+class PrimDesc(C.Structure):
+    _fields_ = [("type", u32), ("shader", u32), ("tri_begin", u32), ("tri_count", u32),
+                ("center", f32 * 3), ("radius", f32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("positions", C.POINTER(f32)), ("normals", C.POINTER(f32)), ("num_vertices", u32),
+                ("indices", C.POINTER(u32)), ("num_triangles", u32),
+                ("prims", C.POINTER(PrimDesc)), ("num_prims", u32),
+                ("shaders", C.POINTER(ShaderDesc)), ("num_shaders", u32),
+                ("light_seed", u64), ("bvh_width", u32), ("leaf_size", u32)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("num_prims", u32), ("num_triangles", u32), ("num_spheres", u32), ("num_lights", u32),
+                ("bvh_width", u32), ("bvh_nodes", u32), ("bvh_leaves", u32), ("bvh_depth", u32),
+                ("device_bytes", u64), ("node_bytes", u32), ("tri_bytes", u32), ("build_seconds", C.c_double)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("lens", u32), ("fovx", f32), ("position", f32 * 3), ("target", f32 * 3), ("upaxis", f32 * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("spp", u32), ("max_bounce", u32), ("seed", u64),
+                ("flags", u32), ("chunk_paths", u32)]
+
+
+class Rays(C.Structure):
+    _fields_ = [("origin", C.c_void_p), ("direction", C.c_void_p), ("tmax", C.c_void_p)]
+
+
+class Hits(C.Structure):
+    _fields_ = [("t", C.c_void_p), ("prim", C.c_void_p), ("tri", C.c_void_p), ("u", C.c_void_p),
+                ("v", C.c_void_p)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("paths", u64), ("closest_rays", u64), ("shadow_rays", u64), ("invalid_samples", u64),
+                ("seconds", C.c_double), ("node_visits", u64), ("tri_tests", u64), ("trace_ms", C.c_double),
+                ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("other_ms", C.c_double),
+                ("trace_launches", u64), ("shade_launches", u64), ("shadow_launches", u64)]
+
+
+KR_PROFILE_EVENTS = 0x1
+KR_PROFILE_COUNTERS = 0x2
+
+
+EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "kr_device_close",
+           "kr_scene_upload", "kr_scene_free", "kr_scene_get_info", "kr_scene_light_samples",
+           "kr_trace_closest", "kr_trace_shadow", "kr_render_create", "kr_render_run", "kr_render_sync",
+           "kr_render_reset", "kr_render_read", "kr_render_get_stats", "kr_render_free",
+           "kr_render_radiance", "kr_render_set_profiling"]
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads libkr.so (raises OSError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is not built: run __graft_entry__.build() or make -C cuda-raytracer_amd")
+    lib = C.CDLL(path)
+    vp = C.c_void_p
+    lib.kr_version.restype = C.c_char_p
+    lib.kr_last_error.argtypes = [C.c_char_p, C.c_size_t]
+    lib.kr_device_count.argtypes = [C.POINTER(C.c_int)]
+    lib.kr_device_open.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.kr_device_close.argtypes = [vp]
+    lib.kr_scene_upload.argtypes = [vp, C.POINTER(SceneDesc), C.POINTER(vp)]
+    lib.kr_scene_free.argtypes = [vp]
+    lib.kr_scene_get_info.argtypes = [vp, C.POINTER(SceneInfo)]
+    lib.kr_scene_light_samples.argtypes = [vp, u32, vp, C.POINTER(f32)]
+    lib.kr_trace_closest.argtypes = [vp, C.POINTER(Rays), u32, C.POINTER(Hits), u32]
+    lib.kr_trace_shadow.argtypes = [vp, C.POINTER(Rays), u32, vp, u32]
+    lib.kr_render_create.argtypes = [vp, C.POINTER(Camera), C.POINTER(Params), C.POINTER(vp)]
+    lib.kr_render_run.argtypes = [vp, u32, u32]
+    lib.kr_render_sync.argtypes = [vp]
+    lib.kr_render_reset.argtypes = [vp]
+    lib.kr_render_read.argtypes = [vp, vp, vp]
+    lib.kr_render_get_stats.argtypes = [vp, C.POINTER(RenderStats)]
+    lib.kr_render_free.argtypes = [vp]
+    lib.kr_render_radiance.argtypes = [vp, C.POINTER(Camera), C.POINTER(Params), vp, vp]
+    lib.kr_render_set_profiling.argtypes = [vp, u32]
+    for name in EXPORTS:
+        if name not in ("kr_version",):
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+class KrError(RuntimeError):
+    def __init__(self, code, where):
+        buf = C.create_string_buffer(1024)
+        if _lib is not None:
+            _lib.kr_last_error(buf, 1024)
+        super().__init__(f"{where} failed with status {code}: {buf.value.decode(errors='replace')}")
+        self.code = code
+
+
+def check(rc, where):
+    if rc != KR_OK:
+        raise KrError(rc, where)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Device:
+    def __init__(self, ordinal: int = 0):
+        lib = load()
+        self.lib = lib
+        h = C.c_void_p()
+        check(lib.kr_device_open(ordinal, C.byref(h)), "kr_device_open")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.kr_device_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class GpuScene:
+    """A scene uploaded through kr_scene_upload."""
+
+    def __init__(self, dev: Device, desc_holder):
+        self.dev = dev
+        self.lib = dev.lib
+        h = C.c_void_p()
+        check(self.lib.kr_scene_upload(dev.h, C.byref(desc_holder.desc), C.byref(h)), "kr_scene_upload")
+        self.h = h
+
+    def info(self) -> SceneInfo:
+        inf = SceneInfo()
+        check(self.lib.kr_scene_get_info(self.h, C.byref(inf)), "kr_scene_get_info")
+        return inf
+
+    def light_samples(self, light: int):
+        pn = np.zeros((1024, 6), np.float32)
+        pdf = f32()
+        check(self.lib.kr_scene_light_samples(self.h, light, _ptr(pn), C.byref(pdf)), "kr_scene_light_samples")
+        return pn, pdf.value
+
+    def trace_closest(self, origin, direction):
+        o = np.ascontiguousarray(origin, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        n = o.shape[0]
+        t = np.empty(n, np.float32)
+        prim = np.empty(n, np.uint32)
+        tri = np.empty(n, np.uint32)
+        u = np.empty(n, np.float32)
+        v = np.empty(n, np.float32)
+        rays = Rays(_ptr(o), _ptr(d), None)
+        hits = Hits(_ptr(t), _ptr(prim), _ptr(tri), _ptr(u), _ptr(v))
+        check(self.lib.kr_trace_closest(self.h, C.byref(rays), n, C.byref(hits), KR_MEM_HOST), "kr_trace_closest")
+        return t, prim, tri, u, v
+
+    def trace_shadow(self, origin, direction, tmax):
+        o = np.ascontiguousarray(origin, np.float32)
+        d = np.ascontiguousarray(direction, np.float32)
+        tm = np.ascontiguousarray(tmax, np.float32)
+        n = o.shape[0]
+        occ = np.empty(n, np.uint8)
+        rays = Rays(_ptr(o), _ptr(d), _ptr(tm))
+        check(self.lib.kr_trace_shadow(self.h, C.byref(rays), n, _ptr(occ), KR_MEM_HOST), "kr_trace_shadow")
+        return occ
+
+    def free(self):
+        if self.h:
+            self.lib.kr_scene_free(self.h)
+            self.h = None
+
+
+class Render:
+    """kr_render handle: device FP64 accumulators for one camera/params."""
+
+    def __init__(self, scene: GpuScene, camera: Camera, params: Params):
+        self.scene = scene
+        self.lib = scene.lib
+        self.params = params
+        h = C.c_void_p()
+        check(self.lib.kr_render_create(scene.h, C.byref(camera), C.byref(params), C.byref(h)), "kr_render_create")
+        self.h = h
+        self.npix = params.width * params.height
+
+    def run(self, spp_begin, spp_count):
+        check(self.lib.kr_render_run(self.h, spp_begin, spp_count), "kr_render_run")
+
+    def sync(self):
+        check(self.lib.kr_render_sync(self.h), "kr_render_sync")
+
+    def reset(self):
+        check(self.lib.kr_render_reset(self.h), "kr_render_reset")
+
+    def read(self):
+        rgb = np.zeros((self.npix, 3), np.float64)
+        valid = np.zeros(self.npix, np.uint32)
+        check(self.lib.kr_render_read(self.h, _ptr(rgb), _ptr(valid)), "kr_render_read")
+        return rgb, valid
+
+    def set_profiling(self, flags: int):
+        check(self.lib.kr_render_set_profiling(self.h, flags), "kr_render_set_profiling")
+
+    def stats(self) -> RenderStats:
+        s = RenderStats()
+        check(self.lib.kr_render_get_stats(self.h, C.byref(s)), "kr_render_get_stats")
+        return s
+
+    def free(self):
+        if self.h:
+            self.lib.kr_render_free(self.h)
+            self.h = None
+
+
+def device_count() -> int:
+    lib = load()
+    n = C.c_int(0)
+    check(lib.kr_device_count(C.byref(n)), "kr_device_count")
+    return n.value

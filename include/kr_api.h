@@ -1,0 +1,254 @@
+/*
+ * kr_api.h -- C ABI of the MI355X (gfx950) Koishi Radiance path tracer.
+ *
+ * This is the drop-in boundary for the reference's hot path. The reference
+ * moves a polymorphic scene to the GPU with poly::object/poly::vector deep
+ * copies (src/poly/object.hpp, src/poly/vector.hpp) and launches a megakernel
+ * through poly::kernel (src/poly/kernel.hpp:329-333) from
+ * CudaSingleGPUTracer::execute (src/core/tracer/cudaSingleGPU.hpp:52-93).
+ * Here the scene crosses as flat arrays, the integrator is a wavefront of
+ * hand-written HIP kernels, and every call returns an int status.
+ *
+ * Conventions
+ *  - Every function returns KR_OK (0) or a KR_E_* code. No C++ exception
+ *    crosses this boundary. kr_last_error() returns the message of the last
+ *    failure on the calling thread (replaces KTHROW, src/util/debug.hpp:68-75).
+ *  - Host arrays passed in are copied before the call returns; the caller
+ *    keeps ownership. Output arrays are caller-allocated.
+ *  - A kr_device owns one HIP stream. Calls on one device (and on the scenes
+ *    and renders made from it) must be serialised by the caller; separate
+ *    devices are independent.
+ *  - Coordinates follow the reference: Z-up camera basis by default
+ *    (upaxis (0,0,1), src/util/config.hpp:32).
+ */
+#ifndef KR_API_H
+#define KR_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KR_API_VERSION 1
+
+enum kr_status {
+    KR_OK = 0,
+    KR_E_INVALID = 1,     /* bad argument or malformed scene            */
+    KR_E_HIP = 2,         /* HIP runtime failure                         */
+    KR_E_OOM = 3,         /* device or host allocation failed            */
+    KR_E_OVERFLOW = 4,    /* a queue or stack bound was exceeded         */
+    KR_E_UNSUPPORTED = 5, /* feature not provided by this build          */
+    KR_E_NO_DEVICE = 6    /* no HIP device / extension not usable        */
+};
+
+typedef struct kr_device kr_device;
+typedef struct kr_scene kr_scene;
+typedef struct kr_render kr_render;
+
+/* ---- scene description (flat, host memory) ---------------------------- */
+
+/* Shader type tags. Replace Factory<Shader> names
+ * (src/ext/shader/shader.impl.cu:15-24): "Lambert", "GlossyBSDF",
+ * "Emission". */
+enum kr_shader_type {
+    KR_SHADER_LAMBERT = 0,
+    KR_SHADER_GLOSSY = 1,
+    KR_SHADER_EMISSION = 2
+};
+
+/* Distribution tags for GlossyBSDF's "distribution" entry. Replace
+ * Factory<SphericalDistribution> (src/ext/distribution/distribution.impl.cu:8-11). */
+enum kr_distribution_type {
+    KR_DIST_BECKMANN = 0,
+    KR_DIST_COSINE = 1
+};
+
+typedef struct kr_shader_desc {
+    uint32_t type;         /* kr_shader_type                                   */
+    uint32_t distribution; /* kr_distribution_type (GlossyBSDF only)           */
+    float color[3];        /* Lambert/GlossyBSDF/Emission "color", default 1   */
+    float R[3];            /* Lambert "R", default 1                            */
+    float roughness;       /* Beckmann "roughness", default 0.5 (alpha2 = r^2) */
+    float strength;        /* Emission "strength", default 2                    */
+} kr_shader_desc;
+
+enum kr_prim_type {
+    KR_PRIM_MESH = 0,
+    KR_PRIM_SPHERE = 1
+};
+
+/* One reference Primitive (src/core/primitive/primitive.hpp:10-37): a Mesh
+ * (one per imported object/material group) or a Sphere. Order matters: it is
+ * the order of Scene::primitives, which fixes the order of area lights. */
+typedef struct kr_prim_desc {
+    uint32_t type;      /* kr_prim_type                                     */
+    uint32_t shader;    /* index into kr_scene_desc.shaders                 */
+    uint32_t tri_begin; /* mesh: first triangle in kr_scene_desc.indices/3  */
+    uint32_t tri_count; /* mesh: number of triangles                        */
+    float center[3];    /* sphere: centre "o"                               */
+    float radius;       /* sphere: "r"                                      */
+} kr_prim_desc;
+
+typedef struct kr_scene_desc {
+    const float *positions;   /* num_vertices * 3                              */
+    const float *normals;     /* num_vertices * 3 (per-vertex shading normals) */
+    uint32_t num_vertices;
+    const uint32_t *indices;  /* num_triangles * 3, into positions/normals     */
+    uint32_t num_triangles;
+    const kr_prim_desc *prims;
+    uint32_t num_prims;
+    const kr_shader_desc *shaders;
+    uint32_t num_shaders;
+    uint64_t light_seed;      /* seeds the 1024-entry area-light sample tables */
+    uint32_t bvh_width;       /* 2, 4 or 8; 0 = library default (4)            */
+    uint32_t leaf_size;       /* max triangles per leaf; 0 = default (4)       */
+} kr_scene_desc;
+
+typedef struct kr_scene_info {
+    uint32_t num_prims, num_triangles, num_spheres, num_lights;
+    uint32_t bvh_width, bvh_nodes, bvh_leaves, bvh_depth;
+    uint64_t device_bytes;     /* geometry + BVH bytes resident in HBM         */
+    uint32_t node_bytes;       /* bytes per BVH node                           */
+    uint32_t tri_bytes;        /* bytes per triangle read by traversal         */
+    double build_seconds;
+} kr_scene_info;
+
+/* ---- camera and render parameters ------------------------------------- */
+
+enum kr_lens_type { KR_LENS_PINHOLE = 0, KR_LENS_ORTHOGRAPHIC = 1 };
+
+/* Mirrors CameraConfig (src/util/config.hpp:12-35). "target" is a
+ * direction, not a point (src/core/misc/lens.hpp:119). */
+typedef struct kr_camera {
+    uint32_t lens;
+    float fovx;           /* degrees */
+    float position[3];
+    float target[3];
+    float upaxis[3];
+} kr_camera;
+
+typedef struct kr_params {
+    uint32_t width, height;
+    uint32_t spp;        /* total samples per pixel; fixes the lens strata
+                            pw = ceil(sqrt(spp)) (src/core/misc/lens.hpp:117) */
+    uint32_t max_bounce; /* RadianceKernel "maxBounce" (src/core/kernel/radiance.hpp:14) */
+    uint64_t seed;       /* counter-based RNG key                               */
+    uint32_t flags;      /* KR_RENDER_* bits                                    */
+    uint32_t chunk_paths;/* paths in flight per wavefront chunk; 0 = default   */
+} kr_params;
+
+#define KR_RENDER_NO_MATERIAL_SORT 0x1u /* shade with one mixed-material queue */
+
+/* ---- ray query buffers -------------------------------------------------- */
+
+#define KR_MEM_HOST 0u
+#define KR_MEM_DEVICE 1u /* all pointers of the call are HIP device pointers */
+
+typedef struct kr_rays {
+    const float *origin;    /* n * 3 */
+    const float *direction; /* n * 3, unit length */
+    const float *tmax;      /* n; segment length for shadow queries, else NULL */
+} kr_rays;
+
+#define KR_MISS 0xFFFFFFFFu
+
+typedef struct kr_hits {
+    float *t;       /* INFINITY on miss                                   */
+    uint32_t *prim; /* primitive index, KR_MISS on miss                   */
+    uint32_t *tri;  /* triangle index within the primitive (input order)  */
+    float *u, *v;   /* barycentric weights of vertex 1 and vertex 2       */
+} kr_hits;
+
+typedef struct kr_render_stats {
+    uint64_t paths;          /* camera paths traced                        */
+    uint64_t closest_rays;   /* closest-hit queries                        */
+    uint64_t shadow_rays;    /* shadow (segment) queries                   */
+    uint64_t invalid_samples;/* samples rejected as NaN/Inf                */
+    double seconds;          /* wall time of the last kr_render_run + sync */
+    /* filled when KR_PROFILE_COUNTERS is on: closest-hit traversal work */
+    uint64_t node_visits;    /* wide-BVH nodes loaded                      */
+    uint64_t tri_tests;      /* ray/triangle tests                         */
+    /* filled when KR_PROFILE_EVENTS is on: HIP-event time on the render's
+       stream, summed over launches since the last reset                   */
+    double trace_ms, shade_ms, shadow_ms, other_ms;
+    uint64_t trace_launches, shade_launches, shadow_launches;
+} kr_render_stats;
+
+#define KR_PROFILE_EVENTS 0x1u   /* time every kernel with HIP events        */
+#define KR_PROFILE_COUNTERS 0x2u /* count node visits / triangle tests      */
+
+/* ---- library and device ------------------------------------------------- */
+
+/* Library version string. */
+const char *kr_version(void);
+
+/* Copies the calling thread's last error message into buf. */
+int kr_last_error(char *buf, size_t len);
+
+/* Number of visible HIP devices. */
+int kr_device_count(int *count);
+
+/* Opens device `ordinal` and creates its stream. */
+int kr_device_open(int ordinal, kr_device **out);
+int kr_device_close(kr_device *dev);
+
+/* ---- scene -------------------------------------------------------------
+ * Replaces the transfer of core::Scene (src/core/meta/scene.hpp:15-71,
+ * built by src/core/meta/scene.cu:18-219) through poly::vector/object.
+ * Builds the wide BVH on the host, derives the area lights (every primitive
+ * whose shader is Emission, in primitive order, src/core/meta/scene.cu:200-216),
+ * builds each light's 1024-entry sample table
+ * (src/core/primitive/mesh.hpp:61-98) and uploads everything to HBM. */
+int kr_scene_upload(kr_device *dev, const kr_scene_desc *desc, kr_scene **out);
+int kr_scene_free(kr_scene *scene);
+int kr_scene_get_info(const kr_scene *scene, kr_scene_info *info);
+
+/* Copies light `light`'s sample table: p_n receives 1024 * 6 floats
+ * (position xyz, normal xyz), pdf the area pdf (1 / total area). */
+int kr_scene_light_samples(const kr_scene *scene, uint32_t light, float *p_n, float *pdf);
+
+/* ---- ray queries --------------------------------------------------------
+ * kr_trace_closest replaces Scene::intersect(Ray) (src/core/meta/scene.hpp:35-55)
+ * over Mesh::intersect(Ray) (src/core/primitive/mesh.cu:200-267) and
+ * Sphere::intersect (src/core/primitive/sphere.hpp:71-94).
+ * kr_trace_shadow replaces Scene::intersect(Seg) (src/core/meta/scene.hpp:57-67);
+ * occluded[i] = 1 when segment i (origin, direction, tmax) is blocked. */
+int kr_trace_closest(kr_scene *scene, const kr_rays *rays, uint32_t n, kr_hits *hits,
+                     uint32_t mem);
+int kr_trace_shadow(kr_scene *scene, const kr_rays *rays, uint32_t n, uint8_t *occluded,
+                    uint32_t mem);
+
+/* ---- Radiance integrator -------------------------------------------------
+ * Replaces Tracer::execute (src/core/tracer/tracer.hpp:20-21) running
+ * RadianceKernel::execute (src/core/kernel/radiance.cu:14-81) per sample and
+ * the per-pixel averaging of CPUMulticoreTracer
+ * (src/core/tracer/cpuMulticore.cu:47-64).
+ * A render accumulates in FP64 on the device: rgb[3*(y*w+x)+c] is the sum
+ * of the valid (finite) samples of pixel (x, y), valid[y*w+x] their count;
+ * the reference's pixel value is rgb / valid. */
+int kr_render_create(kr_scene *scene, const kr_camera *camera, const kr_params *params,
+                     kr_render **out);
+/* Traces samples [spp_begin, spp_begin + spp_count) of every pixel and adds
+ * them to the accumulators. Asynchronous on the device stream. */
+int kr_render_run(kr_render *render, uint32_t spp_begin, uint32_t spp_count);
+int kr_render_sync(kr_render *render);
+/* Clears the accumulators. */
+int kr_render_reset(kr_render *render);
+/* Copies the accumulators to host memory (either pointer may be NULL). */
+int kr_render_read(kr_render *render, double *rgb, uint32_t *valid);
+int kr_render_get_stats(kr_render *render, kr_render_stats *stats);
+/* KR_PROFILE_* bits; takes effect from the next kr_render_run. */
+int kr_render_set_profiling(kr_render *render, uint32_t flags);
+int kr_render_free(kr_render *render);
+
+/* One-shot render of all params->spp samples into caller buffers. */
+int kr_render_radiance(kr_scene *scene, const kr_camera *camera, const kr_params *params,
+                       double *rgb, uint32_t *valid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KR_API_H */
