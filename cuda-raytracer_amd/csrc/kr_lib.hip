@@ -751,7 +751,7 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     s->tri_prim = tri_prim;
     s->sphere_prim = sph_prim;
     // lights: every Emission primitive, in primitive order (scene.cu:200-216)
-    for (uint32_t i = 0; i < d->num_prims; ++i) {
+    for (uint32_t i = 0; i < d->num_prims && !(d->flags & KR_SCENE_NO_AREA_LIGHTS); ++i) {
         const kr_prim_desc &p = d->prims[i];
         const kr_shader_desc &sh = d->shaders[p.shader];
         if (sh.type != KR_SHADER_EMISSION) continue;

@@ -1,0 +1,159 @@
+// renderer.cpp -- Renderer::render and the tracer registry over the C ABI.
+#include "renderer.h"
+
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <stdexcept>
+
+#include "log.h"
+#include "png.h"
+
+namespace koishi {
+namespace core {
+
+namespace {
+
+void kr_check(int rc, const char *what) {
+    if (rc == KR_OK) return;
+    char buf[1024] = {0};
+    kr_last_error(buf, sizeof buf);
+    throw std::logic_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + buf);
+}
+
+// RAII over the kr_* handles
+struct DeviceScene {
+    kr_device *dev = nullptr;
+    kr_scene *scene = nullptr;
+    kr_render *render = nullptr;
+    ~DeviceScene() {
+        if (render) kr_render_free(render);
+        if (scene) kr_scene_free(scene);
+        if (dev) kr_device_close(dev);
+    }
+};
+
+// The wavefront integrator behind Tracer::execute: uploads the flat scene,
+// renders every pixel's samples in FP64 accumulators on the device and reads
+// back sums and valid counts (src/core/tracer/cpuMulticore.cu:47-64 semantics).
+class HIPWavefrontTracer : public Tracer {
+public:
+    explicit HIPWavefrontTracer(const Json &props) {
+        if (const Json *k = props.find("kernel")) kernel_ = (*k)[0].str();
+    }
+    void execute(Image &image, const kr_camera &camera, const LoadedScene &scene, uint32_t spp,
+                 const RenderOptions &opt) override {
+        const std::string kernel = opt.kernel.empty() ? scene.kernel : opt.kernel;
+        if (kernel != "Radiance") {
+            if (kernel == "Normal" || kernel == "Visibility" || kernel == "HalfVector")
+                throw std::logic_error("kernel " + kernel + " is not provided by this build (Radiance only)");
+            throw std::logic_error("no such type: " + kernel);
+        }
+        DeviceScene ds;
+        kr_check(kr_device_open(opt.device, &ds.dev), "kr_device_open");
+        kr_scene_desc d = scene.desc(opt.bvh_width);
+        kr_check(kr_scene_upload(ds.dev, &d, &ds.scene), "kr_scene_upload");
+        kr_scene_info info{};
+        kr_scene_get_info(ds.scene, &info);
+        KLOG("Scene on device:", info.num_triangles, "triangles,", info.num_spheres, "spheres,", info.num_lights,
+             "lights, BVH" + std::to_string(info.bvh_width), info.bvh_nodes, "nodes");
+        KLOG(info.num_lights, "lights in scene");
+        kr_params p{};
+        p.width = image.w;
+        p.height = image.h;
+        p.spp = spp;
+        p.max_bounce = scene.max_bounce;
+        p.seed = opt.seed;
+        kr_check(kr_render_create(ds.scene, &camera, &p, &ds.render), "kr_render_create");
+        KINFO("tracer", "Tracing start");
+        auto t0 = std::chrono::steady_clock::now();
+        kr_check(kr_render_run(ds.render, 0, spp), "kr_render_run");
+        kr_check(kr_render_sync(ds.render), "kr_render_sync");
+        double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        KINFO("tracer", "Tracer joint in", sec, "seconds");
+        image.rgb.assign(3 * size_t(image.w) * image.h, 0.0);
+        image.valid.assign(size_t(image.w) * image.h, 0);
+        kr_check(kr_render_read(ds.render, image.rgb.data(), image.valid.data()), "kr_render_read");
+        kr_render_stats st{};
+        kr_render_get_stats(ds.render, &st);
+        KLOG(double(st.paths) / sec, "paths/s,", double(st.closest_rays + st.shadow_rays) / sec * 1e-6, "Mrays/s,",
+             st.invalid_samples, "samples rejected (NaN/Inf)");
+    }
+
+private:
+    std::string kernel_ = "Radiance";
+};
+
+std::map<std::string, TracerCtor> &registry() {
+    static std::map<std::string, TracerCtor> r = {
+        {"HIPWavefront", [](const Json &p) { return std::unique_ptr<Tracer>(new HIPWavefrontTracer(p)); }},
+        {"CPUMulticore", [](const Json &p) { return std::unique_ptr<Tracer>(new HIPWavefrontTracer(p)); }},
+    };
+    return r;
+}
+
+}  // namespace
+
+std::unique_ptr<Tracer> create_tracer(const std::string &name, const Json &props) {
+    auto &r = registry();
+    auto it = r.find(name);
+    if (it == r.end()) throw std::logic_error("no such type: " + name);
+    if (name == "CPUMulticore") KLOG("Tracer CPUMulticore is served by the HIPWavefront integrator");
+    return it->second(props);
+}
+
+std::vector<std::string> tracer_names() {
+    std::vector<std::string> v;
+    for (auto &kv : registry()) v.push_back(kv.first);
+    return v;
+}
+
+std::vector<std::string> kernel_names() { return {"Radiance"}; }
+
+void Image::dump_png(const std::string &path) const {
+    std::vector<uint8_t> bytes(3 * size_t(w) * h);
+    for (size_t i = 0; i < size_t(w) * h; ++i)
+        for (int c = 0; c < 3; ++c) bytes[3 * i + c] = to_byte(rgb[3 * i + c] / double(valid[i]));
+    write_png(path, w, h, bytes.data());
+}
+
+// Raw dump: "KRRAW1\0\0", uint32 w, uint32 h, w*h*3 float64 pixel means
+// (sum / valid), w*h uint32 valid counts; little endian.
+void Image::dump_raw(const std::string &path) const {
+    std::ofstream f(path, std::ios::binary);
+    if (!f) throw std::logic_error("cannot write '" + path + "'");
+    const char magic[8] = {'K', 'R', 'R', 'A', 'W', '1', 0, 0};
+    f.write(magic, 8);
+    f.write(reinterpret_cast<const char *>(&w), 4);
+    f.write(reinterpret_cast<const char *>(&h), 4);
+    std::vector<double> mean(rgb.size());
+    for (size_t i = 0; i < valid.size(); ++i)
+        for (int c = 0; c < 3; ++c) mean[3 * i + c] = rgb[3 * i + c] / double(valid[i]);
+    f.write(reinterpret_cast<const char *>(mean.data()), std::streamsize(mean.size() * 8));
+    f.write(reinterpret_cast<const char *>(valid.data()), std::streamsize(valid.size() * 4));
+}
+
+void Renderer::render(const std::string &path, const std::string &dest, uint32_t spp, const RenderOptions &opt) {
+    KINFO("renderer", "Render start");
+    KINFO("renderer", "Loading config '" + path + "'");
+    LoadedScene scene = load_scene_file(path);
+    if (scene.cameras.empty()) throw std::logic_error("no camera in the scene");
+    const kr_camera &camera = scene.cameras[0];
+    Image image;
+    image.w = w_;
+    image.h = h_;
+    KLOG("Target resolution:", w_, "x", h_, ",", spp, "spp");
+    std::string tname = opt.tracer.empty() ? scene.tracer : opt.tracer;
+    auto tracer = create_tracer(tname, scene.tracer_props);
+    KLOG("Start intergrating");
+    tracer->execute(image, camera, scene, spp, opt);
+    KLOG("Finished intergrating");
+    KINFO("renderer", "Writting image to file");
+    image.dump_png(dest);
+    if (!opt.dump_raw.empty()) image.dump_raw(opt.dump_raw);
+    KINFO("renderer", "Written to '" + dest + "'");
+    KINFO("renderer", "Render finished successfully");
+}
+
+}  // namespace core
+}  // namespace koishi

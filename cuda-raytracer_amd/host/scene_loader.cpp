@@ -1,0 +1,376 @@
+// scene_loader.cpp -- reference scene JSON + OBJ import -> kr_scene_desc arrays.
+#include "scene_loader.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "log.h"
+
+namespace koishi {
+
+namespace {
+
+std::string read_file(const std::string &path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw std::logic_error("cannot open '" + path + "'");
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    return ss.str();
+}
+
+std::string dir_of(const std::string &path) {
+    size_t k = path.find_last_of('/');
+    return k == std::string::npos ? std::string(".") : path.substr(0, k);
+}
+
+void f3(const Json &j, float out[3], const char *what) {
+    if (!j.is_array() || j.size() != 3) throw std::logic_error(std::string(what) + ": expected [x, y, z]");
+    for (int k = 0; k < 3; ++k) out[k] = float(j[k].number());
+}
+
+// get(props, key, default) of src/util/jsel.hpp:264-279
+float getf(const Json &props, const char *key, float dflt) {
+    const Json *j = props.find(key);
+    return j ? float(j->number()) : dflt;
+}
+void get3(const Json &props, const char *key, const float dflt[3], float out[3]) {
+    const Json *j = props.find(key);
+    if (j)
+        f3(*j, out, key);
+    else
+        for (int k = 0; k < 3; ++k) out[k] = dflt[k];
+}
+
+// OBJ index: 1-based, negative = relative to the end
+long resolve(long i, size_t n) {
+    if (i > 0) return i - 1;
+    if (i < 0) return long(n) + i;
+    return -1;
+}
+
+struct Key {
+    long v, n;
+    bool operator==(const Key &o) const { return v == o.v && n == o.n; }
+};
+struct KeyHash {
+    size_t operator()(const Key &k) const { return std::hash<long>()(k.v * 1000003L ^ k.n); }
+};
+
+}  // namespace
+
+std::vector<ObjMesh> load_obj(const std::string &path, std::vector<std::string> &material_names) {
+    std::ifstream f(path);
+    if (!f) throw std::logic_error("cannot open '" + path + "'");
+    std::vector<float> V, N;
+    std::vector<ObjMesh> meshes;
+    std::vector<std::unordered_map<Key, uint32_t, KeyHash>> remap;
+    std::vector<std::vector<char>> needs_normal;  // per mesh: vertex lacks vn
+    std::string object = "", material = "DefaultMaterial";
+    bool used_default = false;
+    std::vector<std::string> used;  // usemtl names in first-use order
+    int cur = -1;
+    auto open_mesh = [&]() {
+        meshes.push_back(ObjMesh{object, material, {}, {}, {}});
+        remap.emplace_back();
+        needs_normal.emplace_back();
+        cur = int(meshes.size()) - 1;
+    };
+    std::string line;
+    size_t lineno = 0;
+    while (std::getline(f, line)) {
+        ++lineno;
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        std::istringstream ls(line);
+        std::string tag;
+        if (!(ls >> tag) || tag[0] == '#') continue;
+        if (tag == "v") {
+            float x, y, z;
+            if (!(ls >> x >> y >> z)) throw std::logic_error(path + ":" + std::to_string(lineno) + ": bad vertex");
+            V.insert(V.end(), {x, y, z});
+        } else if (tag == "vn") {
+            float x, y, z;
+            if (!(ls >> x >> y >> z)) throw std::logic_error(path + ":" + std::to_string(lineno) + ": bad normal");
+            N.insert(N.end(), {x, y, z});
+        } else if (tag == "o" || tag == "g") {
+            std::string name;
+            std::getline(ls >> std::ws, name);
+            object = name;
+            cur = -1;
+        } else if (tag == "usemtl") {
+            std::string name;
+            ls >> name;
+            // assimp creates a named material on first mention, faces or not
+            if (name != "DefaultMaterial" && std::find(used.begin(), used.end(), name) == used.end())
+                used.push_back(name);
+            if (name != material) {
+                material = name;
+                cur = -1;
+            }
+        } else if (tag == "f") {
+            if (cur < 0 || meshes[cur].material != material || meshes[cur].object != object) open_mesh();
+            if (material == "DefaultMaterial") used_default = true;
+            std::vector<uint32_t> poly;
+            std::string tok;
+            while (ls >> tok) {
+                long vi = 0, ni = 0;
+                size_t s1 = tok.find('/');
+                vi = std::strtol(tok.substr(0, s1).c_str(), nullptr, 10);
+                if (s1 != std::string::npos) {
+                    size_t s2 = tok.find('/', s1 + 1);
+                    if (s2 != std::string::npos && s2 + 1 < tok.size())
+                        ni = std::strtol(tok.substr(s2 + 1).c_str(), nullptr, 10);
+                }
+                long v = resolve(vi, V.size() / 3), n = resolve(ni, N.size() / 3);
+                if (v < 0 || size_t(v) >= V.size() / 3)
+                    throw std::logic_error(path + ":" + std::to_string(lineno) + ": vertex index out of range");
+                if (n >= 0 && size_t(n) >= N.size() / 3)
+                    throw std::logic_error(path + ":" + std::to_string(lineno) + ": normal index out of range");
+                Key k{v, n};
+                ObjMesh &m = meshes[cur];
+                auto it = remap[cur].find(k);
+                uint32_t id;
+                if (it == remap[cur].end()) {
+                    id = uint32_t(m.positions.size() / 3);
+                    m.positions.insert(m.positions.end(), {V[3 * v], V[3 * v + 1], V[3 * v + 2]});
+                    if (n >= 0)
+                        m.normals.insert(m.normals.end(), {N[3 * n], N[3 * n + 1], N[3 * n + 2]});
+                    else
+                        m.normals.insert(m.normals.end(), {0.f, 0.f, 0.f});
+                    needs_normal[cur].push_back(n < 0);
+                    remap[cur].emplace(k, id);
+                } else {
+                    id = it->second;
+                }
+                poly.push_back(id);
+            }
+            if (poly.size() < 3) throw std::logic_error(path + ":" + std::to_string(lineno) + ": face with < 3 vertices");
+            for (size_t k = 1; k + 1 < poly.size(); ++k)
+                meshes[cur].indices.insert(meshes[cur].indices.end(), {poly[0], poly[k], poly[k + 1]});
+        }
+        // vt, s, mtllib, l, p: not used by the Radiance path
+    }
+    // smooth normals where the file has none: area-weighted face normals
+    for (size_t mi = 0; mi < meshes.size(); ++mi) {
+        ObjMesh &m = meshes[mi];
+        bool any = false;
+        for (char c : needs_normal[mi]) any |= c != 0;
+        if (!any) continue;
+        std::vector<double> acc(m.positions.size(), 0.0);
+        for (size_t t = 0; t < m.indices.size(); t += 3) {
+            const float *a = &m.positions[3 * m.indices[t]], *b = &m.positions[3 * m.indices[t + 1]],
+                        *c = &m.positions[3 * m.indices[t + 2]];
+            double e1[3] = {double(b[0]) - a[0], double(b[1]) - a[1], double(b[2]) - a[2]};
+            double e2[3] = {double(c[0]) - a[0], double(c[1]) - a[1], double(c[2]) - a[2]};
+            double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+            for (int k = 0; k < 3; ++k)
+                for (int c3 = 0; c3 < 3; ++c3) acc[3 * m.indices[t + k] + c3] += n[c3];
+        }
+        for (size_t v = 0; v < needs_normal[mi].size(); ++v) {
+            if (!needs_normal[mi][v]) continue;
+            double l = std::sqrt(acc[3 * v] * acc[3 * v] + acc[3 * v + 1] * acc[3 * v + 1] + acc[3 * v + 2] * acc[3 * v + 2]);
+            for (int c3 = 0; c3 < 3; ++c3) m.normals[3 * v + c3] = l > 0 ? float(acc[3 * v + c3] / l) : 0.f;
+        }
+    }
+    material_names.clear();
+    if (used_default) material_names.push_back("DefaultMaterial");
+    for (auto &u : used) material_names.push_back(u);
+    std::vector<ObjMesh> out;
+    for (auto &m : meshes)
+        if (!m.indices.empty()) out.push_back(std::move(m));
+    return out;
+}
+
+kr_shader_desc shader_from_config(const Json &cfg) {
+    if (!cfg.is_array() || cfg.size() < 1 || !cfg[0].is_string())
+        throw std::logic_error("shader config must be [type, {props}]");
+    const std::string &type = cfg[0].str();
+    Json empty = Json::object();
+    const Json &props = cfg.size() > 1 ? cfg[1] : empty;
+    kr_shader_desc d{};
+    const float one[3] = {1, 1, 1};
+    get3(props, "color", one, d.color);
+    get3(props, "R", one, d.R);
+    d.roughness = 0.5f;
+    d.strength = getf(props, "strength", 2.f);
+    if (type == "Lambert") {
+        d.type = KR_SHADER_LAMBERT;
+    } else if (type == "GlossyBSDF") {
+        d.type = KR_SHADER_GLOSSY;
+        const Json &dist = props.at("distribution");  // required (glossy.hpp:62-63)
+        std::string dn;
+        const Json *dp = nullptr;
+        if (dist.is_string()) {
+            dn = dist.str();
+        } else {
+            if (!dist.is_array() || dist.size() < 1) throw std::logic_error("distribution must be [type, {props}]");
+            dn = dist[0].str();
+            if (dist.size() > 1) dp = &dist[1];
+        }
+        if (dn == "Beckmann") {
+            d.distribution = KR_DIST_BECKMANN;
+            d.roughness = dp ? getf(*dp, "roughness", .5f) : .5f;
+        } else if (dn == "Cosine") {
+            d.distribution = KR_DIST_COSINE;
+        } else {
+            throw std::logic_error("no such type: " + dn);
+        }
+    } else if (type == "Emission") {
+        d.type = KR_SHADER_EMISSION;
+    } else if (type == "Blackhole" || type == "RefractionBSDF" || type == "Mixed" || type == "Transparent" ||
+               type == "GlassBSDF") {
+        throw std::logic_error("shader type " + type + " is not provided by this build (Radiance path: Lambert, "
+                               "GlossyBSDF, Emission)");
+    } else {
+        throw std::logic_error("no such type: " + type);
+    }
+    return d;
+}
+
+namespace {
+kr_camera camera_from_json(const Json &c) {
+    kr_camera cam{};
+    cam.lens = KR_LENS_PINHOLE;
+    if (const Json *l = c.find("lens")) {
+        const std::string &s = l->str();
+        if (s == "pinhole")
+            cam.lens = KR_LENS_PINHOLE;
+        else if (s == "orthographic")
+            cam.lens = KR_LENS_ORTHOGRAPHIC;
+        else
+            throw std::logic_error("unknown lens type: " + s);
+    }
+    cam.fovx = getf(c, "fovx", 90.f);
+    const float pos[3] = {1, 0, 0}, tgt[3] = {-1, 0, 0}, up[3] = {0, 0, 1};
+    get3(c, "position", pos, cam.position);
+    get3(c, "target", tgt, cam.target);
+    get3(c, "upaxis", up, cam.upaxis);
+    return cam;
+}
+}  // namespace
+
+LoadedScene load_scene_json(const Json &doc, const std::string &base_dir) {
+    LoadedScene out;
+    const Json &scene = doc.at("scene");
+    // tracer: Config [name, props], default ["CPUMulticore", {}] (renderer.hpp:20-21)
+    if (const Json *t = doc.find("tracer")) {
+        out.tracer = (*t)[0].str();
+        if (t->size() > 1) out.tracer_props = (*t)[1];
+    }
+    // kernel: tracer props "kernel", default ["Radiance", {}] (tracer.hpp:14-17)
+    if (const Json *k = out.tracer_props.find("kernel")) {
+        out.kernel = (*k)[0].str();
+        if (k->size() > 1) out.max_bounce = uint32_t(getf((*k)[1], "maxBounce", 16.f));
+    }
+    std::vector<std::string> names;
+    bool imported = false;
+    if (const Json *assets = scene.find("assets")) {
+        for (const Json &a : assets->items()) {
+            const std::string &name = a[0].str();
+            Json empty = Json::object();
+            const Json &props = a.size() > 1 ? a[1] : empty;
+            if (name == "import") {
+                if (imported) throw std::logic_error("importing multiple scenes is not allowed");
+                imported = true;
+                std::string path = props.at("path").str();
+                if (!path.empty() && path[0] != '/') {
+                    std::ifstream probe(path);
+                    if (!probe) path = base_dir + "/" + path;
+                }
+                std::vector<std::string> mats;
+                auto meshes = load_obj(path, mats);
+                // assimp's material list comes first; spheres append (scene.cu:52,141-154)
+                names = mats;
+                for (auto &m : meshes) {
+                    uint32_t base_v = uint32_t(out.positions.size() / 3);
+                    kr_prim_desc p{};
+                    p.type = KR_PRIM_MESH;
+                    p.shader = uint32_t(std::find(names.begin(), names.end(), m.material) - names.begin());
+                    p.tri_begin = uint32_t(out.indices.size() / 3);
+                    p.tri_count = uint32_t(m.indices.size() / 3);
+                    out.positions.insert(out.positions.end(), m.positions.begin(), m.positions.end());
+                    out.normals.insert(out.normals.end(), m.normals.begin(), m.normals.end());
+                    for (uint32_t i : m.indices) out.indices.push_back(base_v + i);
+                    out.prims.push_back(p);
+                }
+                KINFO("scene", "Imported", meshes.size(), "meshes");
+            } else if (name == "sphere") {
+                kr_prim_desc p{};
+                p.type = KR_PRIM_SPHERE;
+                f3(props.at("o"), p.center, "o");
+                p.radius = float(props.at("r").number());
+                const std::string &sn = props.at("shader").str();
+                auto it = std::find(names.begin(), names.end(), sn);
+                if (it == names.end()) {
+                    names.push_back(sn);
+                    it = names.end() - 1;
+                }
+                p.shader = uint32_t(it - names.begin());
+                out.prims.push_back(p);
+            } else if (name.find("light") != std::string::npos || name.find("Light") != std::string::npos) {
+                if (name == "pointLight")
+                    throw std::logic_error("pointLight is not provided by this build (area lights only)");
+                throw std::logic_error("unknown light type: " + name);
+            } else {
+                throw std::logic_error("unknown asset type: " + name);
+            }
+        }
+    }
+    if (const Json *cams = scene.find("camera"))
+        for (const Json &c : cams->items()) out.cameras.push_back(camera_from_json(c));
+    // shaders by name (scene.cu:180-198); a missing config disables light
+    // promotion (scene.cu:193-200)
+    out.shader_names = names;
+    out.shaders.resize(names.size());
+    out.shader_defined.assign(names.size(), false);
+    const Json *table = scene.find("shaders");
+    for (size_t i = 0; i < names.size(); ++i) {
+        KLOG("Looking for shader:", names[i]);
+        const Json *cfg = table ? table->find(names[i]) : nullptr;
+        if (cfg) {
+            out.shaders[i] = shader_from_config(*cfg);
+            out.shader_defined[i] = true;
+            KLOG(" =", i);
+        } else {
+            KLOG("Configuration for shader {", names[i], "} not found.");
+            out.lights_enabled = false;
+            out.shaders[i] = kr_shader_desc{};
+            out.shaders[i].type = KR_SHADER_LAMBERT;
+        }
+    }
+    for (auto &p : out.prims)
+        if (!out.shader_defined[p.shader])
+            throw std::logic_error("primitive uses shader {" + names[p.shader] + "}, which has no configuration");
+    return out;
+}
+
+LoadedScene load_scene_file(const std::string &path) {
+    Json doc = Json::parse(read_file(path));
+    return load_scene_json(doc, dir_of(path));
+}
+
+kr_scene_desc LoadedScene::desc(uint32_t bvh_width, uint32_t leaf_size) const {
+    kr_scene_desc d{};
+    d.positions = positions.data();
+    d.normals = normals.data();
+    d.num_vertices = uint32_t(positions.size() / 3);
+    d.indices = indices.data();
+    d.num_triangles = uint32_t(indices.size() / 3);
+    d.prims = prims.data();
+    d.num_prims = uint32_t(prims.size());
+    d.shaders = shaders.data();
+    d.num_shaders = uint32_t(shaders.size());
+    d.light_seed = light_seed;
+    d.bvh_width = bvh_width;
+    d.leaf_size = leaf_size;
+    d.flags = lights_enabled ? 0u : KR_SCENE_NO_AREA_LIGHTS;
+    return d;
+}
+
+}  // namespace koishi

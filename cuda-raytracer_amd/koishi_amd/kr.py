@@ -45,7 +45,10 @@ class SceneDesc(C.Structure):
                 ("indices", C.POINTER(u32)), ("num_triangles", u32),
                 ("prims", C.POINTER(PrimDesc)), ("num_prims", u32),
                 ("shaders", C.POINTER(ShaderDesc)), ("num_shaders", u32),
-                ("light_seed", u64), ("bvh_width", u32), ("leaf_size", u32)]
+                ("light_seed", u64), ("bvh_width", u32), ("leaf_size", u32), ("flags", u32)]
+
+
+KR_SCENE_NO_AREA_LIGHTS = 0x1
 
 
 class SceneInfo(C.Structure):

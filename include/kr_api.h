@@ -104,7 +104,12 @@ typedef struct kr_scene_desc {
     uint64_t light_seed;      /* seeds the 1024-entry area-light sample tables */
     uint32_t bvh_width;       /* 2, 4 or 8; 0 = library default (4)            */
     uint32_t leaf_size;       /* max triangles per leaf; 0 = default (4)       */
+    uint32_t flags;           /* KR_SCENE_* bits                               */
 } kr_scene_desc;
+
+/* No area lights are derived: the reference skips light promotion when any
+ * shader name has no configuration (src/core/meta/scene.cu:193-216). */
+#define KR_SCENE_NO_AREA_LIGHTS 0x1u
 
 typedef struct kr_scene_info {
     uint32_t num_prims, num_triangles, num_spheres, num_lights;

@@ -441,7 +441,7 @@ template <class R> struct Scene {
         // Light promotion (src/core/meta/scene.cu:200-216): every primitive with an
         // Emission shader becomes an AreaLight with E = color * strength
         // (src/ext/shader/emission.hpp:107-114), in primitive order.
-        for (uint32_t i = 0; i < prims.size(); ++i) {
+        for (uint32_t i = 0; i < prims.size() && !(d.flags & KR_SCENE_NO_AREA_LIGHTS); ++i) {
             const kr_shader_desc &sh = shaders[prims[i].shader];
             if (sh.type != KR_SHADER_EMISSION) continue;
             V3<R> e = mk<R>(sh.color) * R(sh.strength);
