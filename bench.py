@@ -44,26 +44,49 @@ def cpu_baseline(scene, target_seconds):
     p = scene.params(seed=1)
     w, h = p.width, p.height
     nthreads = max(1, (os.cpu_count() or 2) - 1)
-    # probe on 8 rows, then size the sample to ~target_seconds
+    # probe on 8 rows at 1 spp, then size the sample (rows, then samples per
+    # pixel of the frame's own strata) to ~target_seconds
     rows = np.arange(0, h, max(1, h // 8))[:8]
     pix = (rows[:, None] * w + np.arange(w)[None, :]).ravel()
     _, _, st = orc.render(p, 0, 1, nthreads, pix)
     rate = pix.size / max(st["seconds"], 1e-6)
-    nrows = int(min(h, max(8, target_seconds * rate / w)))
+    want = target_seconds * rate
+    nrows = int(min(h, max(8, want / w)))
+    spp = int(min(p.spp, max(1, want / (nrows * w))))
     rows = np.linspace(0, h - 1, nrows).astype(np.int64)
     pix = (rows[:, None] * w + np.arange(w)[None, :]).ravel()
-    _, _, st = orc.render(p, 0, 1, nthreads, pix)
+    _, _, st = orc.render(p, 0, spp, nthreads, pix)
     orc.free()
+    paths = pix.size * spp
     return {
-        "value": pix.size / st["seconds"],
+        "value": paths / st["seconds"],
         "unit": "paths/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": f"{nrows} of {h} rows x {w} px x 1 spp ({pix.size} paths, {st['seconds']:.1f} s) of the "
+        "sample": f"{nrows} of {h} rows x {w} px x {spp} spp ({paths} paths, {st['seconds']:.1f} s) of the "
                   f"same frame; FP32 restatement of CPUMulticoreTracer + RadianceKernel, reference BVH and "
                   f"traversal, -O3 -ffast-math -fno-finite-math-only -funroll-loops; "
                   f"{os.cpu_count()} logical CPUs, {nthreads} tracer threads",
     }
+
+
+def rmse_vs_oracle(workload, rgb, valid, width):
+    """RMSE of the rendered frame against the committed converged FP64 oracle
+    image (tests/golden/oracle_<workload>.npz: a fixed pixel subset, same lens
+    strata, 65536 samples per pixel), on linear radiance, unclamped and
+    clamped to [0, 1] like the reference's 8-bit output."""
+    import numpy as np
+    path = os.path.join(ROOT, "tests", "golden", f"oracle_{workload}.npz")
+    if not os.path.exists(path):
+        return None
+    f = np.load(path)
+    pix = f["pixels"].astype(np.int64)
+    ref = f["mean"]
+    mine = rgb[pix] / np.maximum(valid[pix], 1)[:, None]
+    d = mine - ref
+    dc = np.clip(mine, 0, 1) - np.clip(ref, 0, 1)
+    return {"unclamped": float(np.sqrt((d * d).mean())), "clamped": float(np.sqrt((dc * dc).mean())),
+            "pixels": int(pix.size), "oracle_samples_per_pixel": int(f["seeds"]) * int(f["spp"])}
 
 
 def main():
@@ -140,6 +163,8 @@ def main():
     trace_bytes = st.closest_rays * bytes_per_query
     achieved = trace_bytes / (st.trace_ms * 1e-3) / 1e9 if st.trace_ms > 0 else 0.0
 
+    rmse = rmse_vs_oracle(args.workload, rgb, valid, p.width)
+
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_trace_mesh_traffic.json")
     if os.path.exists(tpath):
@@ -175,6 +200,8 @@ def main():
             "width": p.width, "height": p.height, "spp": p.spp, "max_bounce": p.max_bounce,
             "triangles": info.num_triangles, "bvh_width": info.bvh_width, "paths_per_step": paths_per_step,
             "mrays_per_s": (st.closest_rays + st.shadow_rays) / elapsed / 1e6,
+            "rmse_vs_fp64_oracle": rmse,
+            "wall_clock_to_rmse_s": elapsed / args.steps,
             "kernel_ms_per_step": {"trace": st.trace_ms / args.steps, "shade": st.shade_ms / args.steps,
                                    "shadow": st.shadow_ms / args.steps, "other": st.other_ms / args.steps},
         },

@@ -27,6 +27,7 @@
 #include "kr_api.h"
 #include "kr_device.h"
 #include "kr_layout.h"
+#include "kr_nrc.h"
 
 using namespace kr;
 
@@ -265,6 +266,12 @@ __global__ __launch_bounds__(kBlock) void k_query_shadow(DevScene S, const float
         f3 rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
         occ[i] = trace_occluded<W, kLdsDepth>(S, ro, rd, tmax[i], st) ? 1 : 0;
     }
+}
+
+// Plain HBM copy (the bandwidth / counter calibration probe).
+__global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
 }
 
 __global__ void k_reset(uint32_t *counters, uint32_t keep_ray_slot) {
@@ -597,6 +604,254 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const float *L, uint64_t 
 
 }  // namespace
 
+// ------------------------------------------------------- radiance cache --
+namespace {
+constexpr uint32_t kStreamTrain = 0x5452414Eu;  // training-path selection
+constexpr int kRec = 28;  // vertex record: raw 16 | beta 3 | c 3 | beta*E 3 | pad 3
+
+// Sample k of a training batch: explicit arrays (pool_total == nullptr:
+// index k) or the render's ring buffer (window of the most recent samples).
+struct TrainSource {
+    const float *raw;      // kRaw per sample
+    const float *target;   // 4 per sample
+    const uint32_t *pool_total;
+    uint32_t n, cap, step, steps;
+};
+
+__device__ __forceinline__ void train_window(const TrainSource &src, uint32_t batch, uint32_t &start,
+                                             uint32_t &count, uint32_t &mod) {
+    if (!src.pool_total) {
+        start = 0;
+        count = src.n;
+        mod = src.n ? src.n : 1;
+        return;
+    }
+    uint32_t total = *src.pool_total;
+    if (total == 0) {
+        start = 0;
+        count = 0;
+        mod = 1;
+        return;
+    }
+    if (total <= src.cap) {
+        // ring not yet wrapped: samples live in [0, total); cycle through them
+        mod = total;
+        count = batch < total ? batch : total;
+        start = uint32_t(uint64_t(src.step) * count % total);
+    } else {
+        // most recent steps * count samples, oldest window first
+        mod = src.cap;
+        count = batch < src.cap ? batch : src.cap;
+        uint64_t back = uint64_t(src.steps - src.step) * count % src.cap;
+        start = uint32_t((uint64_t(total) % src.cap + src.cap - back) % src.cap);
+    }
+}
+
+// Forward + backward of one wave (64 samples) of a training batch; writes
+// the wave's partial gradient row and adds its loss to *loss.
+__global__ __launch_bounds__(64) void k_nrc_fwd_bwd(nrc::Weights W, TrainSource src, uint32_t batch, float *partial,
+                                                    float *loss) {
+    using namespace nrc;
+    __shared__ v4s Hs[24 + 5 * 16][64];  // input fragments of every layer
+    __shared__ v4f Ds[16][64];           // dZ fragments of the current layer
+    int lane = threadIdx.x, s = lane & 15, q = lane >> 4;
+    uint32_t start, count, mod;
+    train_window(src, batch, start, count, mod);
+    uint32_t base = blockIdx.x * 64;
+    auto index_of = [&](int n) -> int64_t {
+        uint32_t k = base + uint32_t(n);
+        if (k >= count) return -1;
+        return int64_t((uint64_t(start) + k) % mod);
+    };
+    v4s in[6][4];
+    encode_inputs(in, [&](int n) -> const float * {
+        int64_t i = index_of(n);
+        return i < 0 ? nullptr : src.raw + size_t(i) * kRaw;
+    }, W);
+#pragma unroll
+    for (int kt = 0; kt < 6; ++kt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) Hs[kt * 4 + nt][lane] = in[kt][nt];
+    // forward, keeping each layer's input fragments in LDS
+    v4s h[4][4], g[4][4];
+    hidden_layer<6, 4>(in, h, W.wt + bw_off(0), W.master + b_off(0));
+#pragma unroll
+    for (int l = 1; l <= 4; ++l) {
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) Hs[24 + (l - 1) * 16 + kt * 4 + nt][lane] = h[kt][nt];
+        hidden_layer<4, 4>(h, g, W.wt + bw_off(l), W.master + b_off(l));
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) h[kt][nt] = g[kt][nt];
+    }
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) Hs[24 + 4 * 16 + kt * 4 + nt][lane] = h[kt][nt];
+    v4f zo[4];
+    output_layer(h, zo, W.wt + bw_off(5), W.master + b_off(5));
+    // relative L2 loss: (y - t)^2 / (y^2 + 0.01), mean over samples and RGB
+    float inv = count ? 1.f / (3.f * float(count)) : 0.f;
+    float lsum = 0.f;
+    v4f dz[4][4];
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        v4f d = {0.f, 0.f, 0.f, 0.f};
+        int64_t i = index_of(nt * 16 + s);
+        if (q == 0 && i >= 0) {
+            const float *t = src.target + size_t(i) * 4;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float y = zo[nt][c];
+                float den = y * y + 0.01f;
+                float diff = y - t[c];
+                lsum += diff * diff / den;
+                d[c] = 2.f * diff / den * inv;
+            }
+        }
+        dz[0][nt] = d;
+    }
+    // wave reduction of the loss
+    for (int off = 32; off > 0; off >>= 1) lsum += __shfl_xor(lsum, off);
+    if (lane == 0 && loss) atomicAdd(loss, lsum * inv);
+    float *grow = partial + size_t(blockIdx.x) * kNumParams;
+    // backward, layer 5 .. 0
+#pragma unroll
+    for (int l = kLayers - 1; l >= 0; --l) {
+        const int IN = kDims[l], OUT = kDims[l + 1];
+        const int MT = OUT / 16, KT = IN / 16;
+        const int hbase = l == 0 ? 0 : 24 + (l - 1) * 16;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                if (mt < MT) Ds[mt * 4 + nt][lane] = dz[mt][nt];
+        __syncthreads();
+        // dW[i][o] = sum_n H[n][i] dZ[n][o]; db[o] = sum_n dZ[n][o]
+        for (int e = lane; e < IN * OUT; e += 64) {
+            int i = e / OUT, o = e % OUT;
+            float acc = 0.f;
+            int hkt = i >> 4, hq = (i & 15) >> 2, hr = i & 3;
+            int dmt = o >> 4, dq = (o & 15) >> 2, dr = o & 3;
+            for (int n = 0; n < 64; ++n) {
+                int nt = n >> 4, sl = n & 15;
+                float hv = from_bf16(uint16_t(Hs[hbase + hkt * 4 + nt][sl + 16 * hq][hr]));
+                float dv = Ds[dmt * 4 + nt][sl + 16 * dq][dr];
+                acc += hv * dv;
+            }
+            grow[w_off(l) + e] = acc;
+        }
+        if (lane < OUT) {
+            int dmt = lane >> 4, dq = (lane & 15) >> 2, dr = lane & 3;
+            float acc = 0.f;
+            for (int n = 0; n < 64; ++n) acc += Ds[dmt * 4 + (n >> 4)][(n & 15) + 16 * dq][dr];
+            grow[b_off(l) + lane] = acc;
+        }
+        __syncthreads();
+        if (l == 0) break;
+        // dH^T (in x samples) = W (in x out) dZ^T; dZ_{l-1} = dH * (H_l > 0)
+        v4s dzb[4][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt)
+                if (mt < MT) dzb[mt][nt] = pack4(dz[mt][nt][0], dz[mt][nt][1], dz[mt][nt][2], dz[mt][nt][3]);
+        const uint16_t *w = W.w + bw_off(l);
+#pragma unroll
+        for (int mi = 0; mi < 4; ++mi) {  // IN = 64 for l >= 1
+            v4s a[4];
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt)
+                if (kt < MT) a[kt] = *reinterpret_cast<const v4s *>(w + (mi * 16 + s) * OUT + kt * 16 + 4 * q);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt)
+                    if (kt < MT) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], dzb[kt][nt], acc, 0, 0, 0);
+                v4s hv = Hs[hbase + mi * 4 + nt][lane];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    if (!(from_bf16(uint16_t(hv[r])) > 0.f)) acc[r] = 0.f;
+                dz[mi][nt] = acc;
+            }
+        }
+        (void)KT;
+    }
+}
+
+// Sums the partial rows, takes one Adam step on the FP32 masters and
+// refreshes the BF16 copies (W^T for the forward, W for the backward).
+__global__ void k_nrc_adam(float *master, uint16_t *wt, uint16_t *w, float *m, float *v, const float *partial,
+                           uint32_t rows, float lr, float b1, float b2, float eps, float c1, float c2,
+                           const uint32_t *pool_total) {
+    using namespace nrc;
+    if (pool_total && *pool_total == 0) return;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kNumParams; p += gridDim.x * blockDim.x) {
+        float g = 0.f;
+        for (uint32_t r = 0; r < rows; ++r) g += partial[size_t(r) * kNumParams + p];
+        float mm = b1 * m[p] + (1.f - b1) * g;
+        float vv = b2 * v[p] + (1.f - b2) * g * g;
+        m[p] = mm;
+        v[p] = vv;
+        float val = master[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+        master[p] = val;
+        for (int l = 0; l < kLayers; ++l) {
+            int in = kDims[l], out = kDims[l + 1];
+            if (p >= w_off(l) && p < b_off(l)) {
+                int e = p - w_off(l), i = e / out, o = e % out;
+                uint16_t hb = to_bf16(val);
+                w[bw_off(l) + i * out + o] = hb;
+                wt[bw_off(l) + o * in + i] = hb;
+            }
+        }
+    }
+}
+
+__global__ void k_nrc_sum_rows(const float *partial, uint32_t rows, float *out) {
+    using namespace nrc;
+    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kNumParams; p += gridDim.x * blockDim.x) {
+        float g = 0.f;
+        for (uint32_t r = 0; r < rows; ++r) g += partial[size_t(r) * kNumParams + p];
+        out[p] = g;
+    }
+}
+
+__global__ void k_nrc_encode(nrc::Weights W, const float *raw, uint32_t n, float *features) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        nrc::RawRec r = nrc::load_raw(raw + size_t(i) * nrc::kRaw);
+        for (int f = 0; f < nrc::kIn; ++f) features[size_t(i) * nrc::kIn + f] = nrc::feature(f, r, W);
+    }
+}
+
+// Prediction for explicit records; one wave per 64 records.
+__global__ __launch_bounds__(256) void k_nrc_predict(nrc::Weights W, const float *raw, uint32_t n, float *rgb) {
+    using namespace nrc;
+    int lane = __lane_id(), s = lane & 15, q = lane >> 4;
+    uint32_t waves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t base = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * 64; base < n; base += waves * 64) {
+        v4s in[6][4];
+        encode_inputs(in, [&](int k) -> const float * {
+            return base + k < n ? raw + size_t(base + k) * kRaw : nullptr;
+        }, W);
+        v4f out[4];
+        forward(in, out, W);
+        if (q == 0)
+            for (int nt = 0; nt < 4; ++nt) {
+                uint32_t i = base + nt * 16 + s;
+                if (i < n) {
+                    rgb[3 * size_t(i)] = out[nt][0];
+                    rgb[3 * size_t(i) + 1] = out[nt][1];
+                    rgb[3 * size_t(i) + 2] = out[nt][2];
+                }
+            }
+    }
+}
+}  // namespace
+
 // ------------------------------------------------------------ C ABI --
 #define KR_EXPORT extern "C" __attribute__((visibility("default")))
 
@@ -645,6 +900,40 @@ KR_EXPORT int kr_device_close(kr_device *dev) {
     (void)hipStreamSynchronize(dev->stream);
     (void)hipStreamDestroy(dev->stream);
     delete dev;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_device_bandwidth(kr_device *dev, size_t bytes, uint32_t iters, double *gbps) {
+    if (!dev || !gbps || bytes < 16 || iters == 0) return fail(KR_E_INVALID, "bad argument");
+    (void)hipSetDevice(dev->ordinal);
+    size_t n = bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    int rc = dalloc(&a, n);
+    if (rc) return rc;
+    rc = dalloc(&b, n);
+    if (rc) {
+        (void)hipFree(a);
+        return rc;
+    }
+    hipStream_t st = dev->stream;
+    int grid = int(std::min<size_t>((n + kBlock - 1) / kBlock, size_t(std::max(dev->num_cu, 1)) * 32));
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemsetAsync(a, 0, n * 16, st);
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, st, a, b, n);  // warm-up
+    (void)hipEventRecord(e0, st);
+    for (uint32_t i = 0; i < iters; ++i) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(kBlock), 0, st, a, b, n);
+    (void)hipEventRecord(e1, st);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return fail(KR_E_HIP, std::string("k_copy: ") + hipGetErrorString(e));
+    *gbps = 2.0 * double(n * 16) * iters / (double(ms) * 1e-3) / 1e9;
     return KR_OK;
 }
 
@@ -1070,7 +1359,10 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
     st3(L.dv, dv);
     st3(L.o, hload(cam->position));
     uint64_t total = uint64_t(r->npix) * prm->spp;
-    uint64_t P = prm->chunk_paths ? prm->chunk_paths : (4u << 20);
+    // Default chunk: up to 32M paths in flight (~5.5 GB of queues), so that
+    // each bounce's kernels see tens of millions of rays and their tails and
+    // launch gaps are amortised.
+    uint64_t P = prm->chunk_paths ? prm->chunk_paths : (32u << 20);
     P = std::min<uint64_t>(P, total);
     P = std::max<uint64_t>(P, 1);
     r->P = uint32_t(P);
@@ -1343,3 +1635,289 @@ KR_EXPORT int kr_render_radiance(kr_scene *s, const kr_camera *cam, const kr_par
     return rc;
 }
 
+
+// ------------------------------------------------------- radiance cache --
+struct kr_nrc {
+    kr_device *dev = nullptr;
+    kr_nrc_config cfg{};
+    float *master = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    uint16_t *wt = nullptr, *w = nullptr;
+    float *partial = nullptr;  // [batch / 64][kNumParams]
+    float *loss = nullptr;     // device scalar
+    uint32_t rows = 0;         // batch / 64
+    uint32_t step = 0;         // Adam steps taken
+    uint64_t samples = 0;
+    float last_loss = 0.f;
+    // training pool filled by renders (ring buffer)
+    float *pool_raw = nullptr, *pool_tgt = nullptr;
+    uint32_t *pool_total = nullptr;
+    uint32_t pool_cap = 0;
+    std::vector<void *> allocs;
+};
+
+namespace {
+nrc::Weights nrc_view(const kr_nrc *n) {
+    nrc::Weights W;
+    W.wt = n->wt;
+    W.w = n->w;
+    W.master = n->master;
+    for (int c = 0; c < 3; ++c) {
+        W.box_min[c] = n->cfg.box_min[c];
+        float ext = n->cfg.box_max[c] - n->cfg.box_min[c];
+        W.box_scale[c] = ext > 0.f ? 1.f / ext : 1.f;
+    }
+    return W;
+}
+
+// BF16 copies of the master weights, made on the host (set_params / create).
+void nrc_host_bf16(const float *master, std::vector<uint16_t> &w, std::vector<uint16_t> &wt) {
+    using namespace nrc;
+    w.assign(kNumWeights, 0);
+    wt.assign(kNumWeights, 0);
+    for (int l = 0; l < kLayers; ++l) {
+        int in = kDims[l], out = kDims[l + 1];
+        for (int i = 0; i < in; ++i)
+            for (int o = 0; o < out; ++o) {
+                float v = master[w_off(l) + i * out + o];
+                uint32_t u;
+                std::memcpy(&u, &v, 4);
+                u += 0x7FFFu + ((u >> 16) & 1u);
+                uint16_t h = uint16_t(u >> 16);
+                w[bw_off(l) + i * out + o] = h;
+                wt[bw_off(l) + o * in + i] = h;
+            }
+    }
+}
+
+int nrc_upload_params(kr_nrc *n, const float *host) {
+    std::vector<uint16_t> w, wt;
+    nrc_host_bf16(host, w, wt);
+    hipStream_t st = n->dev->stream;
+    KR_HIP(hipMemcpyAsync(n->master, host, sizeof(float) * nrc::kNumParams, hipMemcpyHostToDevice, st));
+    KR_HIP(hipMemcpyAsync(n->w, w.data(), 2 * w.size(), hipMemcpyHostToDevice, st));
+    KR_HIP(hipMemcpyAsync(n->wt, wt.data(), 2 * wt.size(), hipMemcpyHostToDevice, st));
+    KR_HIP(hipMemsetAsync(n->adam_m, 0, sizeof(float) * nrc::kNumParams, st));
+    KR_HIP(hipMemsetAsync(n->adam_v, 0, sizeof(float) * nrc::kNumParams, st));
+    KR_HIP(hipStreamSynchronize(st));
+    n->step = 0;
+    return KR_OK;
+}
+
+// One training step on a source; the caller syncs.
+int nrc_step(kr_nrc *n, const TrainSource &src, uint32_t batch, uint32_t rows) {
+    hipStream_t st = n->dev->stream;
+    KR_HIP(hipMemsetAsync(n->loss, 0, sizeof(float), st));
+    hipLaunchKernelGGL(k_nrc_fwd_bwd, dim3(rows), dim3(64), 0, st, nrc_view(n), src, batch, n->partial, n->loss);
+    n->step += 1;
+    const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
+    float c1 = float(1.0 - std::pow(double(b1), double(n->step)));
+    float c2 = float(1.0 - std::pow(double(b2), double(n->step)));
+    float lr = n->cfg.learning_rate > 0.f ? n->cfg.learning_rate : 1e-2f;
+    hipLaunchKernelGGL(k_nrc_adam, dim3(96), dim3(256), 0, st, n->master, n->wt, n->w, n->adam_m, n->adam_v,
+                       n->partial, rows, lr, b1, b2, eps, c1, c2, src.pool_total);
+    KR_HIP(hipGetLastError());
+    return KR_OK;
+}
+
+template <class T>
+int nrc_alloc(kr_nrc *n, T **p, size_t cnt) {
+    int rc = dalloc(p, cnt);
+    if (!rc) n->allocs.push_back(*p);
+    return rc;
+}
+}  // namespace
+
+KR_EXPORT int kr_nrc_free(kr_nrc *n) {
+    if (!n) return KR_OK;
+    (void)hipSetDevice(n->dev->ordinal);
+    (void)hipStreamSynchronize(n->dev->stream);
+    for (void *p : n->allocs) (void)hipFree(p);
+    delete n;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_nrc_create(kr_device *dev, const kr_nrc_config *cfg, kr_nrc **out) {
+    using namespace nrc;
+    if (!dev || !cfg || !out) return fail(KR_E_INVALID, "NULL argument");
+    *out = nullptr;
+    (void)hipSetDevice(dev->ordinal);
+    auto *n = new kr_nrc();
+    n->dev = dev;
+    n->cfg = *cfg;
+    if (!n->cfg.batch) n->cfg.batch = 16384;
+    if (!n->cfg.steps_per_frame) n->cfg.steps_per_frame = 4;
+    if (!n->cfg.train_log2) n->cfg.train_log2 = 4;
+    if (n->cfg.rough_threshold <= 0.f) n->cfg.rough_threshold = 0.3f;
+    n->cfg.batch = (n->cfg.batch + 63) / 64 * 64;
+    n->rows = n->cfg.batch / 64;
+    n->pool_cap = 1u << 18;
+    int rc = KR_OK;
+    if (!rc) rc = nrc_alloc(n, &n->master, kNumParams);
+    if (!rc) rc = nrc_alloc(n, &n->adam_m, kNumParams);
+    if (!rc) rc = nrc_alloc(n, &n->adam_v, kNumParams);
+    if (!rc) rc = nrc_alloc(n, &n->wt, kNumWeights);
+    if (!rc) rc = nrc_alloc(n, &n->w, kNumWeights);
+    if (!rc) rc = nrc_alloc(n, &n->partial, size_t(n->rows) * kNumParams);
+    if (!rc) rc = nrc_alloc(n, &n->loss, 1);
+    if (!rc) rc = nrc_alloc(n, &n->pool_raw, size_t(n->pool_cap) * kRaw);
+    if (!rc) rc = nrc_alloc(n, &n->pool_tgt, size_t(n->pool_cap) * 4);
+    if (!rc) rc = nrc_alloc(n, &n->pool_total, 1);
+    if (rc) {
+        kr_nrc_free(n);
+        return rc;
+    }
+    // Glorot-uniform weights from a SplitMix64 stream, zero biases
+    std::vector<float> host(kNumParams, 0.f);
+    uint64_t state = 0x9E3779B97F4A7C15ull * (uint64_t(cfg->seed) + 1);
+    auto next = [&]() {
+        state += 0x9E3779B97F4A7C15ull;
+        uint64_t z = state;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return double(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < kLayers; ++l) {
+        double lim = std::sqrt(6.0 / (kDims[l] + kDims[l + 1]));
+        for (int e = 0; e < kDims[l] * kDims[l + 1]; ++e) host[w_off(l) + e] = float((2 * next() - 1) * lim);
+    }
+    rc = nrc_upload_params(n, host.data());
+    if (!rc) {
+        hipError_t e = hipMemsetAsync(n->pool_total, 0, sizeof(uint32_t), dev->stream);
+        if (e != hipSuccess) rc = fail(KR_E_HIP, "hipMemsetAsync");
+    }
+    if (rc) {
+        kr_nrc_free(n);
+        return rc;
+    }
+    *out = n;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_nrc_get_params(kr_nrc *n, float *params) {
+    if (!n || !params) return fail(KR_E_INVALID, "NULL argument");
+    KR_HIP(hipMemcpyAsync(params, n->master, sizeof(float) * nrc::kNumParams, hipMemcpyDeviceToHost, n->dev->stream));
+    KR_HIP(hipStreamSynchronize(n->dev->stream));
+    return KR_OK;
+}
+
+KR_EXPORT int kr_nrc_set_params(kr_nrc *n, const float *params) {
+    if (!n || !params) return fail(KR_E_INVALID, "NULL argument");
+    return nrc_upload_params(n, params);
+}
+
+KR_EXPORT int kr_nrc_encode(kr_nrc *n, const float *raw, uint32_t cnt, float *features) {
+    if (!n || !raw || !features) return fail(KR_E_INVALID, "NULL argument");
+    if (!cnt) return KR_OK;
+    DevBufs b;
+    float *r, *f;
+    int rc;
+    if ((rc = b.get(&r, size_t(cnt) * nrc::kRaw)) || (rc = b.get(&f, size_t(cnt) * nrc::kIn))) return rc;
+    hipStream_t st = n->dev->stream;
+    KR_HIP(hipMemcpyAsync(r, raw, sizeof(float) * cnt * nrc::kRaw, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_nrc_encode, dim3((cnt + 255) / 256), dim3(256), 0, st, nrc_view(n), r, cnt, f);
+    KR_HIP(hipGetLastError());
+    KR_HIP(hipMemcpyAsync(features, f, sizeof(float) * cnt * nrc::kIn, hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
+    return KR_OK;
+}
+
+KR_EXPORT int kr_nrc_predict(kr_nrc *n, const float *raw, uint32_t cnt, float *rgb) {
+    if (!n || !raw || !rgb) return fail(KR_E_INVALID, "NULL argument");
+    if (!cnt) return KR_OK;
+    DevBufs b;
+    float *r, *o;
+    int rc;
+    if ((rc = b.get(&r, size_t(cnt) * nrc::kRaw)) || (rc = b.get(&o, size_t(cnt) * 3))) return rc;
+    hipStream_t st = n->dev->stream;
+    KR_HIP(hipMemcpyAsync(r, raw, sizeof(float) * cnt * nrc::kRaw, hipMemcpyHostToDevice, st));
+    uint32_t waves = (cnt + 63) / 64;
+    hipLaunchKernelGGL(k_nrc_predict, dim3((waves + 3) / 4), dim3(256), 0, st, nrc_view(n), r, cnt, o);
+    KR_HIP(hipGetLastError());
+    KR_HIP(hipMemcpyAsync(rgb, o, sizeof(float) * cnt * 3, hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
+    return KR_OK;
+}
+
+namespace {
+int nrc_explicit_source(kr_nrc *n, DevBufs &b, const float *raw, const float *target, uint32_t cnt, TrainSource &src,
+                        float **partial, uint32_t *rows) {
+    float *r, *t;
+    int rc;
+    if ((rc = b.get(&r, size_t(cnt) * nrc::kRaw)) || (rc = b.get(&t, size_t(cnt) * 4))) return rc;
+    std::vector<float> t4(size_t(cnt) * 4, 0.f);
+    for (uint32_t i = 0; i < cnt; ++i)
+        for (int c = 0; c < 3; ++c) t4[4 * size_t(i) + c] = target[3 * size_t(i) + c];
+    hipStream_t st = n->dev->stream;
+    KR_HIP(hipMemcpyAsync(r, raw, sizeof(float) * cnt * nrc::kRaw, hipMemcpyHostToDevice, st));
+    KR_HIP(hipMemcpyAsync(t, t4.data(), sizeof(float) * t4.size(), hipMemcpyHostToDevice, st));
+    KR_HIP(hipStreamSynchronize(st));
+    src.raw = r;
+    src.target = t;
+    src.pool_total = nullptr;
+    src.n = cnt;
+    src.cap = cnt;
+    src.step = 0;
+    src.steps = 1;
+    *rows = (cnt + 63) / 64;
+    if ((rc = b.get(partial, size_t(*rows) * nrc::kNumParams))) return rc;
+    return KR_OK;
+}
+}  // namespace
+
+KR_EXPORT int kr_nrc_gradient(kr_nrc *n, const float *raw, const float *target, uint32_t cnt, float *grad,
+                              float *loss) {
+    if (!n || !raw || !target || !grad || !cnt) return fail(KR_E_INVALID, "bad argument");
+    DevBufs b;
+    TrainSource src;
+    float *part;
+    uint32_t rows;
+    int rc = nrc_explicit_source(n, b, raw, target, cnt, src, &part, &rows);
+    if (rc) return rc;
+    float *g;
+    if ((rc = b.get(&g, nrc::kNumParams))) return rc;
+    hipStream_t st = n->dev->stream;
+    KR_HIP(hipMemsetAsync(n->loss, 0, sizeof(float), st));
+    hipLaunchKernelGGL(k_nrc_fwd_bwd, dim3(rows), dim3(64), 0, st, nrc_view(n), src, cnt, part, n->loss);
+    hipLaunchKernelGGL(k_nrc_sum_rows, dim3(96), dim3(256), 0, st, part, rows, g);
+    KR_HIP(hipGetLastError());
+    KR_HIP(hipMemcpyAsync(grad, g, sizeof(float) * nrc::kNumParams, hipMemcpyDeviceToHost, st));
+    if (loss) KR_HIP(hipMemcpyAsync(loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
+    return KR_OK;
+}
+
+KR_EXPORT int kr_nrc_train(kr_nrc *n, const float *raw, const float *target, uint32_t cnt, uint32_t steps, float *loss) {
+    if (!n || !raw || !target || !cnt) return fail(KR_E_INVALID, "bad argument");
+    DevBufs b;
+    TrainSource src;
+    float *part;
+    uint32_t rows;
+    int rc = nrc_explicit_source(n, b, raw, target, cnt, src, &part, &rows);
+    if (rc) return rc;
+    float *saved = n->partial;
+    n->partial = part;
+    for (uint32_t s = 0; s < steps && !rc; ++s) rc = nrc_step(n, src, cnt, rows);
+    n->partial = saved;
+    if (rc) return rc;
+    float l = 0.f;
+    KR_HIP(hipMemcpyAsync(&l, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->dev->stream));
+    KR_HIP(hipStreamSynchronize(n->dev->stream));
+    n->samples += uint64_t(cnt) * steps;
+    n->last_loss = l;
+    if (loss) *loss = l;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_nrc_get_stats(kr_nrc *n, uint64_t *samples_trained, float *last_loss) {
+    if (!n) return fail(KR_E_INVALID, "NULL argument");
+    if (samples_trained) *samples_trained = n->samples;
+    if (last_loss) *last_loss = n->last_loss;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    if (n) return fail(KR_E_UNSUPPORTED, "cache integration not built yet");
+    return KR_OK;
+}

@@ -1,0 +1,195 @@
+// kr_nrc.h -- neural radiance cache on gfx950 MFMA: a 96-64-64-64-64-64-16
+// ReLU MLP with BF16 weights and activations and FP32 accumulation, trained
+// online with Adam (FP32 master weights).
+//
+// Every layer is evaluated transposed, Y^T = W^T X^T, with
+// v_mfma_f32_16x16x16_bf16 on a wave of 64 samples (4 tiles of 16). In that
+// form the 16x16 output tile a lane holds (4 consecutive output features of
+// one sample) is exactly the B-operand fragment the next layer needs, so
+// activations stay in registers from the input encoding to the output.
+//
+// Fragment layout of v_mfma_f32_16x16x16_bf16 (wave64), lane l = 16 q + s:
+//   A[m][k]: lane holds A[s][4q + r], r = 0..3
+//   B[k][n]: lane holds B[4q + r][s]
+//   D[m][n]: lane holds D[4q + r][s]
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace kr {
+namespace nrc {
+
+constexpr int kIn = 96, kInFeatures = 80, kWidth = 64, kOut = 16, kLayers = 6, kFreqs = 12;
+constexpr int kRaw = 16;  // vertex record: pos(3) wo(3) normal(3) roughness(1) albedo(3) pad(3)
+constexpr int kDims[kLayers + 1] = {kIn, kWidth, kWidth, kWidth, kWidth, kWidth, kOut};
+// flat FP32 parameter order: W0 (in x out, row major), b0, W1, b1, ...
+constexpr int w_off(int l) {
+    int o = 0;
+    for (int i = 0; i < l; ++i) o += kDims[i] * kDims[i + 1] + kDims[i + 1];
+    return o;
+}
+constexpr int b_off(int l) { return w_off(l) + kDims[l] * kDims[l + 1]; }
+constexpr int kNumParams = w_off(kLayers);
+// BF16 weight copies (no biases): layer l starts at bw_off(l)
+constexpr int bw_off(int l) {
+    int o = 0;
+    for (int i = 0; i < l; ++i) o += kDims[i] * kDims[i + 1];
+    return o;
+}
+constexpr int kNumWeights = bw_off(kLayers);
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct Weights {
+    const uint16_t *wt;  // per layer W^T [out][in], bf16
+    const uint16_t *w;   // per layer W   [in][out], bf16
+    const float *master; // FP32 parameters (biases read from here)
+    float box_min[3], box_scale[3];
+};
+
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return uint16_t((u >> 16) | 0x40u);  // NaN
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+__device__ __forceinline__ v4s pack4(float a, float b, float c, float d) {
+    v4s v;
+    v[0] = short(to_bf16(a));
+    v[1] = short(to_bf16(b));
+    v[2] = short(to_bf16(c));
+    v[3] = short(to_bf16(d));
+    return v;
+}
+
+__device__ __forceinline__ void angles(float x, float y, float z, float &th, float &ph) {
+    th = acosf(fminf(fmaxf(z, -1.f), 1.f)) * (1.0f / 3.14159265358979f);
+    ph = atan2f(y, x) * (0.5f / 3.14159265358979f) + 0.5f;
+}
+
+// A vertex record's fields in registers.
+struct RawRec {
+    float px, py, pz, wx, wy, wz, nx, ny, nz, rough, ar, ag, ab;
+};
+
+__device__ __forceinline__ RawRec load_raw(const float *raw) {
+    const float4 *r4 = reinterpret_cast<const float4 *>(raw);
+    float4 a = r4[0], b = r4[1], c = r4[2], d = r4[3];
+    return RawRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x};
+}
+
+// Feature `idx` of a vertex record: sin / cos(2^k pi p) of the normalised
+// position (idx < 72, ordered [sin xyz, cos xyz] per k), the outgoing
+// direction and normal as (theta/pi, phi/2pi + 1/2), roughness, albedo,
+// zero padding. Selects instead of indexed loads keep it in registers.
+__device__ __forceinline__ float feature(int idx, const RawRec &r, const Weights &W) {
+    if (idx < 72) {
+        int k = idx / 6, j = idx - 6 * k;
+        int c = j >= 3 ? j - 3 : j;
+        float p = c == 0 ? (r.px - W.box_min[0]) * W.box_scale[0]
+                         : (c == 1 ? (r.py - W.box_min[1]) * W.box_scale[1] : (r.pz - W.box_min[2]) * W.box_scale[2]);
+        float a = float(1 << k) * 3.14159265358979f * p;
+        return j < 3 ? sinf(a) : cosf(a);
+    }
+    if (idx < 76) {
+        bool nrm = idx >= 74;
+        float x = nrm ? r.nx : r.wx, y = nrm ? r.ny : r.wy, z = nrm ? r.nz : r.wz;
+        float th, ph;
+        angles(x, y, z, th, ph);
+        return (idx & 1) ? ph : th;
+    }
+    if (idx == 76) return r.rough;
+    if (idx == 77) return r.ar;
+    if (idx == 78) return r.ag;
+    if (idx == 79) return r.ab;
+    return 0.f;
+}
+
+// Input fragments of a wave's 64 samples: in[kt][nt] for kt < 6, nt < 4.
+// raw_of(n) gives the record of sample n of the wave (nullptr: padding,
+// encoded as zeros).
+template <class RawOf>
+__device__ __forceinline__ void encode_inputs(v4s in[6][4], RawOf raw_of, const Weights &W) {
+    int lane = __lane_id(), s = lane & 15, q = lane >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        const float *raw = raw_of(nt * 16 + s);
+        RawRec r{};
+        if (raw) r = load_raw(raw);
+#pragma unroll
+        for (int kt = 0; kt < 6; ++kt) {
+            float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f;
+            if (raw) {
+                int b = kt * 16 + 4 * q;
+                f0 = feature(b, r, W);
+                f1 = feature(b + 1, r, W);
+                f2 = feature(b + 2, r, W);
+                f3 = feature(b + 3, r, W);
+            }
+            in[kt][nt] = pack4(f0, f1, f2, f3);
+        }
+    }
+}
+
+// Hidden layer: Y(mt, nt) = ReLU(sum_kt W^T(mt, kt) X^T(kt, nt) + b), packed
+// to BF16 fragments for the next layer. KT = in/16, MT = out/16.
+template <int KT, int MT>
+__device__ __forceinline__ void hidden_layer(const v4s x[KT][4], v4s y[MT][4], const uint16_t *wt, const float *bias) {
+    int lane = __lane_id(), s = lane & 15, q = lane >> 4;
+    constexpr int IN = KT * 16;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        v4s a[KT];
+#pragma unroll
+        for (int kt = 0; kt < KT; ++kt)
+            a[kt] = *reinterpret_cast<const v4s *>(wt + (mt * 16 + s) * IN + kt * 16 + 4 * q);
+        float4 bb = *reinterpret_cast<const float4 *>(bias + mt * 16 + 4 * q);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < KT; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], x[kt][nt], acc, 0, 0, 0);
+            y[mt][nt] = pack4(fmaxf(acc[0] + bb.x, 0.f), fmaxf(acc[1] + bb.y, 0.f), fmaxf(acc[2] + bb.z, 0.f),
+                              fmaxf(acc[3] + bb.w, 0.f));
+        }
+    }
+}
+
+// Output layer (64 -> 16, linear): out[nt] = outputs 4q..4q+3 of sample nt*16 + s.
+__device__ __forceinline__ void output_layer(const v4s x[4][4], v4f out[4], const uint16_t *wt, const float *bias) {
+    int lane = __lane_id(), s = lane & 15, q = lane >> 4;
+    v4s a[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) a[kt] = *reinterpret_cast<const v4s *>(wt + s * kWidth + kt * 16 + 4 * q);
+    float4 bb = *reinterpret_cast<const float4 *>(bias + 4 * q);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], x[kt][nt], acc, 0, 0, 0);
+        acc[0] += bb.x;
+        acc[1] += bb.y;
+        acc[2] += bb.z;
+        acc[3] += bb.w;
+        out[nt] = acc;
+    }
+}
+
+// Forward of a wave's 64 samples; out[nt] in lanes q = 0 holds the RGB
+// prediction of sample nt*16 + s.
+__device__ __forceinline__ void forward(const v4s in[6][4], v4f out[4], const Weights &W) {
+    v4s h[4][4], g[4][4];
+    hidden_layer<6, 4>(in, h, W.wt + bw_off(0), W.master + b_off(0));
+    hidden_layer<4, 4>(h, g, W.wt + bw_off(1), W.master + b_off(1));
+    hidden_layer<4, 4>(g, h, W.wt + bw_off(2), W.master + b_off(2));
+    hidden_layer<4, 4>(h, g, W.wt + bw_off(3), W.master + b_off(3));
+    hidden_layer<4, 4>(g, h, W.wt + bw_off(4), W.master + b_off(4));
+    output_layer(h, out, W.wt + bw_off(5), W.master + b_off(5));
+}
+
+}  // namespace nrc
+}  // namespace kr

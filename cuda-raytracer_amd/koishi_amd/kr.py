@@ -85,12 +85,21 @@ class RenderStats(C.Structure):
 KR_PROFILE_EVENTS = 0x1
 KR_PROFILE_COUNTERS = 0x2
 
+NRC_RAW, NRC_FEATURES, NRC_PARAMS = 16, 96, 23888
+
+
+class NrcConfig(C.Structure):
+    _fields_ = [("box_min", f32 * 3), ("box_max", f32 * 3), ("learning_rate", f32), ("seed", u32),
+                ("batch", u32), ("steps_per_frame", u32), ("train_log2", u32), ("rough_threshold", f32)]
+
 
 EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "kr_device_close",
            "kr_scene_upload", "kr_scene_free", "kr_scene_get_info", "kr_scene_light_samples",
            "kr_trace_closest", "kr_trace_shadow", "kr_render_create", "kr_render_run", "kr_render_sync",
            "kr_render_reset", "kr_render_read", "kr_render_get_stats", "kr_render_free",
-           "kr_render_radiance", "kr_render_set_profiling"]
+           "kr_render_radiance", "kr_render_set_profiling", "kr_device_bandwidth", "kr_nrc_create", "kr_nrc_free",
+           "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
+           "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc"]
 
 _lib = None
 
@@ -124,6 +133,17 @@ def load(path: str = LIB_PATH):
     lib.kr_render_free.argtypes = [vp]
     lib.kr_render_radiance.argtypes = [vp, C.POINTER(Camera), C.POINTER(Params), vp, vp]
     lib.kr_render_set_profiling.argtypes = [vp, u32]
+    lib.kr_device_bandwidth.argtypes = [vp, C.c_size_t, u32, C.POINTER(C.c_double)]
+    lib.kr_nrc_create.argtypes = [vp, C.POINTER(NrcConfig), C.POINTER(vp)]
+    lib.kr_nrc_free.argtypes = [vp]
+    lib.kr_nrc_get_params.argtypes = [vp, vp]
+    lib.kr_nrc_set_params.argtypes = [vp, vp]
+    lib.kr_nrc_encode.argtypes = [vp, vp, u32, vp]
+    lib.kr_nrc_predict.argtypes = [vp, vp, u32, vp]
+    lib.kr_nrc_gradient.argtypes = [vp, vp, vp, u32, vp, C.POINTER(f32)]
+    lib.kr_nrc_train.argtypes = [vp, vp, vp, u32, u32, C.POINTER(f32)]
+    lib.kr_nrc_get_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(f32)]
+    lib.kr_render_attach_nrc.argtypes = [vp, vp]
     for name in EXPORTS:
         if name not in ("kr_version",):
             getattr(lib, name).restype = C.c_int
@@ -156,6 +176,12 @@ class Device:
         h = C.c_void_p()
         check(lib.kr_device_open(ordinal, C.byref(h)), "kr_device_open")
         self.h = h
+
+    def bandwidth(self, nbytes: int, iters: int = 10) -> float:
+        """GB/s of the plain copy kernel k_copy over nbytes (read + write)."""
+        g = C.c_double()
+        check(self.lib.kr_device_bandwidth(self.h, nbytes, iters, C.byref(g)), "kr_device_bandwidth")
+        return g.value
 
     def close(self):
         if self.h:
@@ -247,6 +273,9 @@ class Render:
         check(self.lib.kr_render_read(self.h, _ptr(rgb), _ptr(valid)), "kr_render_read")
         return rgb, valid
 
+    def attach_nrc(self, nrc):
+        check(self.lib.kr_render_attach_nrc(self.h, nrc.h if nrc is not None else None), "kr_render_attach_nrc")
+
     def set_profiling(self, flags: int):
         check(self.lib.kr_render_set_profiling(self.h, flags), "kr_render_set_profiling")
 
@@ -266,3 +295,64 @@ def device_count() -> int:
     n = C.c_int(0)
     check(lib.kr_device_count(C.byref(n)), "kr_device_count")
     return n.value
+
+
+class Nrc:
+    """kr_nrc handle: the neural radiance cache on one device."""
+
+    def __init__(self, dev: Device, box_min, box_max, seed=5, learning_rate=0.0, batch=0, steps_per_frame=0,
+                 train_log2=0, rough_threshold=0.0):
+        self.lib = dev.lib
+        cfg = NrcConfig((f32 * 3)(*box_min), (f32 * 3)(*box_max), learning_rate, seed, batch, steps_per_frame,
+                        train_log2, rough_threshold)
+        h = C.c_void_p()
+        check(self.lib.kr_nrc_create(dev.h, C.byref(cfg), C.byref(h)), "kr_nrc_create")
+        self.h = h
+
+    def get_params(self):
+        p = np.zeros(NRC_PARAMS, np.float32)
+        check(self.lib.kr_nrc_get_params(self.h, _ptr(p)), "kr_nrc_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(self.lib.kr_nrc_set_params(self.h, _ptr(p)), "kr_nrc_set_params")
+
+    def encode(self, raw):
+        raw = np.ascontiguousarray(raw, np.float32)
+        out = np.zeros((raw.shape[0], NRC_FEATURES), np.float32)
+        check(self.lib.kr_nrc_encode(self.h, _ptr(raw), raw.shape[0], _ptr(out)), "kr_nrc_encode")
+        return out
+
+    def predict(self, raw):
+        raw = np.ascontiguousarray(raw, np.float32)
+        out = np.zeros((raw.shape[0], 3), np.float32)
+        check(self.lib.kr_nrc_predict(self.h, _ptr(raw), raw.shape[0], _ptr(out)), "kr_nrc_predict")
+        return out
+
+    def gradient(self, raw, target):
+        raw = np.ascontiguousarray(raw, np.float32)
+        tgt = np.ascontiguousarray(target, np.float32)
+        g = np.zeros(NRC_PARAMS, np.float32)
+        loss = f32()
+        check(self.lib.kr_nrc_gradient(self.h, _ptr(raw), _ptr(tgt), raw.shape[0], _ptr(g), C.byref(loss)),
+              "kr_nrc_gradient")
+        return g, loss.value
+
+    def train(self, raw, target, steps=1):
+        raw = np.ascontiguousarray(raw, np.float32)
+        tgt = np.ascontiguousarray(target, np.float32)
+        loss = f32()
+        check(self.lib.kr_nrc_train(self.h, _ptr(raw), _ptr(tgt), raw.shape[0], steps, C.byref(loss)),
+              "kr_nrc_train")
+        return loss.value
+
+    def stats(self):
+        n, l = u64(), f32()
+        check(self.lib.kr_nrc_get_stats(self.h, C.byref(n), C.byref(l)), "kr_nrc_get_stats")
+        return n.value, l.value
+
+    def free(self):
+        if self.h:
+            self.lib.kr_nrc_free(self.h)
+            self.h = None

@@ -246,10 +246,15 @@ def icosahedron():
     return v, f
 
 
-def geodesic_sphere(freq: int):
+def geodesic_sphere(freq: int, rotation=None):
     """Unit geodesic sphere: every icosahedron face split into freq^2 triangles.
-    Shared edge/corner points are merged through canonical (corner, weight) keys."""
+    Shared edge/corner points are merged through canonical (corner, weight) keys.
+    `rotation` (3x3) turns the base icosahedron first, so that no edges lie in
+    a coordinate plane (the icosahedron's own symmetry planes are x, y, z = 0,
+    where a camera's centre row or column of rays would graze them exactly)."""
     V, F = icosahedron()
+    if rotation is not None:
+        V = V @ np.asarray(rotation).T
     n = freq
     ij = [(i, j) for i in range(n + 1) for j in range(n + 1 - i)]
     ij = np.array(ij, np.int64)  # weights of corner B (i) and C (j)
@@ -333,7 +338,7 @@ def vertex_normals(pos: np.ndarray, tris: np.ndarray) -> np.ndarray:
 
 
 def displaced_icosphere(freq: int, seed: int, amplitude: float = 0.15):
-    pts, tris = geodesic_sphere(freq)
+    pts, tris = geodesic_sphere(freq, quat_matrix(SplitMix64(seed ^ 0x1C05)))
     h = value_noise(pts, seed)
     pos = pts * (1.0 + amplitude * h)[:, None]
     nrm = vertex_normals(pos, tris)

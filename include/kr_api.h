@@ -199,6 +199,13 @@ int kr_device_count(int *count);
 int kr_device_open(int ordinal, kr_device **out);
 int kr_device_close(kr_device *dev);
 
+/* Streams a plain device-to-device copy of `bytes` bytes (16-byte vector
+ * loads and stores, kernel k_copy) `iters` times on the device's stream and
+ * reports the achieved bandwidth (read + write bytes / time). No reference
+ * counterpart: the HBM probe the roofline and the traffic counters are
+ * calibrated against. */
+int kr_device_bandwidth(kr_device *dev, size_t bytes, uint32_t iters, double *gbps);
+
 /* ---- scene -------------------------------------------------------------
  * Replaces the transfer of core::Scene (src/core/meta/scene.hpp:15-71,
  * built by src/core/meta/scene.cu:18-219) through poly::vector/object.
@@ -251,6 +258,60 @@ int kr_render_free(kr_render *render);
 /* One-shot render of all params->spp samples into caller buffers. */
 int kr_render_radiance(kr_scene *scene, const kr_camera *camera, const kr_params *params,
                        double *rgb, uint32_t *valid);
+
+/* ---- neural radiance cache ----------------------------------------------
+ * No counterpart in the reference: an online-trained radiance cache
+ * (Mueller et al., SIGGRAPH 2021) that ends paths early. The MLP is
+ * 96-64-64-64-64-64-16 (ReLU, 3 outputs used), BF16 weights and activations
+ * on MFMA with FP32 accumulation, Adam on FP32 master weights.
+ *
+ * Input of one query, a "vertex record" of KR_NRC_RAW floats:
+ *   position xyz, outgoing direction xyz (towards the previous vertex),
+ *   shading normal xyz, roughness (0 for Lambert / Emission), albedo rgb, 0 0 0.
+ * Parameters are KR_NRC_PARAMS floats: W0 (96 x 64, row major, in x out),
+ * b0 (64), W1 (64 x 64), b1, ..., W5 (64 x 16), b5 (16). */
+#define KR_NRC_RAW 16
+#define KR_NRC_FEATURES 96
+#define KR_NRC_PARAMS 23888
+
+typedef struct kr_nrc kr_nrc;
+
+typedef struct kr_nrc_config {
+    float box_min[3], box_max[3]; /* scene box for the position encoding      */
+    float learning_rate;          /* Adam step size; 0 = 1e-2                  */
+    uint32_t seed;                /* weight initialisation                     */
+    uint32_t batch;               /* training samples per step; 0 = 16384      */
+    uint32_t steps_per_frame;     /* Adam steps after each render run; 0 = 4   */
+    uint32_t train_log2;          /* 1 / 2^train_log2 of paths train; 0 = 4    */
+    float rough_threshold;        /* GlossyBSDF roughness counted as rough;
+                                     0 = 0.3                                   */
+} kr_nrc_config;
+
+int kr_nrc_create(kr_device *dev, const kr_nrc_config *config, kr_nrc **out);
+int kr_nrc_free(kr_nrc *nrc);
+int kr_nrc_get_params(kr_nrc *nrc, float *params);
+int kr_nrc_set_params(kr_nrc *nrc, const float *params); /* also resets Adam */
+/* Feature vectors (n x KR_NRC_FEATURES floats) of n vertex records. */
+int kr_nrc_encode(kr_nrc *nrc, const float *raw, uint32_t n, float *features);
+/* Cache prediction (n x 3 floats) for n vertex records. */
+int kr_nrc_predict(kr_nrc *nrc, const float *raw, uint32_t n, float *rgb);
+/* Relative-L2 loss and its gradient (KR_NRC_PARAMS floats) over n records
+ * with targets (n x 3); no update. */
+int kr_nrc_gradient(kr_nrc *nrc, const float *raw, const float *target, uint32_t n, float *grad,
+                    float *loss);
+/* `steps` Adam steps, each on all n records; loss of the last step. */
+int kr_nrc_train(kr_nrc *nrc, const float *raw, const float *target, uint32_t n, uint32_t steps,
+                 float *loss);
+/* Samples trained on so far, and the mean loss of the last training step. */
+int kr_nrc_get_stats(kr_nrc *nrc, uint64_t *samples_trained, float *last_loss);
+
+/* Attaches a cache to a render (NULL detaches). While attached, one path in
+ * 2^train_log2 is traced unbiased to max_bounce and its vertices become
+ * training samples; every other path ends at its second diffuse or rough
+ * vertex, where the cache's prediction replaces the rest of the path. After
+ * each kr_render_run the cache takes steps_per_frame Adam steps on the most
+ * recent samples. */
+int kr_render_attach_nrc(kr_render *render, kr_nrc *nrc);
 
 #ifdef __cplusplus
 }
