@@ -43,7 +43,7 @@ struct ShaderRec {
     uint32_t type;   // KR_SHADER_*
     uint32_t dist;   // KR_DIST_*
     float alpha2;    // roughness^2
-    float pad;
+    float roughness; // GlossyBSDF roughness, 0 for Lambert / Emission
     float cr[4];     // Lambert color*R; Glossy color; Emission (1,1,1)
     float emission[4];
 };

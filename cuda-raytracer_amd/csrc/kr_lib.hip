@@ -124,6 +124,20 @@ struct kr_scene {
 struct RayQ {
     uint32_t *slot;
     float *ox, *oy, *oz, *dx, *dy, *dz, *bx, *by, *bz;
+    uint32_t *flags;  // bit 0: training path; bits 1-2: diffuse/rough vertices so far
+};
+
+// Radiance-cache state of a render (all null / 0 when no cache is attached).
+struct NrcRender {
+    int enabled;
+    uint32_t train_mask;   // a path trains when (hash & train_mask) == 0
+    float rough_threshold; // GlossyBSDF roughness counted as rough
+    uint32_t *tslot_of;    // slot -> training-path index or kMissCode
+    float *rec;            // training vertex records [train_cap][nv][kRec]
+    uint32_t *tcount;      // vertices recorded per training path
+    uint32_t train_cap, nv;
+    uint32_t *qslot;       // cache queries: slot, throughput, vertex record
+    float *qbeta, *qraw;
 };
 struct ShadowQ {
     uint32_t *slot;
@@ -144,10 +158,30 @@ enum : int {
     C_RAY1 = 1,
     C_BUCKET = 2,  // 3 entries
     C_SHADOW = 5,
+    C_NRCQ = 6,   // cache queries
+    C_TRAIN = 7,  // training paths
     C_NUM = 8
 };
 enum : int { S_CLOSEST = 0, S_SHADOW = 1, S_INVALID = 2, S_PATHS = 3, S_NODES = 4, S_TRIS = 5, S_NUM = 8 };
 enum : int { K_TRACE = 0, K_SHADE = 1, K_SHADOW = 2, K_OTHER = 3, K_KINDS = 4 };
+
+struct kr_nrc {
+    kr_device *dev = nullptr;
+    kr_nrc_config cfg{};
+    float *master = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    uint16_t *wt = nullptr, *w = nullptr;
+    float *partial = nullptr;  // [batch / 64][kNumParams]
+    float *loss = nullptr;     // device scalar
+    uint32_t rows = 0;         // batch / 64
+    uint32_t step = 0;         // Adam steps taken
+    uint64_t samples = 0;
+    float last_loss = 0.f;
+    // training pool filled by renders (ring buffer)
+    float *pool_raw = nullptr, *pool_tgt = nullptr;
+    uint32_t *pool_total = nullptr;
+    uint32_t pool_cap = 0;
+    std::vector<void *> allocs;
+};
 
 struct kr_render {
     kr_scene *scene = nullptr;
@@ -177,10 +211,16 @@ struct kr_render {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
     double kind_ms[K_KINDS] = {0, 0, 0, 0};
     uint64_t kind_launches[K_KINDS] = {0, 0, 0, 0};
+    // attached radiance cache
+    kr_nrc *nrc = nullptr;
+    NrcRender nr{};
+    std::vector<void *> nrc_allocs;
 };
 
 namespace {
 void resolve_events(kr_render *r);
+void nrc_after_chunk(kr_render *r);
+int nrc_after_run(kr_render *r);
 }  // namespace
 
 // ------------------------------------------------------------- kernels --
@@ -274,48 +314,78 @@ __global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src,
         dst[i] = src[i];
 }
 
-__global__ void k_reset(uint32_t *counters, uint32_t keep_ray_slot) {
+// Clears the counters whose bit is set in `mask`.
+__global__ void k_reset(uint32_t *counters, uint32_t mask) {
     int i = threadIdx.x;
-    if (i < C_NUM && i != int(keep_ray_slot)) counters[i] = 0;
+    if (i < C_NUM && ((mask >> i) & 1u)) counters[i] = 0;
 }
 
 // Camera rays of chunk [g0, g0 + P): slot j is path g = g0 + j, sample
 // g / npix of pixel g % npix. PinholeLens/OrthographicLens::sample
-// (src/core/misc/lens.hpp:145-164).
+// (src/core/misc/lens.hpp:145-164). With a cache attached, one path in
+// train_mask + 1 (picked by a Philox hash of (pixel, sample)) becomes a
+// training path and gets a record slot.
+constexpr uint32_t kStreamTrainPick = 0x5452414Eu;
+constexpr int kRecFloats = 28;  // vertex record: raw 16 | beta 3 | c 3 | beta*E 3 | pad 3
+
 __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, uint64_t g0, uint32_t P, uint32_t npix, RayQ q,
-                                                   float *L, uint32_t *counters, int qi) {
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
-        uint64_t g = g0 + j;
-        uint32_t s = uint32_t(g / npix), px = uint32_t(g % npix);
-        uint32_t x = px % lens.w, y = px / lens.w;
-        float fx = float(x) - float(lens.w) * 0.5f, fy = float(y) - float(lens.h) * 0.5f;
-        float kx = float(s % lens.pw), ky = float(s / lens.pw);
-        f3 n{lens.n[0], lens.n[1], lens.n[2]}, u{lens.u[0], lens.u[1], lens.u[2]}, v{lens.v[0], lens.v[1], lens.v[2]};
-        f3 du{lens.du[0], lens.du[1], lens.du[2]}, dv{lens.dv[0], lens.dv[1], lens.dv[2]};
-        f3 o{lens.o[0], lens.o[1], lens.o[2]};
-        f3 d;
-        if (lens.lens == KR_LENS_ORTHOGRAPHIC) {
-            f3 dd = u * fx + v * fy;
-            dd = dd + (du * kx + dv * ky);
-            o = o + dd * 4.f / float(lens.w);
-            d = normalize(n);
-        } else {
-            f3 t = n + u * fx + v * fy;
-            d = normalize(t + du * kx + dv * ky);
+                                                   float *L, uint32_t *counters, int qi, NrcRender nr,
+                                                   uint32_t seed_lo, uint32_t seed_hi) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += stride) {
+        uint32_t j = base + threadIdx.x;
+        bool active = j < P;
+        bool train = false;
+        if (active) {
+            uint64_t g = g0 + j;
+            uint32_t s = uint32_t(g / npix), px = uint32_t(g % npix);
+            uint32_t x = px % lens.w, y = px / lens.w;
+            float fx = float(x) - float(lens.w) * 0.5f, fy = float(y) - float(lens.h) * 0.5f;
+            float kx = float(s % lens.pw), ky = float(s / lens.pw);
+            f3 n{lens.n[0], lens.n[1], lens.n[2]}, u{lens.u[0], lens.u[1], lens.u[2]};
+            f3 v{lens.v[0], lens.v[1], lens.v[2]};
+            f3 du{lens.du[0], lens.du[1], lens.du[2]}, dv{lens.dv[0], lens.dv[1], lens.dv[2]};
+            f3 o{lens.o[0], lens.o[1], lens.o[2]};
+            f3 d;
+            if (lens.lens == KR_LENS_ORTHOGRAPHIC) {
+                f3 dd = u * fx + v * fy;
+                dd = dd + (du * kx + dv * ky);
+                o = o + dd * 4.f / float(lens.w);
+                d = normalize(n);
+            } else {
+                f3 t = n + u * fx + v * fy;
+                d = normalize(t + du * kx + dv * ky);
+            }
+            q.slot[j] = j;
+            q.ox[j] = o.x;
+            q.oy[j] = o.y;
+            q.oz[j] = o.z;
+            q.dx[j] = d.x;
+            q.dy[j] = d.y;
+            q.dz[j] = d.z;
+            q.bx[j] = 1.f;
+            q.by[j] = 1.f;
+            q.bz[j] = 1.f;
+            L[3 * j] = 0.f;
+            L[3 * j + 1] = 0.f;
+            L[3 * j + 2] = 0.f;
+            if (nr.enabled) {
+                Philox4 h = philox(px, s, 0xFFFFFFFFu, kStreamTrainPick, seed_lo, seed_hi);
+                train = (h.v[0] & nr.train_mask) == 0u;
+            }
         }
-        q.slot[j] = j;
-        q.ox[j] = o.x;
-        q.oy[j] = o.y;
-        q.oz[j] = o.z;
-        q.dx[j] = d.x;
-        q.dy[j] = d.y;
-        q.dz[j] = d.z;
-        q.bx[j] = 1.f;
-        q.by[j] = 1.f;
-        q.bz[j] = 1.f;
-        L[3 * j] = 0.f;
-        L[3 * j + 1] = 0.f;
-        L[3 * j + 2] = 0.f;
+        uint32_t flags = 0;
+        if (nr.enabled) {
+            uint32_t ts = wave_append(&counters[C_TRAIN], train);
+            if (train && ts < nr.train_cap) {
+                flags = 1u;
+                nr.tcount[ts] = 0;
+            } else {
+                ts = kMissCode;
+            }
+            if (active) nr.tslot_of[j] = ts;
+        }
+        if (active) q.flags[j] = flags;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) counters[qi] = P;
 }
@@ -379,6 +449,7 @@ struct ShadeArgs {
     unsigned long long *stats;
     uint64_t g0;
     uint32_t npix, bounce, max_bounce, seed_lo, seed_hi;
+    NrcRender nr;
 };
 
 // RadianceKernel::execute loop body (src/core/kernel/radiance.cu:26-71) for
@@ -391,10 +462,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
         uint32_t k = base + threadIdx.x;
         bool active = k < total;
-        bool push_shadow = false, push_ray = false;
-        uint32_t slot = 0;
+        bool push_shadow = false, push_ray = false, push_query = false;
+        uint32_t slot = 0, nflags = 0;
         f3 so{0, 0, 0}, sd{0, 0, 0}, contrib{0, 0, 0}, no{0, 0, 0}, nd{0, 0, 0}, nb{0, 0, 0};
         float stmax = 0.f;
+        float qrec[13];
         if (active) {
             uint32_t i = k < n0 ? a.b0[k] : (k < n0 + n1 ? a.b1[k - n0] : a.b2[k - n0 - n1]);
             slot = a.q.slot[i];
@@ -432,6 +504,45 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             const ShaderRec sh = S.shaders[S.prims[prim].shader];
             f3 L{0.f, 0.f, 0.f};
             bool touch_L = false;
+            // radiance cache: training paths record this vertex; other paths
+            // end at their second diffuse or rough vertex with a cache query
+            uint32_t flags = a.q.flags[i];
+            bool train = (flags & 1u) != 0;
+            uint32_t ndr = (flags >> 1) & 3u;
+            bool terminate = false;
+            float *rec = nullptr;
+            if (a.nr.enabled) {
+                bool dr = sh.type != KR_SHADER_GLOSSY || sh.roughness >= a.nr.rough_threshold;
+                if (!train && dr && ++ndr >= 2u) terminate = true;
+                if (train) {
+                    uint32_t ts = a.nr.tslot_of[slot];
+                    rec = a.nr.rec + (size_t(ts) * a.nr.nv + a.bounce) * kRecFloats;
+                    a.nr.tcount[ts] = a.bounce + 1;
+                }
+                if (terminate || rec) {
+                    float v[13] = {p.x, p.y, p.z, md.x, md.y, md.z, nrm.x, nrm.y, nrm.z, sh.roughness,
+                                   sh.cr[0], sh.cr[1], sh.cr[2]};
+                    for (int c = 0; c < 13; ++c) qrec[c] = v[c];
+                    if (rec) {
+                        for (int c = 0; c < 13; ++c) rec[c] = v[c];
+                        rec[16] = beta.x;
+                        rec[17] = beta.y;
+                        rec[18] = beta.z;
+                        for (int c = 19; c < 25; ++c) rec[c] = 0.f;
+                    }
+                }
+            }
+            if (terminate) {
+                // emission as usual; the cache stands for everything scattered here
+                if (sh.type == KR_SHADER_EMISSION && S.nlights) {
+                    f3 e = beta * f3{sh.emission[0], sh.emission[1], sh.emission[2]};
+                    a.L[3 * slot] += e.x;
+                    a.L[3 * slot + 1] += e.y;
+                    a.L[3 * slot + 2] += e.z;
+                }
+                push_query = true;
+                nb = beta;
+            } else {
             Philox4 r0 = philox(pixel, sample, 2 * a.bounce, 0u, a.seed_lo, a.seed_hi);
             Philox4 r1 = philox(pixel, sample, 2 * a.bounce + 1, 0u, a.seed_lo, a.seed_hi);
             if (S.nlights) {
@@ -478,6 +589,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                     // occluded (an occluded li is 0/pdf), so no query is needed.
                     L = L + c;
                     touch_L = true;
+                    if (rec) {
+                        rec[19] += c.x;
+                        rec[20] += c.y;
+                        rec[21] += c.z;
+                    }
                 } else if (c.x != 0.f || c.y != 0.f || c.z != 0.f) {
                     push_shadow = true;
                     so = sorig;
@@ -487,8 +603,17 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 }
             }
             if (sh.type == KR_SHADER_EMISSION && S.nlights) {
-                L = L + beta * f3{sh.emission[0], sh.emission[1], sh.emission[2]};
+                f3 e = beta * f3{sh.emission[0], sh.emission[1], sh.emission[2]};
+                L = L + e;
                 touch_L = true;
+                if (rec) {
+                    rec[19] += e.x;
+                    rec[20] += e.y;
+                    rec[21] += e.z;
+                    rec[22] = e.x;
+                    rec[23] = e.y;
+                    rec[24] = e.z;
+                }
             }
             if (touch_L) {
                 a.L[3 * slot] += L.x;
@@ -516,6 +641,22 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 no = go;
                 nd = gd;
                 nb = beta;
+                nflags = (train ? 1u : 0u) | (min(ndr, 3u) << 1);
+            }
+            }  // not terminated
+        }
+        if (a.nr.enabled) {
+            uint32_t pq = wave_append(&a.counters[C_NRCQ], push_query);
+            if (push_query) {
+                a.nr.qslot[pq] = slot;
+                a.nr.qbeta[3 * size_t(pq)] = nb.x;
+                a.nr.qbeta[3 * size_t(pq) + 1] = nb.y;
+                a.nr.qbeta[3 * size_t(pq) + 2] = nb.z;
+                float4 *dst = reinterpret_cast<float4 *>(a.nr.qraw + size_t(pq) * nrc::kRaw);
+                dst[0] = make_float4(qrec[0], qrec[1], qrec[2], qrec[3]);
+                dst[1] = make_float4(qrec[4], qrec[5], qrec[6], qrec[7]);
+                dst[2] = make_float4(qrec[8], qrec[9], qrec[10], qrec[11]);
+                dst[3] = make_float4(qrec[12], 0.f, 0.f, 0.f);
             }
         }
         uint32_t ps = wave_append(&a.counters[C_SHADOW], push_shadow);
@@ -544,13 +685,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             a.qn.bx[pr] = nb.x;
             a.qn.by[pr] = nb.y;
             a.qn.bz[pr] = nb.z;
+            a.qn.flags[pr] = nflags;
         }
     }
 }
 
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, ShadowQ sq, uint32_t *counters, float *L,
-                                                   unsigned long long *stats, uint32_t *spill, size_t spill_lanes) {
+                                                   unsigned long long *stats, uint32_t *spill, size_t spill_lanes,
+                                                   NrcRender nr, uint32_t bounce) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
     __shared__ float st_[kLdsDepth * kBlock];
     auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
@@ -566,6 +709,15 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, ShadowQ sq, uint3
                 L[3 * slot] += sq.cx[i];
                 L[3 * slot + 1] += sq.cy[i];
                 L[3 * slot + 2] += sq.cz[i];
+                if (nr.enabled) {
+                    uint32_t ts = nr.tslot_of[slot];
+                    if (ts != kMissCode) {
+                        float *rec = nr.rec + (size_t(ts) * nr.nv + bounce) * kRecFloats;
+                        rec[19] += sq.cx[i];
+                        rec[20] += sq.cy[i];
+                        rec[21] += sq.cz[i];
+                    }
+                }
             }
         }
         wave_count(&stats[S_SHADOW], active);
@@ -1017,6 +1169,7 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
         r.type = s.type;
         r.dist = s.distribution;
         r.alpha2 = s.roughness * s.roughness;
+        r.roughness = s.type == KR_SHADER_GLOSSY ? s.roughness : 0.f;
         if (s.type == KR_SHADER_LAMBERT) {
             for (int c = 0; c < 3; ++c) r.cr[c] = s.color[c] * s.R[c];
         } else if (s.type == KR_SHADER_GLOSSY) {
@@ -1317,6 +1470,7 @@ KR_EXPORT int kr_render_free(kr_render *r) {
         (void)hipEventDestroy(pe.second.second);
     }
     for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
+    for (void *p : r->nrc_allocs) (void)hipFree(p);
     delete r;
     return KR_OK;
 }
@@ -1378,6 +1532,7 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
         float **fs[9] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.bx, &q.by, &q.bz};
         for (auto f : fs)
             if (!rc) rc = get(f, P);
+        if (!rc) rc = get(&q.flags, P);
     }
     if (!rc) rc = get(&r->hit.t, P);
     if (!rc) rc = get(&r->hit.u, P);
@@ -1482,9 +1637,9 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
     bool count = (r->profile & KR_PROFILE_COUNTERS) != 0;
     {
         Timed tm(r, K_OTHER);
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, uint32_t(0xFFFFFFFFu));
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, 0xFFFFFFFFu);
         hipLaunchKernelGGL(k_camera, dim3(cam_grid), dim3(kBlock), 0, st, r->lens, g0, P, r->npix, r->q[0], r->L,
-                           r->counters, C_RAY0);
+                           r->counters, C_RAY0, r->nr, uint32_t(r->prm.seed), uint32_t(r->prm.seed >> 32));
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
     for (uint32_t b = 0; b <= r->prm.max_bounce; ++b) {
@@ -1493,7 +1648,8 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
         {
             // clear buckets, shadow count and the next ray queue's count
             Timed tm(r, K_OTHER);
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, uint32_t(qi));
+            uint32_t mask = (0x7u << C_BUCKET) | (1u << C_SHADOW) | (1u << qni);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, mask);
         }
         {
             Timed tm(r, K_TRACE);
@@ -1526,6 +1682,7 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
         a.max_bounce = r->prm.max_bounce;
         a.seed_lo = uint32_t(r->prm.seed);
         a.seed_hi = uint32_t(r->prm.seed >> 32);
+        a.nr = r->nr;
         {
             Timed tm(r, K_SHADE);
             hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, a);
@@ -1533,10 +1690,11 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
         {
             Timed tm(r, K_SHADOW);
             hipLaunchKernelGGL((k_shadow<W>), dim3(grid), dim3(kBlock), 0, st, S, r->sq, r->counters, r->L,
-                               r->stats, r->spill, r->spill_lanes);
+                               r->stats, r->spill, r->spill_lanes, r->nr, b);
         }
         cur = 1 - cur;
     }
+    if (r->nr.enabled) nrc_after_chunk(r);
     {
         Timed tm(r, K_OTHER);
         int acc_grid = int(std::min<size_t>((r->npix + kBlock - 1) / kBlock, size_t(grid)));
@@ -1571,6 +1729,10 @@ KR_EXPORT int kr_render_run(kr_render *r, uint32_t spp_begin, uint32_t spp_count
         }
     }
     r->paths += gend - uint64_t(spp_begin) * r->npix;
+    if (r->nr.enabled) {
+        int rc = nrc_after_run(r);
+        if (rc) return rc;
+    }
     r->last_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return KR_OK;
 }
@@ -1637,23 +1799,6 @@ KR_EXPORT int kr_render_radiance(kr_scene *s, const kr_camera *cam, const kr_par
 
 
 // ------------------------------------------------------- radiance cache --
-struct kr_nrc {
-    kr_device *dev = nullptr;
-    kr_nrc_config cfg{};
-    float *master = nullptr, *adam_m = nullptr, *adam_v = nullptr;
-    uint16_t *wt = nullptr, *w = nullptr;
-    float *partial = nullptr;  // [batch / 64][kNumParams]
-    float *loss = nullptr;     // device scalar
-    uint32_t rows = 0;         // batch / 64
-    uint32_t step = 0;         // Adam steps taken
-    uint64_t samples = 0;
-    float last_loss = 0.f;
-    // training pool filled by renders (ring buffer)
-    float *pool_raw = nullptr, *pool_tgt = nullptr;
-    uint32_t *pool_total = nullptr;
-    uint32_t pool_cap = 0;
-    std::vector<void *> allocs;
-};
 
 namespace {
 nrc::Weights nrc_view(const kr_nrc *n) {
@@ -1909,6 +2054,26 @@ KR_EXPORT int kr_nrc_train(kr_nrc *n, const float *raw, const float *target, uin
     return KR_OK;
 }
 
+KR_EXPORT int kr_nrc_read_pool(kr_nrc *n, uint32_t max, float *raw, float *target, uint32_t *count) {
+    if (!n || !count || (max && (!raw || !target))) return fail(KR_E_INVALID, "bad argument");
+    hipStream_t st = n->dev->stream;
+    uint32_t total = 0;
+    KR_HIP(hipMemcpyAsync(&total, n->pool_total, 4, hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
+    uint32_t avail = std::min(total, n->pool_cap);
+    uint32_t cnt = std::min(avail, max);
+    if (cnt) {
+        std::vector<float> t4(size_t(cnt) * 4);
+        KR_HIP(hipMemcpyAsync(raw, n->pool_raw, sizeof(float) * nrc::kRaw * cnt, hipMemcpyDeviceToHost, st));
+        KR_HIP(hipMemcpyAsync(t4.data(), n->pool_tgt, sizeof(float) * 4 * cnt, hipMemcpyDeviceToHost, st));
+        KR_HIP(hipStreamSynchronize(st));
+        for (uint32_t i = 0; i < cnt; ++i)
+            for (int c = 0; c < 3; ++c) target[3 * size_t(i) + c] = t4[4 * size_t(i) + c];
+    }
+    *count = cnt;
+    return KR_OK;
+}
+
 KR_EXPORT int kr_nrc_get_stats(kr_nrc *n, uint64_t *samples_trained, float *last_loss) {
     if (!n) return fail(KR_E_INVALID, "NULL argument");
     if (samples_trained) *samples_trained = n->samples;
@@ -1916,8 +2081,132 @@ KR_EXPORT int kr_nrc_get_stats(kr_nrc *n, uint64_t *samples_trained, float *last
     return KR_OK;
 }
 
+// Cache queries and training-sample extraction after a chunk's bounces.
+__global__ __launch_bounds__(256) void k_nrc_infer(nrc::Weights W, NrcRender nr, const uint32_t *counters, float *L) {
+    using namespace nrc;
+    int lane = __lane_id(), s = lane & 15, q = lane >> 4;
+    uint32_t n = counters[C_NRCQ];
+    uint32_t waves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t base = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * 64; base < n; base += waves * 64) {
+        v4s in[6][4];
+        encode_inputs(in, [&](int k) -> const float * {
+            return base + k < n ? nr.qraw + size_t(base + k) * kRaw : nullptr;
+        }, W);
+        v4f out[4];
+        forward(in, out, W);
+        if (q == 0)
+            for (int nt = 0; nt < 4; ++nt) {
+                uint32_t i = base + nt * 16 + s;
+                if (i < n) {
+                    uint32_t slot = nr.qslot[i];
+                    // a radiance prediction below zero is clamped to zero
+                    L[3 * size_t(slot)] += nr.qbeta[3 * size_t(i)] * fmaxf(out[nt][0], 0.f);
+                    L[3 * size_t(slot) + 1] += nr.qbeta[3 * size_t(i) + 1] * fmaxf(out[nt][1], 0.f);
+                    L[3 * size_t(slot) + 2] += nr.qbeta[3 * size_t(i) + 2] * fmaxf(out[nt][2], 0.f);
+                }
+            }
+    }
+}
+
+// Training targets of the chunk's training paths: for vertex j the radiance
+// the rest of the path carried back to it, (sum_{k>=j} c_k - beta_j E_j) / beta_j,
+// where c_k is everything vertex k added to the path's L.
+__global__ void k_nrc_extract(NrcRender nr, const uint32_t *counters, float *pool_raw, float *pool_tgt,
+                              uint32_t *pool_total, uint32_t cap) {
+    uint32_t ntrain = min(counters[C_TRAIN], nr.train_cap);
+    for (uint32_t ts = blockIdx.x * blockDim.x + threadIdx.x; ts < ntrain; ts += gridDim.x * blockDim.x) {
+        uint32_t nv = nr.tcount[ts];
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        for (int j = int(nv) - 1; j >= 0; --j) {
+            const float *rec = nr.rec + (size_t(ts) * nr.nv + uint32_t(j)) * kRecFloats;
+            sx += rec[19];
+            sy += rec[20];
+            sz += rec[21];
+            float bx = rec[16], by = rec[17], bz = rec[18];
+            if (!(bx > 0.f && by > 0.f && bz > 0.f)) continue;
+            float tx = (sx - rec[22]) / bx, ty = (sy - rec[23]) / by, tz = (sz - rec[24]) / bz;
+            if (!(isfinite(tx) && isfinite(ty) && isfinite(tz))) continue;
+            uint32_t slot = atomicAdd(pool_total, 1u) % cap;
+            float4 *dst = reinterpret_cast<float4 *>(pool_raw + size_t(slot) * nrc::kRaw);
+            dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
+            dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+            dst[2] = make_float4(rec[8], rec[9], rec[10], rec[11]);
+            dst[3] = make_float4(rec[12], 0.f, 0.f, 0.f);
+            reinterpret_cast<float4 *>(pool_tgt)[slot] = make_float4(tx, ty, tz, 0.f);
+        }
+    }
+}
+
+namespace {
+void nrc_after_chunk(kr_render *r) {
+    kr_nrc *n = r->nrc;
+    hipStream_t st = r->scene->dev->stream;
+    {
+        Timed tm(r, K_OTHER);
+        int grid = int(std::min<size_t>((size_t(r->P) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
+        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_view(n), r->nr, r->counters, r->L);
+        int egrid = int(std::min<size_t>((size_t(r->nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
+        hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, r->nr, r->counters, n->pool_raw, n->pool_tgt,
+                           n->pool_total, n->pool_cap);
+    }
+}
+
+int nrc_after_run(kr_render *r) {
+    kr_nrc *n = r->nrc;
+    Timed tm(r, K_OTHER);
+    for (uint32_t i = 0; i < n->cfg.steps_per_frame; ++i) {
+        TrainSource src;
+        src.raw = n->pool_raw;
+        src.target = n->pool_tgt;
+        src.pool_total = n->pool_total;
+        src.n = 0;
+        src.cap = n->pool_cap;
+        src.step = i;
+        src.steps = n->cfg.steps_per_frame;
+        int rc = nrc_step(n, src, n->cfg.batch, n->rows);
+        if (rc) return rc;
+        n->samples += n->cfg.batch;
+    }
+    return KR_OK;
+}
+}  // namespace
+
 KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     if (!r) return fail(KR_E_INVALID, "NULL render");
-    if (n) return fail(KR_E_UNSUPPORTED, "cache integration not built yet");
+    (void)hipSetDevice(r->scene->dev->ordinal);
+    (void)hipStreamSynchronize(r->scene->dev->stream);
+    for (void *p : r->nrc_allocs) (void)hipFree(p);
+    r->nrc_allocs.clear();
+    r->nr = NrcRender{};
+    r->nrc = nullptr;
+    if (!n) return KR_OK;
+    if (n->dev != r->scene->dev) return fail(KR_E_INVALID, "cache and render live on different devices");
+    NrcRender &nr = r->nr;
+    uint32_t log2 = n->cfg.train_log2;
+    nr.train_mask = (log2 >= 32) ? 0xFFFFFFFFu : ((1u << log2) - 1u);
+    nr.rough_threshold = n->cfg.rough_threshold;
+    nr.nv = r->prm.max_bounce + 1;
+    uint64_t expect = uint64_t(r->P) >> log2;
+    nr.train_cap = uint32_t(std::min<uint64_t>(r->P, expect + expect / 8 + 4096));
+    auto get = [&](auto **p, size_t cnt) {
+        int rc = dalloc(p, cnt);
+        if (!rc) r->nrc_allocs.push_back(*p);
+        return rc;
+    };
+    int rc = KR_OK;
+    if (!rc) rc = get(&nr.tslot_of, r->P);
+    if (!rc) rc = get(&nr.rec, size_t(nr.train_cap) * nr.nv * kRecFloats);
+    if (!rc) rc = get(&nr.tcount, nr.train_cap);
+    if (!rc) rc = get(&nr.qslot, r->P);
+    if (!rc) rc = get(&nr.qbeta, 3 * size_t(r->P));
+    if (!rc) rc = get(&nr.qraw, size_t(r->P) * nrc::kRaw);
+    if (rc) {
+        for (void *p : r->nrc_allocs) (void)hipFree(p);
+        r->nrc_allocs.clear();
+        r->nr = NrcRender{};
+        return rc;
+    }
+    nr.enabled = 1;
+    r->nrc = n;
     return KR_OK;
 }

@@ -99,7 +99,7 @@ EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "
            "kr_render_reset", "kr_render_read", "kr_render_get_stats", "kr_render_free",
            "kr_render_radiance", "kr_render_set_profiling", "kr_device_bandwidth", "kr_nrc_create", "kr_nrc_free",
            "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
-           "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc"]
+           "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc", "kr_nrc_read_pool"]
 
 _lib = None
 
@@ -144,6 +144,7 @@ def load(path: str = LIB_PATH):
     lib.kr_nrc_train.argtypes = [vp, vp, vp, u32, u32, C.POINTER(f32)]
     lib.kr_nrc_get_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(f32)]
     lib.kr_render_attach_nrc.argtypes = [vp, vp]
+    lib.kr_nrc_read_pool.argtypes = [vp, u32, vp, vp, C.POINTER(u32)]
     for name in EXPORTS:
         if name not in ("kr_version",):
             getattr(lib, name).restype = C.c_int
@@ -346,6 +347,13 @@ class Nrc:
         check(self.lib.kr_nrc_train(self.h, _ptr(raw), _ptr(tgt), raw.shape[0], steps, C.byref(loss)),
               "kr_nrc_train")
         return loss.value
+
+    def read_pool(self, max_samples):
+        raw = np.zeros((max_samples, NRC_RAW), np.float32)
+        tgt = np.zeros((max_samples, 3), np.float32)
+        cnt = u32()
+        check(self.lib.kr_nrc_read_pool(self.h, max_samples, _ptr(raw), _ptr(tgt), C.byref(cnt)), "kr_nrc_read_pool")
+        return raw[:cnt.value], tgt[:cnt.value]
 
     def stats(self):
         n, l = u64(), f32()

@@ -302,6 +302,10 @@ int kr_nrc_gradient(kr_nrc *nrc, const float *raw, const float *target, uint32_t
 /* `steps` Adam steps, each on all n records; loss of the last step. */
 int kr_nrc_train(kr_nrc *nrc, const float *raw, const float *target, uint32_t n, uint32_t steps,
                  float *loss);
+/* Copies up to `max` of the training samples renders have gathered (vertex
+ * records, max x KR_NRC_RAW floats; targets, max x 3) in ring order; *count
+ * receives how many were copied. */
+int kr_nrc_read_pool(kr_nrc *nrc, uint32_t max, float *raw, float *target, uint32_t *count);
 /* Samples trained on so far, and the mean loss of the last training step. */
 int kr_nrc_get_stats(kr_nrc *nrc, uint64_t *samples_trained, float *last_loss);
 
