@@ -103,8 +103,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: this build has no multi-GPU path (paths are independent; "
-              "sharding by pixel tiles or sample ranges is described in DESIGN.md). Run with --gpus 1.",
+        print("bench.py measures one MI355X: there is no multi-GPU path (paths are independent and would "
+              "shard by sample range with one FP64 sum-reduce, which is not built). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 

@@ -1891,7 +1891,14 @@ KR_EXPORT int kr_nrc_create(kr_device *dev, const kr_nrc_config *cfg, kr_nrc **o
     n->cfg = *cfg;
     if (!n->cfg.batch) n->cfg.batch = 16384;
     if (!n->cfg.steps_per_frame) n->cfg.steps_per_frame = 4;
-    if (!n->cfg.train_log2) n->cfg.train_log2 = 4;
+    if (!n->cfg.train_log2)
+        n->cfg.train_log2 = 4;
+    else if (n->cfg.train_log2 == KR_NRC_TRAIN_ALL)
+        n->cfg.train_log2 = 0;
+    else if (n->cfg.train_log2 > 31) {
+        delete n;
+        return fail(KR_E_INVALID, "train_log2 must be at most 31 (or KR_NRC_TRAIN_ALL)");
+    }
     if (n->cfg.rough_threshold <= 0.f) n->cfg.rough_threshold = 0.3f;
     n->cfg.batch = (n->cfg.batch + 63) / 64 * 64;
     n->rows = n->cfg.batch / 64;
@@ -2077,7 +2084,12 @@ KR_EXPORT int kr_nrc_read_pool(kr_nrc *n, uint32_t max, float *raw, float *targe
 KR_EXPORT int kr_nrc_get_stats(kr_nrc *n, uint64_t *samples_trained, float *last_loss) {
     if (!n) return fail(KR_E_INVALID, "NULL argument");
     if (samples_trained) *samples_trained = n->samples;
-    if (last_loss) *last_loss = n->last_loss;
+    if (last_loss) {
+        // the loss of the last step, whether kr_nrc_train or a render ran it
+        KR_HIP(hipMemcpyAsync(&n->last_loss, n->loss, sizeof(float), hipMemcpyDeviceToHost, n->dev->stream));
+        KR_HIP(hipStreamSynchronize(n->dev->stream));
+        *last_loss = n->last_loss;
+    }
     return KR_OK;
 }
 
@@ -2183,7 +2195,7 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     if (n->dev != r->scene->dev) return fail(KR_E_INVALID, "cache and render live on different devices");
     NrcRender &nr = r->nr;
     uint32_t log2 = n->cfg.train_log2;
-    nr.train_mask = (log2 >= 32) ? 0xFFFFFFFFu : ((1u << log2) - 1u);
+    nr.train_mask = (1u << log2) - 1u;  // log2 <= 31 (kr_nrc_create)
     nr.rough_threshold = n->cfg.rough_threshold;
     nr.nv = r->prm.max_bounce + 1;
     uint64_t expect = uint64_t(r->P) >> log2;

@@ -86,6 +86,7 @@ KR_PROFILE_EVENTS = 0x1
 KR_PROFILE_COUNTERS = 0x2
 
 NRC_RAW, NRC_FEATURES, NRC_PARAMS = 16, 96, 23888
+NRC_TRAIN_ALL = 0xFFFFFFFF  # Nrc(train_log2=...): every path trains
 
 
 class NrcConfig(C.Structure):

@@ -276,13 +276,16 @@ int kr_render_radiance(kr_scene *scene, const kr_camera *camera, const kr_params
 
 typedef struct kr_nrc kr_nrc;
 
+#define KR_NRC_TRAIN_ALL 0xFFFFFFFFu /* kr_nrc_config.train_log2: every path trains */
+
 typedef struct kr_nrc_config {
     float box_min[3], box_max[3]; /* scene box for the position encoding      */
     float learning_rate;          /* Adam step size; 0 = 1e-2                  */
     uint32_t seed;                /* weight initialisation                     */
     uint32_t batch;               /* training samples per step; 0 = 16384      */
     uint32_t steps_per_frame;     /* Adam steps after each render run; 0 = 4   */
-    uint32_t train_log2;          /* 1 / 2^train_log2 of paths train; 0 = 4    */
+    uint32_t train_log2;          /* 1 / 2^train_log2 of paths train (at most
+                                     31); 0 = 4; KR_NRC_TRAIN_ALL: every path  */
     float rough_threshold;        /* GlossyBSDF roughness counted as rough;
                                      0 = 0.3                                   */
 } kr_nrc_config;

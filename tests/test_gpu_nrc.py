@@ -130,7 +130,7 @@ def test_all_training_paths_leave_the_image_unchanged(device):
         r = kr.Render(gs, s.camera_struct(), p)
         n = None
         if with_cache:
-            n = kr.Nrc(device, *_box(s), train_log2=32)
+            n = kr.Nrc(device, *_box(s), train_log2=kr.NRC_TRAIN_ALL)
             r.attach_nrc(n)
         r.run(0, p.spp)
         r.sync()
@@ -160,7 +160,7 @@ def test_training_targets_are_path_radiance(device):
     gs = kr.GpuScene(device, s.to_desc())
     p = s.params(seed=3)
     r = kr.Render(gs, s.camera_struct(), p)
-    n = kr.Nrc(device, *_box(s), train_log2=32, steps_per_frame=1)
+    n = kr.Nrc(device, *_box(s), train_log2=kr.NRC_TRAIN_ALL, steps_per_frame=1)
     r.attach_nrc(n)
     r.run(0, 1)
     r.sync()
@@ -189,7 +189,7 @@ def test_cached_render_learns_and_stays_close(device):
     plain.sync()
     prgb, pval = plain.read()
     plain.free()
-    n = kr.Nrc(device, *_box(s))
+    n = kr.Nrc(device, *_box(s), train_log2=5)
     r = kr.Render(gs, s.camera_struct(), p)
     r.attach_nrc(n)
     losses = []
@@ -200,7 +200,7 @@ def test_cached_render_learns_and_stays_close(device):
         losses.append(n.stats()[1])
     crgb, cval = r.read()
     assert (cval > 0).all() and np.isfinite(crgb).all()
-    assert losses[-1] < losses[0]
+    assert np.mean(losses[-3:]) < losses[0], losses
     pm = (prgb / np.maximum(pval, 1)[:, None]).mean()
     cm = (crgb / np.maximum(cval, 1)[:, None]).mean()
     assert abs(cm - pm) < 0.1 * pm, (cm, pm)
