@@ -89,14 +89,138 @@ def rmse_vs_oracle(workload, rgb, valid, width):
             "pixels": int(pix.size), "oracle_samples_per_pixel": int(f["seeds"]) * int(f["spp"])}
 
 
+def bench_batch(args, kr, sg, barrier):
+    """Batched uneven workload: 16 seeded scenes of 10k-2M triangles, mixed
+    resolutions and 16-256 spp, rendered as one wavefront from one captured
+    HIP graph after the autotuner picked BVH width, chunk size and blocks
+    per CU. One step = every sample of every job."""
+    import numpy as np
+    log("[bench] building batch workload")
+    scenes, specs = sg.batch_workload()
+    jobs = [(sp["scene"], scenes[sp["scene"]].camera_struct(),
+             kr.Params(sp["width"], sp["height"], sp["spp"], sp["max_bounce"], sp["seed"], 0, 0)) for sp in specs]
+    total = sum(p.width * p.height * p.spp for _, _, p in jobs)
+    ntris = sum(int(np.asarray(s.indices).shape[0]) for s in scenes)
+    dev = kr.Device(0)
+    b = kr.Batch(dev, scenes, jobs)
+    t0 = time.perf_counter()
+    tun = b.autotune()
+    tune_s = time.perf_counter() - t0
+    log(f"[bench] autotune {tune_s:.1f}s: BVH{tun.bvh_width}, {tun.chunk_paths} paths/chunk, "
+        f"{tun.blocks_per_cu} blocks/CU ({tun.paths_per_second / 1e6:.0f} M paths/s in trials)")
+    for i in range(args.warmup):
+        b.reset()
+        b.run()
+        b.sync()
+        log(f"[bench] warmup {i + 1}/{args.warmup} done ({b.info().seconds:.3f}s device)")
+    info = b.info()
+    log(f"[bench] graph: {info.graph_nodes} kernel nodes over {info.chunks} chunks, captured in "
+        f"{info.capture_seconds:.2f}s")
+    barrier()
+    elapsed = 0.0
+    for i in range(args.steps):
+        b.reset()
+        barrier()
+        t1 = time.perf_counter()
+        b.run()
+        b.sync()
+        barrier()
+        elapsed += time.perf_counter() - t1
+        log(f"[bench] step {i + 1}/{args.steps}: {b.info().seconds:.3f}s device")
+    value = total * args.steps / elapsed
+
+    # the same jobs one by one (plain renders at their default settings, BVH4),
+    # with per-kernel events and a 1-spp counters pass for the traversal roofline
+    log("[bench] jobs one by one ...")
+    seq = 0.0
+    trace_ms = trace_launches = closest = 0
+    nodes = tris = cl_cnt = 0
+    node_bytes = 0
+    ups = [kr.GpuScene(dev, s.to_desc(bvh_width=4)) for s in scenes]
+    for k, cam, p in jobs:
+        r = kr.Render(ups[k], cam, p)
+        r.set_profiling(kr.KR_PROFILE_COUNTERS)
+        r.run(0, 1)
+        r.sync()
+        st = r.stats()
+        nodes += st.node_visits
+        tris += st.tri_tests
+        cl_cnt += st.closest_rays
+        node_bytes = ups[k].info().node_bytes
+        r.reset()
+        r.set_profiling(kr.KR_PROFILE_EVENTS)
+        barrier()
+        t1 = time.perf_counter()
+        r.run(0, p.spp)
+        r.sync()
+        barrier()
+        seq += time.perf_counter() - t1
+        st = r.stats()
+        trace_ms += st.trace_ms
+        trace_launches += st.trace_launches
+        closest += st.closest_rays
+        r.free()
+    for u in ups:
+        u.free()
+    bytes_per_query = QUEUE_BYTES_PER_QUERY + (nodes * node_bytes + tris * TRI_BYTES) / max(1, cl_cnt)
+    achieved = closest * bytes_per_query / (trace_ms * 1e-3) / 1e9 if trace_ms > 0 else 0.0
+    log(f"[bench] one by one: {seq:.3f}s ({total / seq / 1e6:.0f} M paths/s); batched {elapsed / args.steps:.3f}s")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(scenes[specs[0]["scene"]], args.cpu_seconds)
+        cpu["sample"] = "job 0 of the batch: " + cpu["sample"]
+        log(f"[bench] cpu baseline {cpu['value']:.0f} paths/s on {cpu['cores']} threads")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "paths/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batched uneven workload: 16 seeded scenes (10k-2M triangles, 6.25 M in total), "
+                        "resolutions 256^2..1920x1080, 16-256 spp, maxBounce 8, one graph launch",
+            "jobs": len(jobs), "triangles": ntris, "paths_per_step": total,
+            "tuning": tun.as_dict(), "autotune_seconds": tune_s,
+            "graph_nodes": info.graph_nodes, "chunks": info.chunks, "capture_seconds": info.capture_seconds,
+            "one_by_one_seconds": seq, "speedup_vs_one_by_one": seq / (elapsed / args.steps),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_trace (closest-hit wide-BVH traversal), timed on the jobs rendered one by one",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_query": bytes_per_query,
+            "queries_per_launch": closest / max(1, trace_launches),
+            "avg_launch_ms": trace_ms / max(1, trace_launches),
+        },
+        "cpu_baseline": cpu,
+    }
+    b.free()
+    dev.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mesh", choices=["mesh", "spheres", "large"])
+    ap.add_argument("--workload", default="mesh", choices=["mesh", "spheres", "large", "batch"])
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--chunk-paths", type=int, default=0)
+    ap.add_argument("--full-nodes", action="store_true", help="FP32 BVH child boxes instead of 8-bit quantised")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-torch", action="store_true", help="skip torch.cuda.synchronize (profiling runs)")
@@ -119,10 +243,14 @@ def main():
         if torch is not None:
             torch.cuda.synchronize()
 
+    if args.workload == "batch":
+        return bench_batch(args, kr, sg, barrier)
+
     log(f"[bench] building workload {args.workload}")
     scene = sg.WORKLOADS[args.workload]()
     dev = kr.Device(0)
-    gs = kr.GpuScene(dev, scene.to_desc(bvh_width=args.bvh_width))
+    gs = kr.GpuScene(dev, scene.to_desc(bvh_width=args.bvh_width,
+                                        flags=kr.KR_SCENE_FULL_NODES if args.full_nodes else 0))
     info = gs.info()
     log(f"[bench] scene: {info.num_triangles} tris, {info.num_spheres} spheres, {info.num_lights} lights, "
         f"BVH{info.bvh_width} {info.bvh_nodes} nodes depth {info.bvh_depth}, build {info.build_seconds:.2f}s")
@@ -167,7 +295,9 @@ def main():
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_trace_mesh_traffic.json")
-    if os.path.exists(tpath):
+    # the committed counter summary is for the default mesh frame only
+    if (args.workload == "mesh" and info.bvh_width == 4 and not args.chunk_paths and not args.full_nodes
+            and os.path.exists(tpath)):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 
@@ -198,7 +328,8 @@ def main():
                          "large": "large scene: 40 displaced icosphere instances 1,036,800 tris, 4 quad "
                                   "lights, 1920x1080, 64 spp, maxBounce 8"}[args.workload],
             "width": p.width, "height": p.height, "spp": p.spp, "max_bounce": p.max_bounce,
-            "triangles": info.num_triangles, "bvh_width": info.bvh_width, "paths_per_step": paths_per_step,
+            "triangles": info.num_triangles, "bvh_width": info.bvh_width, "bvh_node_bytes": info.node_bytes,
+            "paths_per_step": paths_per_step,
             "mrays_per_s": (st.closest_rays + st.shadow_rays) / elapsed / 1e6,
             "rmse_vs_fp64_oracle": rmse,
             "wall_clock_to_rmse_s": elapsed / args.steps,

@@ -263,4 +263,81 @@ bool build_wide_bvh(const float *tri_verts, uint32_t num_tris, uint32_t width, u
     return true;
 }
 
+
+namespace {
+float qscale(int e) { return std::ldexp(1.0f, e); }
+float qdecode(uint32_t q, float s, float p) { return std::fmaf(float(q), s, p); }
+}  // namespace
+
+void compress_wide_nodes(const WideBvh &bvh, std::vector<uint32_t> &out) {
+    const uint32_t W = bvh.width, NF = node_floats(W), NW = qnode_bytes(W) / 4;
+    out.assign(size_t(bvh.num_nodes) * NW, 0u);
+    for (uint32_t n = 0; n < bvh.num_nodes; ++n) {
+        const float *f = bvh.nodes.data() + size_t(n) * NF;
+        uint32_t *o = out.data() + size_t(n) * NW;
+        const uint32_t *ref = reinterpret_cast<const uint32_t *>(f + 6 * W);
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::numeric_limits<float>::infinity();
+            hi[a] = -std::numeric_limits<float>::infinity();
+        }
+        for (uint32_t c = 0; c < W; ++c) {
+            if (ref[c] == kEmpty) continue;
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], f[(2 * a) * W + c]);
+                hi[a] = std::max(hi[a], f[(2 * a + 1) * W + c]);
+            }
+        }
+        int e[3];
+        float sc[3];
+        for (int a = 0; a < 3; ++a) {
+            if (!(lo[a] <= hi[a])) lo[a] = hi[a] = 0.f;  // no children (not produced by the builder)
+            // 254 quanta cover the extent, leaving one for the round-up
+            double ext = double(hi[a]) - double(lo[a]);
+            int ea = ext > 0 ? int(std::ceil(std::log2(ext / 254.0))) : -125;
+            // one quantum at least one ulp of the coordinates, so that
+            // fma(q, s, p) can reach every float in the box and q = 255
+            // still lies beyond hi after rounding
+            float m = std::max(std::fabs(lo[a]), std::fabs(hi[a]));
+            float ulp = std::nextafter(m, std::numeric_limits<float>::infinity()) - m;
+            if (ulp > 0) ea = std::max(ea, int(std::ceil(std::log2(double(ulp)))));
+            ea = std::max(-125, std::min(125, ea));
+            while (ea < 125 && double(qscale(ea)) * 254.0 < ext) ++ea;
+            e[a] = ea;
+            sc[a] = qscale(ea);
+            std::memcpy(&o[a], &lo[a], 4);
+        }
+        o[3] = uint32_t(uint8_t(int8_t(e[0]))) | uint32_t(uint8_t(int8_t(e[1]))) << 8 |
+               uint32_t(uint8_t(int8_t(e[2]))) << 16;
+        for (uint32_t c = 0; c < W; ++c) o[4 + c] = ref[c];
+        uint8_t *q = reinterpret_cast<uint8_t *>(o + 4 + W);
+        for (uint32_t c = 0; c < W; ++c) {
+            if (ref[c] == kEmpty) continue;
+            for (int a = 0; a < 3; ++a) {
+                float cl = f[(2 * a) * W + c], ch = f[(2 * a + 1) * W + c];
+                double rl = std::floor((double(cl) - double(lo[a])) / double(sc[a]));
+                double rh = std::ceil((double(ch) - double(lo[a])) / double(sc[a]));
+                int ql = int(std::max(0.0, std::min(255.0, rl)));
+                int qh = int(std::max(0.0, std::min(255.0, rh)));
+                while (ql > 0 && qdecode(uint32_t(ql), sc[a], lo[a]) > cl) --ql;
+                while (qh < 255 && qdecode(uint32_t(qh), sc[a], lo[a]) < ch) ++qh;
+                q[(2 * a) * W + c] = uint8_t(ql);
+                q[(2 * a + 1) * W + c] = uint8_t(qh);
+            }
+        }
+    }
+}
+
+void decode_qnode_box(const uint32_t *node, uint32_t W, uint32_t child, float lo[3], float hi[3]) {
+    const uint8_t *q = reinterpret_cast<const uint8_t *>(node + 4 + W);
+    for (int a = 0; a < 3; ++a) {
+        float p;
+        std::memcpy(&p, &node[a], 4);
+        int e = int(int8_t(uint8_t(node[3] >> (8 * a))));
+        float s = qscale(e);
+        lo[a] = qdecode(q[(2 * a) * W + child], s, p);
+        hi[a] = qdecode(q[(2 * a + 1) * W + child], s, p);
+    }
+}
+
 }  // namespace kr

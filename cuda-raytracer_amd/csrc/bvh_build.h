@@ -19,4 +19,11 @@ struct WideBvh {
 bool build_wide_bvh(const float *tri_verts, uint32_t num_tris, uint32_t width, uint32_t leaf_size, WideBvh &out,
                     std::string &err);
 
+// Re-encodes the nodes of `bvh` in the compressed layout (qnode_bytes(width)
+// bytes each, conservative 8-bit child boxes).
+void compress_wide_nodes(const WideBvh &bvh, std::vector<uint32_t> &out);
+
+// Decoded child box of a compressed node, exactly as the kernels decode it.
+void decode_qnode_box(const uint32_t *node, uint32_t width, uint32_t child, float lo[3], float hi[3]);
+
 }  // namespace kr

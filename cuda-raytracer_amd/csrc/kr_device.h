@@ -252,12 +252,54 @@ __device__ __forceinline__ void load_node(const float4 *nodes, uint32_t idx, Nod
     }
 }
 
+// Compressed node (qnode_bytes(W) bytes, layout in kr_layout.h): decoded
+// with the same fma(q, s, p) the encoder used to make the boxes conservative.
+template <int W>
+__device__ __forceinline__ void load_qnode(const float4 *nodes, uint32_t idx, NodeView<W> &nv) {
+    constexpr int NU4 = int(qnode_bytes(W)) / 16;
+    const uint4 *p = reinterpret_cast<const uint4 *>(nodes) + size_t(idx) * NU4;
+    uint32_t u[NU4 * 4];
+#pragma unroll
+    for (int i = 0; i < NU4; ++i) {
+        uint4 q = p[i];
+        u[4 * i + 0] = q.x;
+        u[4 * i + 1] = q.y;
+        u[4 * i + 2] = q.z;
+        u[4 * i + 3] = q.w;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float org = __uint_as_float(u[a]);
+        int e = int(int8_t(uint8_t(u[3] >> (8 * a))));
+        float sc = __uint_as_float(uint32_t(e + 127) << 23);
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            constexpr int QB = 4 * (4 + W);  // first quantised byte
+            int bl = QB + (2 * a) * W + c, bh = QB + (2 * a + 1) * W + c;
+            uint32_t ql = (u[bl >> 2] >> (8 * (bl & 3))) & 0xFFu;
+            uint32_t qh = (u[bh >> 2] >> (8 * (bh & 3))) & 0xFFu;
+            nv.lo[a][c] = fmaf(float(ql), sc, org);
+            nv.hi[a][c] = fmaf(float(qh), sc, org);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) nv.ref[c] = u[4 + c];
+}
+
+template <int W, bool Q>
+__device__ __forceinline__ void fetch_node(const float4 *nodes, uint32_t idx, NodeView<W> &nv) {
+    if (Q)
+        load_qnode<W>(nodes, idx, nv);
+    else
+        load_node<W>(nodes, idx, nv);
+}
+
 // Closest hit over the wide BVH and the sphere list: ordered traversal,
 // nearest child first, far children pushed with their entry distance and
 // culled on pop. Equivalent to the reference's primitive loop + per-mesh DFS
 // (src/core/meta/scene.hpp:35-47, src/core/primitive/mesh.cu:200-267) up to
 // exact ties.
-template <int W, int LDS_DEPTH>
+template <int W, int LDS_DEPTH, bool Q = false>
 __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, Stack<LDS_DEPTH> &st,
                                                 uint64_t *counts = nullptr) {
     HitRec h;
@@ -284,7 +326,7 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
     while (true) {
         if (!(cur & kLeafBit)) {
             NodeView<W> nv;
-            load_node<W>(S.nodes, cur, nv);
+            fetch_node<W, Q>(S.nodes, cur, nv);
             ++nvis;
             float key[W];
             uint32_t ref[W];
@@ -357,7 +399,7 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
 // Any hit on the segment (o, d, tmax): Scene::intersect(Seg)
 // (src/core/meta/scene.hpp:57-67). Triangles block for 0 < t <= tmax,
 // spheres for a ray hit with t <= tmax (src/core/primitive/primitive.hpp:16-21).
-template <int W, int LDS_DEPTH>
+template <int W, int LDS_DEPTH, bool Q = false>
 __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, float tmax, Stack<LDS_DEPTH> &st) {
     for (uint32_t s = 0; s < S.nspheres; ++s) {
         double ts;
@@ -373,7 +415,7 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
     while (true) {
         if (!(cur & kLeafBit)) {
             NodeView<W> nv;
-            load_node<W>(S.nodes, cur, nv);
+            fetch_node<W, Q>(S.nodes, cur, nv);
             uint32_t next = kEmpty;
 #pragma unroll
             for (int c = 0; c < W; ++c) {

@@ -27,6 +27,19 @@ constexpr uint32_t kMissCode = 0xFFFFFFFFu;
 
 constexpr uint32_t node_floats(uint32_t w) { return (7 * w + 3) / 4 * 4; }
 
+// Compressed wide node (the default): child boxes quantised to 8 bits per
+// plane against the node's own box, as in Ylitie, Karras and Laine, "Efficient
+// Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs" (HPG 2017):
+//   word 0..2  origin p (float xyz; the node box's lower corner)
+//   word 3     per-axis scale exponents e (int8 x 3): scale s_a = 2^e_a
+//   word 4..   child refs (W x uint32, same encoding as above)
+//   then       6W bytes: lo_x[W] hi_x[W] lo_y[W] hi_y[W] lo_z[W] hi_z[W]
+// A child's box is lo = fma(q_lo, s, p), hi = fma(q_hi, s, p) in FP32; the
+// encoder rounds q_lo down and q_hi up until the decoded box contains the
+// exact one, so traversal only ever visits a superset of nodes. 64 B for
+// W = 4 (128 B uncompressed), 96 B for W = 8 (224 B).
+constexpr uint32_t qnode_bytes(uint32_t w) { return (16 + 4 * w + 6 * w + 15) / 16 * 16; }
+
 inline uint32_t make_leaf(uint32_t first, uint32_t count) {
     return kLeafBit | ((count - 1) << kLeafCountShift) | first;
 }

@@ -50,6 +50,8 @@ int fail(int code, const std::string &msg) {
 
 constexpr int kBlock = 256;
 constexpr int kLdsDepth = 12;
+// traversal stack entries per lane beyond the LDS part, in HBM
+constexpr int kSpillDepth = 96;
 
 // Host Philox4x32-10 for the light tables (same function as the device one).
 void philox_host(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
@@ -108,6 +110,7 @@ struct kr_scene {
     kr_device *dev = nullptr;
     DevScene view{};
     uint32_t width = 4;
+    bool qnodes = true;  // compressed nodes (qnode_bytes) or full FP32 boxes
     std::vector<void *> allocs;
     // host copies needed to answer queries
     std::vector<kr_prim_desc> prims;
@@ -130,7 +133,8 @@ struct RayQ {
 // Radiance-cache state of a render (all null / 0 when no cache is attached).
 struct NrcRender {
     int enabled;
-    uint32_t train_mask;   // a path trains when (hash & train_mask) == 0
+    uint32_t train_log2;   // one path in every group of 2^train_log2 consecutive paths trains
+    uint32_t train_mask;   // 2^train_log2 - 1
     float rough_threshold; // GlossyBSDF roughness counted as rough
     uint32_t *tslot_of;    // slot -> training-path index or kMissCode
     float *rec;            // training vertex records [train_cap][nv][kRec]
@@ -150,6 +154,23 @@ struct HitQ {
 struct LensDev {
     float n[3], u[3], v[3], du[3], dv[3], o[3];
     uint32_t lens, pw, w, h;
+};
+
+// One job of a batch, as the kernels see it. A batch's path space is the
+// concatenation of its jobs' sample spaces: job k owns global path indices
+// [begin, begin + npix * spp), local index = s * npix + pixel as in a
+// single render, so a job's paths, RNG streams and pixels are those of the
+// same render done alone.
+struct JobRec {
+    DevScene S;
+    LensDev lens;
+    uint64_t begin;
+    uint32_t npix, max_bounce, seed_lo, seed_hi;
+};
+struct JobTable {
+    const JobRec *rec;
+    const uint64_t *begin;  // begin of every job, ascending
+    uint32_t n;
 };
 
 // counters layout
@@ -238,10 +259,40 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
     return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// Reserves n consecutive entries of *counter for every lane of the wave with
+// one atomic; returns the lane's first entry (lanes in order).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *counter, uint32_t n) {
+    uint32_t lane = __lane_id();
+    uint32_t incl = n;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(incl, o);
+        if (lane >= uint32_t(o)) incl += y;
+    }
+    uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + incl - n;
+}
+
 __device__ __forceinline__ void wave_count(unsigned long long *counter, bool pred) {
     unsigned long long mask = __ballot(pred);
     if (mask && __lane_id() == uint32_t(__ffsll((unsigned long long)mask) - 1))
         atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
+// Job owning global path g: the last k with begin[k] <= g.
+__device__ __forceinline__ uint32_t find_job(const JobTable &T, uint64_t g) {
+    uint32_t lo = 0, hi = T.n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (T.begin[mid] <= g)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
 }
 
 template <int LDS>
@@ -254,12 +305,12 @@ __device__ __forceinline__ Stack<LDS> make_stack(uint32_t *lds_ref, float *lds_t
     size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     st.gstride = uint32_t(spill_lanes);
     st.spill_ref = spill + g;
-    st.spill_t = reinterpret_cast<float *>(spill + spill_lanes * 64) + g;
+    st.spill_t = reinterpret_cast<float *>(spill + spill_lanes * kSpillDepth) + g;
     return st;
 }
 
 // Ray-query entry point: closest hit of n rays (parity hook).
-template <int W>
+template <int W, bool Q>
 __global__ __launch_bounds__(kBlock) void k_query_closest(DevScene S, const float *o, const float *d, uint32_t n,
                                                           float *t, uint32_t *prim, uint32_t *tri, float *u, float *v,
                                                           uint32_t *spill, size_t spill_lanes, uint64_t *visits) {
@@ -270,7 +321,7 @@ __global__ __launch_bounds__(kBlock) void k_query_closest(DevScene S, const floa
         f3 ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
         f3 rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
         uint64_t cnt[2] = {0, 0};
-        HitRec h = trace_closest<W, kLdsDepth>(S, ro, rd, st, visits ? cnt : nullptr);
+        HitRec h = trace_closest<W, kLdsDepth, Q>(S, ro, rd, st, visits ? cnt : nullptr);
         if (visits) {
             visits[2 * i] = cnt[0];
             visits[2 * i + 1] = cnt[1];
@@ -294,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_query_closest(DevScene S, const floa
     }
 }
 
-template <int W>
+template <int W, bool Q>
 __global__ __launch_bounds__(kBlock) void k_query_shadow(DevScene S, const float *o, const float *d, const float *tmax,
                                                          uint32_t n, uint8_t *occ, uint32_t *spill,
                                                          size_t spill_lanes) {
@@ -304,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_query_shadow(DevScene S, const float
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         f3 ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
         f3 rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
-        occ[i] = trace_occluded<W, kLdsDepth>(S, ro, rd, tmax[i], st) ? 1 : 0;
+        occ[i] = trace_occluded<W, kLdsDepth, Q>(S, ro, rd, tmax[i], st) ? 1 : 0;
     }
 }
 
@@ -320,16 +371,40 @@ __global__ void k_reset(uint32_t *counters, uint32_t mask) {
     if (i < C_NUM && ((mask >> i) & 1u)) counters[i] = 0;
 }
 
-// Camera rays of chunk [g0, g0 + P): slot j is path g = g0 + j, sample
-// g / npix of pixel g % npix. PinholeLens/OrthographicLens::sample
-// (src/core/misc/lens.hpp:145-164). With a cache attached, one path in
-// train_mask + 1 (picked by a Philox hash of (pixel, sample)) becomes a
-// training path and gets a record slot.
+// With a cache attached, the global path indices fall into groups of
+// 2^train_log2 consecutive paths (consecutive pixels of one sample); in each
+// group the member picked by a Philox hash of the group index is a training
+// path, and its record slot is the group's index within the chunk, so no
+// atomics are needed to hand out slots.
 constexpr uint32_t kStreamTrainPick = 0x5452414Eu;
 constexpr int kRecFloats = 28;  // vertex record: raw 16 | beta 3 | c 3 | beta*E 3 | pad 3
 
-__global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, uint64_t g0, uint32_t P, uint32_t npix, RayQ q,
-                                                   float *L, uint32_t *counters, int qi, NrcRender nr,
+// PinholeLens / OrthographicLens::sample (src/core/misc/lens.hpp:55-67,
+// 40-53) for sample s of pixel px.
+__device__ __forceinline__ void camera_ray(const LensDev &lens, uint32_t s, uint32_t px, f3 &o, f3 &d) {
+    uint32_t x = px % lens.w, y = px / lens.w;
+    float fx = float(x) - float(lens.w) * 0.5f, fy = float(y) - float(lens.h) * 0.5f;
+    float kx = float(s % lens.pw), ky = float(s / lens.pw);
+    f3 n{lens.n[0], lens.n[1], lens.n[2]}, u{lens.u[0], lens.u[1], lens.u[2]};
+    f3 v{lens.v[0], lens.v[1], lens.v[2]};
+    f3 du{lens.du[0], lens.du[1], lens.du[2]}, dv{lens.dv[0], lens.dv[1], lens.dv[2]};
+    o = f3{lens.o[0], lens.o[1], lens.o[2]};
+    if (lens.lens == KR_LENS_ORTHOGRAPHIC) {
+        f3 dd = u * fx + v * fy;
+        dd = dd + (du * kx + dv * ky);
+        o = o + dd * 4.f / float(lens.w);
+        d = normalize(n);
+    } else {
+        f3 t = n + u * fx + v * fy;
+        d = normalize(t + du * kx + dv * ky);
+    }
+}
+
+// Camera paths of a chunk: slot j is global path g0 + j. BATCH: the path's
+// job (and its lens and local index) comes from the job table.
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, JobTable T, uint64_t g0, uint32_t P, uint32_t npix,
+                                                   RayQ q, float *L, uint32_t *counters, int qi, NrcRender nr,
                                                    uint32_t seed_lo, uint32_t seed_hi) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += stride) {
@@ -338,23 +413,18 @@ __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, uint64_t g0, ui
         bool train = false;
         if (active) {
             uint64_t g = g0 + j;
-            uint32_t s = uint32_t(g / npix), px = uint32_t(g % npix);
-            uint32_t x = px % lens.w, y = px / lens.w;
-            float fx = float(x) - float(lens.w) * 0.5f, fy = float(y) - float(lens.h) * 0.5f;
-            float kx = float(s % lens.pw), ky = float(s / lens.pw);
-            f3 n{lens.n[0], lens.n[1], lens.n[2]}, u{lens.u[0], lens.u[1], lens.u[2]};
-            f3 v{lens.v[0], lens.v[1], lens.v[2]};
-            f3 du{lens.du[0], lens.du[1], lens.du[2]}, dv{lens.dv[0], lens.dv[1], lens.dv[2]};
-            f3 o{lens.o[0], lens.o[1], lens.o[2]};
-            f3 d;
-            if (lens.lens == KR_LENS_ORTHOGRAPHIC) {
-                f3 dd = u * fx + v * fy;
-                dd = dd + (du * kx + dv * ky);
-                o = o + dd * 4.f / float(lens.w);
-                d = normalize(n);
+            f3 o, d;
+            uint32_t s, px;
+            if (BATCH) {
+                const JobRec &J = T.rec[find_job(T, g)];
+                uint64_t l = g - J.begin;
+                s = uint32_t(l / J.npix);
+                px = uint32_t(l % J.npix);
+                camera_ray(J.lens, s, px, o, d);
             } else {
-                f3 t = n + u * fx + v * fy;
-                d = normalize(t + du * kx + dv * ky);
+                s = uint32_t(g / npix);
+                px = uint32_t(g % npix);
+                camera_ray(lens, s, px, o, d);
             }
             q.slot[j] = j;
             q.ox[j] = o.x;
@@ -369,31 +439,35 @@ __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, uint64_t g0, ui
             L[3 * j] = 0.f;
             L[3 * j + 1] = 0.f;
             L[3 * j + 2] = 0.f;
-            if (nr.enabled) {
-                Philox4 h = philox(px, s, 0xFFFFFFFFu, kStreamTrainPick, seed_lo, seed_hi);
-                train = (h.v[0] & nr.train_mask) == 0u;
-            }
         }
         uint32_t flags = 0;
-        if (nr.enabled) {
-            uint32_t ts = wave_append(&counters[C_TRAIN], train);
-            if (train && ts < nr.train_cap) {
-                flags = 1u;
-                nr.tcount[ts] = 0;
-            } else {
-                ts = kMissCode;
-            }
-            if (active) nr.tslot_of[j] = ts;
+        if (!BATCH && nr.enabled && active) {
+            uint64_t g = g0 + j;
+            uint64_t grp = g >> nr.train_log2;
+            uint32_t ts = uint32_t(grp - (g0 >> nr.train_log2));
+            uint32_t member = uint32_t(g) & nr.train_mask;
+            Philox4 h = philox(uint32_t(grp), uint32_t(grp >> 32), 0xFFFFFFFFu, kStreamTrainPick, seed_lo, seed_hi);
+            train = member == (h.v[0] & nr.train_mask);
+            // the group's first slot in the chunk clears its record count, so
+            // a group whose training member lies in another chunk records nothing
+            if (train || member == 0 || j == 0) nr.tcount[ts] = 0;
+            if (train) flags = 1u;
+            nr.tslot_of[j] = train ? ts : kMissCode;
         }
         if (active) q.flags[j] = flags;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) counters[qi] = P;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        counters[qi] = P;
+        if (!BATCH && nr.enabled)
+            counters[C_TRAIN] = uint32_t(((g0 + P - 1) >> nr.train_log2) - (g0 >> nr.train_log2) + 1);
+    }
 }
 
-template <int W, bool COUNT>
-__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, RayQ q, int qi, HitQ hit, uint32_t *b0, uint32_t *b1,
-                                                  uint32_t *b2, uint32_t *counters, unsigned long long *stats,
-                                                  int sort, uint32_t *spill, size_t spill_lanes) {
+template <int W, bool COUNT, bool BATCH, bool Q>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
+                                                  uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
+                                                  unsigned long long *stats, int sort, uint32_t *spill,
+                                                  size_t spill_lanes) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
     __shared__ float st_[kLdsDepth * kBlock];
     auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
@@ -404,15 +478,17 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, RayQ q, int qi, Hi
         bool active = i < n;
         HitRec h;
         h.code = kMissCode;
+        DevScene Sx = S;
+        if (BATCH && active) Sx = T.rec[find_job(T, g0 + q.slot[i])].S;
         if (active) {
             f3 o{q.ox[i], q.oy[i], q.oz[i]}, d{q.dx[i], q.dy[i], q.dz[i]};
             if (COUNT) {
                 uint64_t cnt[2] = {0, 0};
-                h = trace_closest<W, kLdsDepth>(S, o, d, st, cnt);
+                h = trace_closest<W, kLdsDepth, Q>(Sx, o, d, st, cnt);
                 atomicAdd(&stats[S_NODES], (unsigned long long)cnt[0]);
                 atomicAdd(&stats[S_TRIS], (unsigned long long)cnt[1]);
             } else {
-                h = trace_closest<W, kLdsDepth>(S, o, d, st);
+                h = trace_closest<W, kLdsDepth, Q>(Sx, o, d, st);
             }
             hit.t[i] = h.t;
             hit.u[i] = h.u;
@@ -422,8 +498,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, RayQ q, int qi, Hi
         bool hitp = active && h.code != kMissCode;
         uint32_t mat = 0;
         if (hitp && sort) {
-            uint32_t prim = (h.code & kSphereBit) ? S.sphere_prim[h.code & ~kSphereBit] : S.tri_prim[h.code];
-            mat = S.shaders[S.prims[prim].shader].type;
+            uint32_t prim = (h.code & kSphereBit) ? Sx.sphere_prim[h.code & ~kSphereBit] : Sx.tri_prim[h.code];
+            mat = Sx.shaders[Sx.prims[prim].shader].type;
         }
         wave_count(&stats[S_CLOSEST], active);
         uint32_t p0 = wave_append(&counters[C_BUCKET + 0], hitp && mat == 0);
@@ -450,12 +526,13 @@ struct ShadeArgs {
     uint64_t g0;
     uint32_t npix, bounce, max_bounce, seed_lo, seed_hi;
     NrcRender nr;
+    JobTable T;  // batch renders: per-path scene, pixel count, depth and seed
 };
 
 // RadianceKernel::execute loop body (src/core/kernel/radiance.cu:26-71) for
 // one path vertex.
+template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
-    const DevScene &S = a.S;
     uint32_t n0 = a.counters[C_BUCKET + 0], n1 = a.counters[C_BUCKET + 1], n2 = a.counters[C_BUCKET + 2];
     uint32_t total = n0 + n1 + n2;
     uint32_t stride = gridDim.x * blockDim.x;
@@ -475,7 +552,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             float t = a.hit.t[i];
             uint32_t code = a.hit.code[i];
             uint64_t g = a.g0 + slot;
-            uint32_t sample = uint32_t(g / a.npix), pixel = uint32_t(g % a.npix);
+            DevScene S = a.S;
+            uint32_t npix = a.npix, max_bounce = a.max_bounce, seed_lo = a.seed_lo, seed_hi = a.seed_hi;
+            if (BATCH) {
+                const JobRec &J = a.T.rec[find_job(a.T, g)];
+                S = J.S;
+                g -= J.begin;
+                npix = J.npix;
+                max_bounce = J.max_bounce;
+                seed_lo = J.seed_lo;
+                seed_hi = J.seed_hi;
+            }
+            uint32_t sample = uint32_t(g / npix), pixel = uint32_t(g % npix);
             // Scene::intersect frame (src/core/meta/scene.hpp:48-53)
             f3 nrm;
             uint32_t prim;
@@ -543,8 +631,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 push_query = true;
                 nb = beta;
             } else {
-            Philox4 r0 = philox(pixel, sample, 2 * a.bounce, 0u, a.seed_lo, a.seed_hi);
-            Philox4 r1 = philox(pixel, sample, 2 * a.bounce + 1, 0u, a.seed_lo, a.seed_hi);
+            Philox4 r0 = philox(pixel, sample, 2 * a.bounce, 0u, seed_lo, seed_hi);
+            Philox4 r1 = philox(pixel, sample, 2 * a.bounce + 1, 0u, seed_lo, seed_hi);
             if (S.nlights) {
                 // light pick (radiance.cu:32-34) and AreaLight::sample (areaLight.hpp:19-31)
                 float xi = u01(r0.v[0]);
@@ -627,7 +715,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             f3 gd = normalize(fu * wi.x + fv * wi.y + nrm * wi.z);
             f3 go = p + gd * kSpawnEps;
             // Russian roulette (radiance.cu:62-68)
-            bool alive = a.bounce < a.max_bounce;
+            bool alive = a.bounce < max_bounce;
             float rr = smax(beta.x, smax(beta.y, beta.z));
             if (rr < 1.f && a.bounce > 3) {
                 float qq = smax(0.05f, 1.f - rr);
@@ -690,10 +778,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
     }
 }
 
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, ShadowQ sq, uint32_t *counters, float *L,
-                                                   unsigned long long *stats, uint32_t *spill, size_t spill_lanes,
-                                                   NrcRender nr, uint32_t bounce) {
+template <int W, bool BATCH, bool Q>
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, JobTable T, uint64_t g0, ShadowQ sq, uint32_t *counters,
+                                                   float *L, unsigned long long *stats, uint32_t *spill,
+                                                   size_t spill_lanes, NrcRender nr, uint32_t bounce) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
     __shared__ float st_[kLdsDepth * kBlock];
     auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
@@ -704,8 +792,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, ShadowQ sq, uint3
         bool active = i < n;
         if (active) {
             f3 o{sq.ox[i], sq.oy[i], sq.oz[i]}, d{sq.dx[i], sq.dy[i], sq.dz[i]};
-            if (!trace_occluded<W, kLdsDepth>(S, o, d, sq.tmax[i], st)) {
-                uint32_t slot = sq.slot[i];
+            uint32_t slot = sq.slot[i];
+            DevScene Sx = S;
+            if (BATCH) Sx = T.rec[find_job(T, g0 + slot)].S;
+            if (!trace_occluded<W, kLdsDepth, Q>(Sx, o, d, sq.tmax[i], st)) {
                 L[3 * slot] += sq.cx[i];
                 L[3 * slot + 1] += sq.cy[i];
                 L[3 * slot + 2] += sq.cz[i];
@@ -1112,11 +1202,12 @@ int upload(kr_scene *s, const T *host, size_t n, const T **dst) {
 
 size_t grid_lanes(const kr_device *dev) { return size_t(std::max(dev->num_cu, 1)) * 8 * kBlock; }
 
-int ensure_spill(uint32_t *&spill, size_t &lanes, std::vector<void *> &allocs, const kr_device *dev) {
+int ensure_spill(uint32_t *&spill, size_t &lanes, std::vector<void *> &allocs, const kr_device *dev,
+                 size_t want_lanes = 0) {
     if (spill) return KR_OK;
-    lanes = grid_lanes(dev);
-    // 64 stack entries (ref + t) per lane beyond the LDS part
-    int rc = dalloc(&spill, lanes * 64 * 2);
+    lanes = want_lanes ? want_lanes : grid_lanes(dev);
+    // kSpillDepth stack entries (ref + t) per lane beyond the LDS part
+    int rc = dalloc(&spill, lanes * kSpillDepth * 2);
     if (rc) return rc;
     allocs.push_back(spill);
     return KR_OK;
@@ -1278,9 +1369,31 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     (void)hipSetDevice(dev->ordinal);
     DevScene &v = s->view;
     int rc = KR_OK;
-    const float *nodes_f = nullptr;
-    rc = rc ? rc : upload(s, bvh.nodes.data(), bvh.nodes.size(), &nodes_f);
-    v.nodes = reinterpret_cast<const float4 *>(nodes_f);
+    s->qnodes = !(d->flags & KR_SCENE_FULL_NODES);
+    if (s->qnodes) {
+        std::vector<uint32_t> qn;
+        compress_wide_nodes(bvh, qn);
+        // every decoded child box must contain the exact one
+        const uint32_t NF = node_floats(width), NW = qnode_bytes(width) / 4;
+        for (uint32_t n = 0; n < bvh.num_nodes && !rc; ++n) {
+            const float *f = bvh.nodes.data() + size_t(n) * NF;
+            for (uint32_t c = 0; c < width; ++c) {
+                if (qn[size_t(n) * NW + 4 + c] == kEmpty) continue;
+                float lo[3], hi[3];
+                decode_qnode_box(qn.data() + size_t(n) * NW, width, c, lo, hi);
+                for (int a = 0; a < 3; ++a)
+                    if (!(lo[a] <= f[(2 * a) * width + c] && hi[a] >= f[(2 * a + 1) * width + c]))
+                        rc = fail(KR_E_OVERFLOW, "compressed BVH node does not contain its child box");
+            }
+        }
+        const uint32_t *nodes_q = nullptr;
+        rc = rc ? rc : upload(s, qn.data(), qn.size(), &nodes_q);
+        v.nodes = reinterpret_cast<const float4 *>(nodes_q);
+    } else {
+        const float *nodes_f = nullptr;
+        rc = rc ? rc : upload(s, bvh.nodes.data(), bvh.nodes.size(), &nodes_f);
+        v.nodes = reinterpret_cast<const float4 *>(nodes_f);
+    }
     rc = rc ? rc : upload(s, tris.data(), tris.size(), &v.tris);
     rc = rc ? rc : upload(s, nrm.data(), nrm.size(), &v.tri_nrm);
     rc = rc ? rc : upload(s, tri_prim.data(), tri_prim.size(), &v.tri_prim);
@@ -1299,7 +1412,7 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     v.ntris = d->num_triangles;
     uint32_t cap = (width - 1) * (bvh.depth + 1) + 1;
     v.stack_cap = cap;
-    if (cap > kLdsDepth + 64) {
+    if (cap > kLdsDepth + kSpillDepth) {
         kr_scene_free(s);
         return fail(KR_E_OVERFLOW, "BVH too deep for the traversal stack");
     }
@@ -1317,9 +1430,9 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     in.bvh_nodes = bvh.num_nodes;
     in.bvh_leaves = bvh.num_leaves;
     in.bvh_depth = bvh.depth;
-    in.node_bytes = node_floats(width) * 4;
+    in.node_bytes = s->qnodes ? qnode_bytes(width) : node_floats(width) * 4;
     in.tri_bytes = 48;
-    in.device_bytes = bvh.nodes.size() * 4 + tris.size() * 16 + nrm.size() * 4 + tri_prim.size() * 4 +
+    in.device_bytes = size_t(bvh.num_nodes) * in.node_bytes + tris.size() * 16 + nrm.size() * 4 + tri_prim.size() * 4 +
                       sph.size() * 8 + s->light_table.size() * 4;
     in.build_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     *out = s;
@@ -1343,13 +1456,20 @@ KR_EXPORT int kr_scene_light_samples(const kr_scene *s, uint32_t light, float *p
 }
 
 namespace {
-template <class F2, class F4, class F8>
-int dispatch_width(uint32_t w, F2 f2, F4 f4, F8 f8) {
-    if (w == 2) return f2();
-    if (w == 4) return f4();
-    if (w == 8) return f8();
+// Node format of a traversal instantiation: width W, compressed (Q) or not.
+template <int W_, bool Q_>
+struct Fmt {
+    static constexpr int W = W_;
+    static constexpr bool Q = Q_;
+};
+template <class F>
+int dispatch_fmt(uint32_t w, bool q, F f) {
+    if (w == 2) return q ? f(Fmt<2, true>{}) : f(Fmt<2, false>{});
+    if (w == 4) return q ? f(Fmt<4, true>{}) : f(Fmt<4, false>{});
+    if (w == 8) return q ? f(Fmt<8, true>{}) : f(Fmt<8, false>{});
     return fail(KR_E_INVALID, "bad bvh width");
 }
+
 
 struct DevBufs {
     std::vector<void *> ptrs;
@@ -1402,9 +1522,10 @@ KR_EXPORT int kr_trace_closest(kr_scene *s, const kr_rays *rays, uint32_t n, kr_
                            s->spill_lanes, (uint64_t *)nullptr);
         return KR_OK;
     };
-    rc = dispatch_width(
-        s->width, [&] { return launch(k_query_closest<2>); }, [&] { return launch(k_query_closest<4>); },
-        [&] { return launch(k_query_closest<8>); });
+    rc = dispatch_fmt(s->width, s->qnodes, [&](auto fm) {
+        using FM = decltype(fm);
+        return launch(k_query_closest<FM::W, FM::Q>);
+    });
     if (rc) return rc;
     KR_HIP(hipGetLastError());
     if (mem == KR_MEM_HOST) {
@@ -1449,9 +1570,10 @@ KR_EXPORT int kr_trace_shadow(kr_scene *s, const kr_rays *rays, uint32_t n, uint
                            s->spill_lanes);
         return KR_OK;
     };
-    rc = dispatch_width(
-        s->width, [&] { return launch(k_query_shadow<2>); }, [&] { return launch(k_query_shadow<4>); },
-        [&] { return launch(k_query_shadow<8>); });
+    rc = dispatch_fmt(s->width, s->qnodes, [&](auto fm) {
+        using FM = decltype(fm);
+        return launch(k_query_shadow<FM::W, FM::Q>);
+    });
     if (rc) return rc;
     KR_HIP(hipGetLastError());
     if (mem == KR_MEM_HOST) KR_HIP(hipMemcpyAsync(occluded, occ, n, hipMemcpyDeviceToHost, stream));
@@ -1475,20 +1597,10 @@ KR_EXPORT int kr_render_free(kr_render *r) {
     return KR_OK;
 }
 
-KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_params *prm, kr_render **out) {
-    if (!s || !cam || !prm || !out) return fail(KR_E_INVALID, "NULL argument");
-    *out = nullptr;
-    if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(KR_E_INVALID, "empty render");
-    if (uint64_t(prm->width) * prm->height > (1ull << 31)) return fail(KR_E_INVALID, "resolution too large");
-    if (cam->lens != KR_LENS_PINHOLE && cam->lens != KR_LENS_ORTHOGRAPHIC) return fail(KR_E_INVALID, "unknown lens");
-    (void)hipSetDevice(s->dev->ordinal);
-    auto *r = new kr_render();
-    r->scene = s;
-    r->cam = *cam;
-    r->prm = *prm;
-    r->npix = prm->width * prm->height;
-    // Lens ctor (src/core/misc/lens.hpp:113-125), FP32 as in the reference.
-    LensDev &L = r->lens;
+namespace {
+// Lens ctor (src/core/misc/lens.hpp:113-125), FP32 as in the reference.
+LensDev make_lens(const kr_camera *cam, const kr_params *prm) {
+    LensDev L{};
     L.lens = cam->lens;
     L.w = prm->width;
     L.h = prm->height;
@@ -1512,14 +1624,20 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
     st3(L.du, du);
     st3(L.dv, dv);
     st3(L.o, hload(cam->position));
-    uint64_t total = uint64_t(r->npix) * prm->spp;
-    // Default chunk: up to 32M paths in flight (~5.5 GB of queues), so that
-    // each bounce's kernels see tens of millions of rays and their tails and
-    // launch gaps are amortised.
-    uint64_t P = prm->chunk_paths ? prm->chunk_paths : (32u << 20);
-    P = std::min<uint64_t>(P, total);
-    P = std::max<uint64_t>(P, 1);
-    r->P = uint32_t(P);
+    return L;
+}
+
+int check_view(const kr_camera *cam, const kr_params *prm) {
+    if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(KR_E_INVALID, "empty render");
+    if (uint64_t(prm->width) * prm->height > (1ull << 31)) return fail(KR_E_INVALID, "resolution too large");
+    if (cam->lens != KR_LENS_PINHOLE && cam->lens != KR_LENS_ORTHOGRAPHIC) return fail(KR_E_INVALID, "unknown lens");
+    return KR_OK;
+}
+
+// Queues for r->P paths, accumulators for r->npix pixels, counters, the
+// traversal spill stack for `lanes` lanes; r->grid = lanes / kBlock.
+int render_buffers(kr_render *r, size_t lanes) {
+    size_t P = r->P;
     auto get = [&](auto **p, size_t cnt) {
         int rc = dalloc(p, cnt);
         if (!rc) r->allocs.push_back(*p);
@@ -1551,12 +1669,37 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
     if (!rc) rc = get(&r->stats, S_NUM);
     if (!rc) rc = get(&r->acc, 3 * size_t(r->npix));
     if (!rc) rc = get(&r->valid, r->npix);
-    if (!rc) rc = ensure_spill(r->spill, r->spill_lanes, r->allocs, s->dev);
+    if (!rc) rc = ensure_spill(r->spill, r->spill_lanes, r->allocs, r->scene->dev, lanes);
+    if (!rc) r->grid = int(r->spill_lanes / kBlock);
+    return rc;
+}
+}  // namespace
+
+KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_params *prm, kr_render **out) {
+    if (!s || !cam || !prm || !out) return fail(KR_E_INVALID, "NULL argument");
+    *out = nullptr;
+    int rc = check_view(cam, prm);
+    if (rc) return rc;
+    (void)hipSetDevice(s->dev->ordinal);
+    auto *r = new kr_render();
+    r->scene = s;
+    r->cam = *cam;
+    r->prm = *prm;
+    r->npix = prm->width * prm->height;
+    r->lens = make_lens(cam, prm);
+    uint64_t total = uint64_t(r->npix) * prm->spp;
+    // Default chunk: up to 32M paths in flight (~5.5 GB of queues), so that
+    // each bounce's kernels see tens of millions of rays and their tails and
+    // launch gaps are amortised.
+    uint64_t P = prm->chunk_paths ? prm->chunk_paths : (32u << 20);
+    P = std::min<uint64_t>(P, total);
+    P = std::max<uint64_t>(P, 1);
+    r->P = uint32_t(P);
+    rc = render_buffers(r, 0);
     if (rc) {
         kr_render_free(r);
         return rc;
     }
-    r->grid = int(r->spill_lanes / kBlock);
     rc = kr_render_reset(r);
     if (rc) {
         kr_render_free(r);
@@ -1626,11 +1769,26 @@ void resolve_events(kr_render *r) {
     r->pending.clear();
 }
 
-template <int W>
-int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
+// A batch's jobs as the host sees them: path ranges and accumulator slices.
+struct HostJob {
+    uint64_t begin, paths;
+    uint32_t npix, acc_off;
+};
+struct BatchView {
+    JobTable T;
+    const std::vector<HostJob> *jobs;
+};
+
+// Enqueues one chunk of P paths starting at global path g0: camera, then
+// maxBounce + 1 rounds of trace / shade / shadow, then the FP64 accumulation.
+// Only kernel launches on the device stream, so the sequence can be captured
+// into a HIP graph. bv != nullptr: a batch (BATCH = true).
+template <int W, bool BATCH, bool Q>
+int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
     kr_scene *s = r->scene;
     hipStream_t st = s->dev->stream;
     const DevScene &S = s->view;
+    JobTable T = bv ? bv->T : JobTable{nullptr, nullptr, 0};
     int grid = r->grid;
     int cam_grid = int(std::min<size_t>((P + kBlock - 1) / kBlock, size_t(grid)));
     int cur = 0;
@@ -1638,8 +1796,9 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
     {
         Timed tm(r, K_OTHER);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, 0xFFFFFFFFu);
-        hipLaunchKernelGGL(k_camera, dim3(cam_grid), dim3(kBlock), 0, st, r->lens, g0, P, r->npix, r->q[0], r->L,
-                           r->counters, C_RAY0, r->nr, uint32_t(r->prm.seed), uint32_t(r->prm.seed >> 32));
+        hipLaunchKernelGGL((k_camera<BATCH>), dim3(cam_grid), dim3(kBlock), 0, st, r->lens, T, g0, P, r->npix,
+                           r->q[0], r->L, r->counters, C_RAY0, r->nr, uint32_t(r->prm.seed),
+                           uint32_t(r->prm.seed >> 32));
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
     for (uint32_t b = 0; b <= r->prm.max_bounce; ++b) {
@@ -1654,13 +1813,13 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
         {
             Timed tm(r, K_TRACE);
             if (count)
-                hipLaunchKernelGGL((k_trace<W, true>), dim3(grid), dim3(kBlock), 0, st, S, r->q[cur], qi, r->hit,
-                                   r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort, r->spill,
-                                   r->spill_lanes);
+                hipLaunchKernelGGL((k_trace<W, true, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->q[cur],
+                                   qi, r->hit, r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort,
+                                   r->spill, r->spill_lanes);
             else
-                hipLaunchKernelGGL((k_trace<W, false>), dim3(grid), dim3(kBlock), 0, st, S, r->q[cur], qi, r->hit,
-                                   r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort, r->spill,
-                                   r->spill_lanes);
+                hipLaunchKernelGGL((k_trace<W, false, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->q[cur],
+                                   qi, r->hit, r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort,
+                                   r->spill, r->spill_lanes);
         }
         ShadeArgs a;
         a.S = S;
@@ -1683,23 +1842,36 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P) {
         a.seed_lo = uint32_t(r->prm.seed);
         a.seed_hi = uint32_t(r->prm.seed >> 32);
         a.nr = r->nr;
+        a.T = T;
         {
             Timed tm(r, K_SHADE);
-            hipLaunchKernelGGL(k_shade, dim3(grid), dim3(kBlock), 0, st, a);
+            hipLaunchKernelGGL((k_shade<BATCH>), dim3(grid), dim3(kBlock), 0, st, a);
         }
         {
             Timed tm(r, K_SHADOW);
-            hipLaunchKernelGGL((k_shadow<W>), dim3(grid), dim3(kBlock), 0, st, S, r->sq, r->counters, r->L,
-                               r->stats, r->spill, r->spill_lanes, r->nr, b);
+            hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq, r->counters,
+                               r->L, r->stats, r->spill, r->spill_lanes, r->nr, b);
         }
         cur = 1 - cur;
     }
-    if (r->nr.enabled) nrc_after_chunk(r);
+    if (!BATCH && r->nr.enabled) nrc_after_chunk(r);
     {
         Timed tm(r, K_OTHER);
-        int acc_grid = int(std::min<size_t>((r->npix + kBlock - 1) / kBlock, size_t(grid)));
-        hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, r->L, g0, P, r->npix, r->acc,
-                           r->valid, r->stats);
+        if (!BATCH) {
+            int acc_grid = int(std::min<size_t>((r->npix + kBlock - 1) / kBlock, size_t(grid)));
+            hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, r->L, g0, P, r->npix, r->acc,
+                               r->valid, r->stats);
+        } else {
+            // one accumulation per job overlapping the chunk, on its slice
+            for (const HostJob &J : *bv->jobs) {
+                uint64_t lo = std::max(g0, J.begin), hi = std::min(g0 + P, J.begin + J.paths);
+                if (lo >= hi) continue;
+                int acc_grid = int(std::min<size_t>((J.npix + kBlock - 1) / kBlock, size_t(grid)));
+                hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, r->L + 3 * (lo - g0),
+                                   lo - J.begin, uint32_t(hi - lo), J.npix, r->acc + 3 * size_t(J.acc_off),
+                                   r->valid + J.acc_off, r->stats);
+            }
+        }
     }
     KR_HIP(hipGetLastError());
     return KR_OK;
@@ -1717,9 +1889,10 @@ KR_EXPORT int kr_render_run(kr_render *r, uint32_t spp_begin, uint32_t spp_count
     uint64_t g = uint64_t(spp_begin) * r->npix, gend = uint64_t(spp_begin + spp_count) * r->npix;
     while (g < gend) {
         uint32_t P = uint32_t(std::min<uint64_t>(r->P, gend - g));
-        int rc = dispatch_width(
-            r->scene->width, [&] { return run_chunk<2>(r, g, P); }, [&] { return run_chunk<4>(r, g, P); },
-            [&] { return run_chunk<8>(r, g, P); });
+        int rc = dispatch_fmt(r->scene->width, r->scene->qnodes, [&](auto fm) {
+            using FM = decltype(fm);
+            return run_chunk<FM::W, false, FM::Q>(r, g, P, nullptr);
+        });
         if (rc) return rc;
         g += P;
         // keep the number of outstanding event pairs bounded
@@ -2123,11 +2296,32 @@ __global__ __launch_bounds__(256) void k_nrc_infer(nrc::Weights W, NrcRender nr,
 // Training targets of the chunk's training paths: for vertex j the radiance
 // the rest of the path carried back to it, (sum_{k>=j} c_k - beta_j E_j) / beta_j,
 // where c_k is everything vertex k added to the path's L.
-__global__ void k_nrc_extract(NrcRender nr, const uint32_t *counters, float *pool_raw, float *pool_tgt,
-                              uint32_t *pool_total, uint32_t cap) {
+__global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_t *counters, float *pool_raw,
+                                                     float *pool_tgt, uint32_t *pool_total, uint32_t cap) {
     uint32_t ntrain = min(counters[C_TRAIN], nr.train_cap);
-    for (uint32_t ts = blockIdx.x * blockDim.x + threadIdx.x; ts < ntrain; ts += gridDim.x * blockDim.x) {
-        uint32_t nv = nr.tcount[ts];
+    uint32_t stride = gridDim.x * blockDim.x;
+    // whole waves iterate together so that the ring slots can be reserved
+    // with one atomic per wave
+    for (uint32_t base = blockIdx.x * blockDim.x; base < ntrain; base += stride) {
+        uint32_t ts = base + threadIdx.x;
+        uint32_t nv = ts < ntrain ? nr.tcount[ts] : 0u;
+        // pass 1: how many of this path's vertices give a usable target
+        uint32_t keep = 0;
+        {
+            float sx = 0.f, sy = 0.f, sz = 0.f;
+            for (int j = int(nv) - 1; j >= 0; --j) {
+                const float *rec = nr.rec + (size_t(ts) * nr.nv + uint32_t(j)) * kRecFloats;
+                sx += rec[19];
+                sy += rec[20];
+                sz += rec[21];
+                float bx = rec[16], by = rec[17], bz = rec[18];
+                if (!(bx > 0.f && by > 0.f && bz > 0.f)) continue;
+                float tx = (sx - rec[22]) / bx, ty = (sy - rec[23]) / by, tz = (sz - rec[24]) / bz;
+                if (isfinite(tx) && isfinite(ty) && isfinite(tz)) ++keep;
+            }
+        }
+        uint32_t slot = wave_reserve(pool_total, keep);
+        // pass 2: write them
         float sx = 0.f, sy = 0.f, sz = 0.f;
         for (int j = int(nv) - 1; j >= 0; --j) {
             const float *rec = nr.rec + (size_t(ts) * nr.nv + uint32_t(j)) * kRecFloats;
@@ -2138,13 +2332,13 @@ __global__ void k_nrc_extract(NrcRender nr, const uint32_t *counters, float *poo
             if (!(bx > 0.f && by > 0.f && bz > 0.f)) continue;
             float tx = (sx - rec[22]) / bx, ty = (sy - rec[23]) / by, tz = (sz - rec[24]) / bz;
             if (!(isfinite(tx) && isfinite(ty) && isfinite(tz))) continue;
-            uint32_t slot = atomicAdd(pool_total, 1u) % cap;
-            float4 *dst = reinterpret_cast<float4 *>(pool_raw + size_t(slot) * nrc::kRaw);
+            uint32_t r = slot++ % cap;
+            float4 *dst = reinterpret_cast<float4 *>(pool_raw + size_t(r) * nrc::kRaw);
             dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
             dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
             dst[2] = make_float4(rec[8], rec[9], rec[10], rec[11]);
             dst[3] = make_float4(rec[12], 0.f, 0.f, 0.f);
-            reinterpret_cast<float4 *>(pool_tgt)[slot] = make_float4(tx, ty, tz, 0.f);
+            reinterpret_cast<float4 *>(pool_tgt)[r] = make_float4(tx, ty, tz, 0.f);
         }
     }
 }
@@ -2165,6 +2359,7 @@ void nrc_after_chunk(kr_render *r) {
 
 int nrc_after_run(kr_render *r) {
     kr_nrc *n = r->nrc;
+    if (n->cfg.flags & KR_NRC_FROZEN) return KR_OK;
     Timed tm(r, K_OTHER);
     for (uint32_t i = 0; i < n->cfg.steps_per_frame; ++i) {
         TrainSource src;
@@ -2195,11 +2390,12 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     if (n->dev != r->scene->dev) return fail(KR_E_INVALID, "cache and render live on different devices");
     NrcRender &nr = r->nr;
     uint32_t log2 = n->cfg.train_log2;
-    nr.train_mask = (1u << log2) - 1u;  // log2 <= 31 (kr_nrc_create)
+    nr.train_log2 = log2;               // <= 31 (kr_nrc_create)
+    nr.train_mask = (1u << log2) - 1u;
     nr.rough_threshold = n->cfg.rough_threshold;
     nr.nv = r->prm.max_bounce + 1;
-    uint64_t expect = uint64_t(r->P) >> log2;
-    nr.train_cap = uint32_t(std::min<uint64_t>(r->P, expect + expect / 8 + 4096));
+    // groups a chunk of P paths can touch: P / 2^log2 + 2 partial ones
+    nr.train_cap = uint32_t(std::min<uint64_t>(r->P, (uint64_t(r->P) >> log2) + 2));
     auto get = [&](auto **p, size_t cnt) {
         int rc = dalloc(p, cnt);
         if (!rc) r->nrc_allocs.push_back(*p);
@@ -2220,5 +2416,401 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     }
     nr.enabled = 1;
     r->nrc = n;
+    return KR_OK;
+}
+
+// ------------------------------------------------------------- batches --
+namespace {
+// Owned copy of a scene description (the batch rebuilds BVHs at other
+// widths while autotuning).
+struct SceneCopy {
+    std::vector<float> pos, nrm;
+    std::vector<uint32_t> idx;
+    std::vector<kr_prim_desc> prims;
+    std::vector<kr_shader_desc> shaders;
+    kr_scene_desc desc{};
+};
+}  // namespace
+
+struct kr_batch {
+    kr_device *dev = nullptr;
+    std::vector<SceneCopy> scenes;
+    std::vector<kr_scene *> up;  // uploaded at tuning.bvh_width
+    std::vector<kr_batch_job> jobs;
+    std::vector<HostJob> hjobs;  // full sample ranges
+    std::vector<uint32_t> acc_off;
+    uint64_t total_paths = 0, total_pix = 0;
+    uint32_t max_bounce = 0;
+    kr_batch_tuning tuning{};
+    kr_render *r = nullptr;       // queues, counters and the batch's accumulators
+    kr_scene anchor;              // r->scene: device and stream only (the jobs' scenes come from the table)
+    JobRec *d_rec = nullptr;      // full job table
+    uint64_t *d_begin = nullptr;
+    JobRec *d_trec = nullptr;     // trial job table (autotuner)
+    uint64_t *d_tbegin = nullptr;
+    std::vector<void *> allocs;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint32_t graph_nodes = 0, chunks = 0;
+    double capture_seconds = 0, seconds = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool timed = false;
+};
+
+namespace {
+constexpr uint32_t kMaxBlocksPerCu = 16;
+constexpr uint32_t kBatchMaxChunk = 32u << 20;
+
+void batch_drop_graph(kr_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Job tables for the first spp_limit samples of every job (0: all).
+void batch_tables(const kr_batch *b, uint32_t spp_limit, std::vector<JobRec> &rec, std::vector<uint64_t> &begin,
+                  std::vector<HostJob> &hj) {
+    rec.resize(b->jobs.size());
+    begin.resize(b->jobs.size());
+    hj.resize(b->jobs.size());
+    uint64_t g = 0;
+    for (size_t k = 0; k < b->jobs.size(); ++k) {
+        const kr_batch_job &J = b->jobs[k];
+        uint32_t npix = J.params.width * J.params.height;
+        uint32_t spp = spp_limit ? std::min(spp_limit, J.params.spp) : J.params.spp;
+        JobRec &R = rec[k];
+        R.S = b->up[J.scene]->view;
+        R.lens = make_lens(&J.camera, &J.params);
+        R.begin = g;
+        R.npix = npix;
+        R.max_bounce = J.params.max_bounce;
+        R.seed_lo = uint32_t(J.params.seed);
+        R.seed_hi = uint32_t(J.params.seed >> 32);
+        begin[k] = g;
+        hj[k] = HostJob{g, uint64_t(npix) * spp, npix, b->acc_off[k]};
+        g += uint64_t(npix) * spp;
+    }
+}
+
+int batch_upload_tables(kr_batch *b, JobRec *d_rec, uint64_t *d_begin, uint32_t spp_limit,
+                        std::vector<HostJob> &hj) {
+    std::vector<JobRec> rec;
+    std::vector<uint64_t> begin;
+    batch_tables(b, spp_limit, rec, begin, hj);
+    hipStream_t st = b->dev->stream;
+    KR_HIP(hipMemcpyAsync(d_rec, rec.data(), sizeof(JobRec) * rec.size(), hipMemcpyHostToDevice, st));
+    KR_HIP(hipMemcpyAsync(d_begin, begin.data(), sizeof(uint64_t) * begin.size(), hipMemcpyHostToDevice, st));
+    KR_HIP(hipStreamSynchronize(st));
+    return KR_OK;
+}
+
+// (Re)uploads every scene at `width` and rebuilds the full job table.
+int batch_set_width(kr_batch *b, uint32_t width) {
+    if (!b->up.empty() && b->up[0] && b->up[0]->width == width) return KR_OK;
+    batch_drop_graph(b);
+    (void)hipStreamSynchronize(b->dev->stream);
+    for (kr_scene *s : b->up) kr_scene_free(s);
+    b->up.assign(b->scenes.size(), nullptr);
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+        kr_scene_desc d = b->scenes[i].desc;
+        d.bvh_width = width;
+        // one node format for all scenes: the kernels are compiled per format
+        d.flags = (d.flags & ~KR_SCENE_FULL_NODES) | (b->scenes[0].desc.flags & KR_SCENE_FULL_NODES);
+        int rc = kr_scene_upload(b->dev, &d, &b->up[i]);
+        if (rc) {
+            // leave no half-uploaded set behind: the next call re-uploads all
+            for (kr_scene *s : b->up) kr_scene_free(s);
+            b->up.clear();
+            return rc;
+        }
+    }
+    b->tuning.bvh_width = width;
+    return batch_upload_tables(b, b->d_rec, b->d_begin, 0, b->hjobs);
+}
+
+// Enqueues every chunk of a path space of `total` paths.
+int batch_enqueue(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P) {
+    int rc = KR_OK;
+    for (uint64_t g = 0; g < total && !rc; g += P) {
+        uint32_t n = uint32_t(std::min<uint64_t>(P, total - g));
+        rc = dispatch_fmt(b->tuning.bvh_width, b->up[0]->qnodes, [&](auto fm) {
+            using FM = decltype(fm);
+            return run_chunk<FM::W, true, FM::Q>(b->r, g, n, &bv);
+        });
+    }
+    return rc;
+}
+
+void batch_apply_launch(kr_batch *b) {
+    b->r->grid = int(std::min<size_t>(size_t(std::max(b->dev->num_cu, 1)) * b->tuning.blocks_per_cu,
+                                      b->r->spill_lanes / kBlock));
+    b->r->prm.flags = b->tuning.flags;
+}
+
+int batch_capture(kr_batch *b) {
+    batch_drop_graph(b);
+    batch_apply_launch(b);
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = b->dev->stream;
+    BatchView bv{JobTable{b->d_rec, b->d_begin, uint32_t(b->jobs.size())}, &b->hjobs};
+    KR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(b, bv, b->total_paths, b->tuning.chunk_paths);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(KR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    size_t nodes = 0;
+    KR_HIP(hipGraphGetNodes(g, nullptr, &nodes));
+    b->graph_nodes = uint32_t(nodes);
+    KR_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    b->chunks = uint32_t((b->total_paths + b->tuning.chunk_paths - 1) / b->tuning.chunk_paths);
+    b->capture_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return KR_OK;
+}
+}  // namespace
+
+KR_EXPORT int kr_batch_free(kr_batch *b) {
+    if (!b) return KR_OK;
+    (void)hipSetDevice(b->dev->ordinal);
+    (void)hipStreamSynchronize(b->dev->stream);
+    batch_drop_graph(b);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->r) kr_render_free(b->r);
+    for (kr_scene *s : b->up) kr_scene_free(s);
+    for (void *p : b->allocs) (void)hipFree(p);
+    delete b;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint32_t num_scenes,
+                              const kr_batch_job *jobs, uint32_t num_jobs, const kr_batch_tuning *tuning,
+                              kr_batch **out) {
+    if (!dev || !scenes || !jobs || !out) return fail(KR_E_INVALID, "NULL argument");
+    *out = nullptr;
+    if (!num_scenes || !num_jobs) return fail(KR_E_INVALID, "empty batch");
+    kr_batch_tuning tu{4, kBatchMaxChunk, 8, 0, 0.0};
+    if (tuning) tu = *tuning;
+    if (tu.bvh_width != 2 && tu.bvh_width != 4 && tu.bvh_width != 8)
+        return fail(KR_E_INVALID, "bvh_width must be 2, 4 or 8");
+    if (tu.blocks_per_cu < 1 || tu.blocks_per_cu > kMaxBlocksPerCu)
+        return fail(KR_E_INVALID, "blocks_per_cu must be 1..16");
+    if (tu.chunk_paths == 0) tu.chunk_paths = kBatchMaxChunk;
+    uint64_t total = 0, pix = 0;
+    uint32_t mb = 0;
+    for (uint32_t k = 0; k < num_jobs; ++k) {
+        const kr_batch_job &J = jobs[k];
+        if (J.scene >= num_scenes) return fail(KR_E_INVALID, "job scene index out of range");
+        int rc = check_view(&J.camera, &J.params);
+        if (rc) return rc;
+        total += uint64_t(J.params.width) * J.params.height * J.params.spp;
+        pix += uint64_t(J.params.width) * J.params.height;
+        mb = std::max(mb, J.params.max_bounce);
+    }
+    if (pix > 0xFFFFFFFFull) return fail(KR_E_INVALID, "batch has too many pixels");
+    (void)hipSetDevice(dev->ordinal);
+    auto *b = new kr_batch();
+    b->dev = dev;
+    b->jobs.assign(jobs, jobs + num_jobs);
+    b->total_paths = total;
+    b->total_pix = pix;
+    b->max_bounce = mb;
+    b->tuning = tu;
+    b->tuning.bvh_width = 0;  // set by batch_set_width
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < num_jobs; ++k) {
+        b->acc_off.push_back(off);
+        off += jobs[k].params.width * jobs[k].params.height;
+    }
+    b->scenes.resize(num_scenes);
+    for (uint32_t i = 0; i < num_scenes; ++i) {
+        const kr_scene_desc &d = scenes[i];
+        SceneCopy &c = b->scenes[i];
+        if (d.num_triangles && (!d.positions || !d.normals || !d.indices)) {
+            delete b;
+            return fail(KR_E_INVALID, "mesh arrays missing");
+        }
+        if (d.num_prims && !d.prims) {
+            delete b;
+            return fail(KR_E_INVALID, "primitive array missing");
+        }
+        if (d.num_shaders && !d.shaders) {
+            delete b;
+            return fail(KR_E_INVALID, "shader array missing");
+        }
+        if (d.num_triangles) {
+            c.pos.assign(d.positions, d.positions + 3 * size_t(d.num_vertices));
+            c.nrm.assign(d.normals, d.normals + 3 * size_t(d.num_vertices));
+            c.idx.assign(d.indices, d.indices + 3 * size_t(d.num_triangles));
+        }
+        c.prims.assign(d.prims, d.prims + d.num_prims);
+        c.shaders.assign(d.shaders, d.shaders + d.num_shaders);
+        c.desc = d;
+        c.desc.positions = c.pos.data();
+        c.desc.normals = c.nrm.data();
+        c.desc.indices = c.idx.data();
+        c.desc.prims = c.prims.data();
+        c.desc.shaders = c.shaders.data();
+    }
+    auto get = [&](auto **p, size_t cnt) {
+        int rc = dalloc(p, cnt);
+        if (!rc) b->allocs.push_back(*p);
+        return rc;
+    };
+    int rc = get(&b->d_rec, num_jobs);
+    if (!rc) rc = get(&b->d_begin, num_jobs);
+    if (!rc) rc = get(&b->d_trec, num_jobs);
+    if (!rc) rc = get(&b->d_tbegin, num_jobs);
+    if (!rc) rc = batch_set_width(b, tu.bvh_width);
+    if (!rc) {
+        auto *r = new kr_render();
+        b->r = r;
+        b->anchor.dev = dev;
+        r->scene = &b->anchor;
+        r->prm.max_bounce = mb;
+        r->prm.flags = tu.flags;
+        r->npix = uint32_t(pix);
+        r->P = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(std::max(tu.chunk_paths, kBatchMaxChunk), total)));
+        rc = render_buffers(r, size_t(std::max(dev->num_cu, 1)) * kMaxBlocksPerCu * kBlock);
+        if (!rc) rc = kr_render_reset(r);
+    }
+    if (!rc && (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess))
+        rc = fail(KR_E_HIP, "hipEventCreate");
+    if (rc) {
+        kr_batch_free(b);
+        return rc;
+    }
+    b->tuning.chunk_paths = std::min(tu.chunk_paths, b->r->P);
+    *out = b;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_batch_run(kr_batch *b) {
+    if (!b) return fail(KR_E_INVALID, "NULL batch");
+    (void)hipSetDevice(b->dev->ordinal);
+    if (!b->exec) {
+        int rc = batch_capture(b);
+        if (rc) return rc;
+    }
+    hipStream_t st = b->dev->stream;
+    KR_HIP(hipEventRecord(b->e0, st));
+    KR_HIP(hipGraphLaunch(b->exec, st));
+    KR_HIP(hipEventRecord(b->e1, st));
+    b->timed = true;
+    b->r->paths += b->total_paths;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_batch_sync(kr_batch *b) {
+    if (!b) return fail(KR_E_INVALID, "NULL batch");
+    KR_HIP(hipStreamSynchronize(b->dev->stream));
+    if (b->timed) {
+        float ms = 0.f;
+        KR_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+        b->seconds = ms * 1e-3;
+        b->timed = false;
+    }
+    return KR_OK;
+}
+
+KR_EXPORT int kr_batch_reset(kr_batch *b) {
+    if (!b) return fail(KR_E_INVALID, "NULL batch");
+    return kr_render_reset(b->r);
+}
+
+KR_EXPORT int kr_batch_read(kr_batch *b, uint32_t job, double *rgb, uint32_t *valid) {
+    if (!b) return fail(KR_E_INVALID, "NULL batch");
+    if (job >= b->jobs.size()) return fail(KR_E_INVALID, "job index out of range");
+    hipStream_t st = b->dev->stream;
+    size_t off = b->acc_off[job], n = size_t(b->jobs[job].params.width) * b->jobs[job].params.height;
+    if (rgb) KR_HIP(hipMemcpyAsync(rgb, b->r->acc + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    if (valid) KR_HIP(hipMemcpyAsync(valid, b->r->valid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
+    return KR_OK;
+}
+
+KR_EXPORT int kr_batch_get_info(kr_batch *b, kr_batch_info *info) {
+    if (!b || !info) return fail(KR_E_INVALID, "NULL argument");
+    info->total_paths = b->total_paths;
+    info->total_pixels = b->total_pix;
+    info->num_jobs = uint32_t(b->jobs.size());
+    info->num_scenes = uint32_t(b->scenes.size());
+    info->graph_nodes = b->graph_nodes;
+    info->chunks = b->chunks;
+    info->tuning = b->tuning;
+    info->seconds = b->seconds;
+    info->capture_seconds = b->capture_seconds;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_batch_autotune(kr_batch *b, uint32_t trial_spp, kr_batch_tuning *chosen) {
+    if (!b) return fail(KR_E_INVALID, "NULL batch");
+    (void)hipSetDevice(b->dev->ordinal);
+    hipStream_t st = b->dev->stream;
+    if (trial_spp == 0) {
+        // about 64 M trial paths, at least one sample of every job
+        uint64_t pix = std::max<uint64_t>(b->total_pix, 1);
+        trial_spp = uint32_t(std::max<uint64_t>(1, (64ull << 20) / pix));
+    }
+    const uint32_t widths[3] = {2, 4, 8};
+    const uint32_t chunks[4] = {4u << 20, 8u << 20, 16u << 20, 32u << 20};
+    const uint32_t bpcs[3] = {4, 8, 16};
+    kr_batch_tuning best = b->tuning;
+    best.paths_per_second = -1.0;
+    uint32_t flags = b->tuning.flags;
+    for (uint32_t w : widths) {
+        int rc = batch_set_width(b, w);
+        if (rc == KR_E_OVERFLOW) continue;  // a BVH of this width is too deep for the stack
+        if (rc) return rc;
+        std::vector<HostJob> thj;
+        rc = batch_upload_tables(b, b->d_trec, b->d_tbegin, trial_spp, thj);
+        if (rc) return rc;
+        uint64_t ttotal = 0;
+        for (const HostJob &h : thj) ttotal += h.paths;
+        BatchView bv{JobTable{b->d_trec, b->d_tbegin, uint32_t(b->jobs.size())}, &thj};
+        uint32_t last_P = 0;
+        for (uint32_t Pc : chunks) {
+            uint32_t P = std::min(Pc, b->r->P);  // the queues hold at most r->P paths
+            if (P == last_P) continue;
+            last_P = P;
+            for (uint32_t bpc : bpcs) {
+                b->tuning.chunk_paths = P;
+                b->tuning.blocks_per_cu = bpc;
+                batch_apply_launch(b);
+                double best_s = 1e30;
+                for (int rep = 0; rep < 2; ++rep) {
+                    KR_HIP(hipEventRecord(b->e0, st));
+                    rc = batch_enqueue(b, bv, ttotal, P);
+                    if (rc) return rc;
+                    KR_HIP(hipEventRecord(b->e1, st));
+                    KR_HIP(hipEventSynchronize(b->e1));
+                    float ms = 0.f;
+                    KR_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+                    best_s = std::min(best_s, double(ms) * 1e-3);
+                }
+                double rate = double(ttotal) / best_s;
+                if (rate > best.paths_per_second) {
+                    best = b->tuning;
+                    best.bvh_width = w;
+                    best.chunk_paths = P;
+                    best.blocks_per_cu = bpc;
+                    best.flags = flags;
+                    best.paths_per_second = rate;
+                }
+            }
+        }
+    }
+    int rc = batch_set_width(b, best.bvh_width);
+    if (rc) return rc;
+    b->tuning = best;
+    batch_drop_graph(b);
+    rc = kr_render_reset(b->r);
+    if (rc) return rc;
+    if (chosen) *chosen = best;
     return KR_OK;
 }

@@ -84,10 +84,55 @@ private:
     std::string kernel_ = "Radiance";
 };
 
+// The same integrator through kr_batch: the render is a batch of one job,
+// captured into a HIP graph and replayed with one launch.
+class HIPBatchedTracer : public Tracer {
+public:
+    explicit HIPBatchedTracer(const Json &) {}
+    void execute(Image &image, const kr_camera &camera, const LoadedScene &scene, uint32_t spp,
+                 const RenderOptions &opt) override {
+        const std::string kernel = opt.kernel.empty() ? scene.kernel : opt.kernel;
+        if (kernel != "Radiance") throw std::logic_error("no such type: " + kernel);
+        struct Handles {
+            kr_device *dev = nullptr;
+            kr_batch *batch = nullptr;
+            ~Handles() {
+                if (batch) kr_batch_free(batch);
+                if (dev) kr_device_close(dev);
+            }
+        } h;
+        kr_check(kr_device_open(opt.device, &h.dev), "kr_device_open");
+        kr_scene_desc d = scene.desc(opt.bvh_width);
+        kr_batch_job job{};
+        job.scene = 0;
+        job.camera = camera;
+        job.params.width = image.w;
+        job.params.height = image.h;
+        job.params.spp = spp;
+        job.params.max_bounce = scene.max_bounce;
+        job.params.seed = opt.seed;
+        kr_batch_tuning tu{opt.bvh_width ? opt.bvh_width : 4u, 32u << 20, 8, 0, 0.0};
+        kr_check(kr_batch_create(h.dev, &d, 1, &job, 1, &tu, &h.batch), "kr_batch_create");
+        KINFO("tracer", "Tracing start");
+        auto t0 = std::chrono::steady_clock::now();
+        kr_check(kr_batch_run(h.batch), "kr_batch_run");
+        kr_check(kr_batch_sync(h.batch), "kr_batch_sync");
+        double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        KINFO("tracer", "Tracer joint in", sec, "seconds");
+        image.rgb.assign(3 * size_t(image.w) * image.h, 0.0);
+        image.valid.assign(size_t(image.w) * image.h, 0);
+        kr_check(kr_batch_read(h.batch, 0, image.rgb.data(), image.valid.data()), "kr_batch_read");
+        kr_batch_info info{};
+        kr_batch_get_info(h.batch, &info);
+        KLOG(double(info.total_paths) / sec, "paths/s,", info.graph_nodes, "graph nodes");
+    }
+};
+
 std::map<std::string, TracerCtor> &registry() {
     static std::map<std::string, TracerCtor> r = {
         {"HIPWavefront", [](const Json &p) { return std::unique_ptr<Tracer>(new HIPWavefrontTracer(p)); }},
         {"CPUMulticore", [](const Json &p) { return std::unique_ptr<Tracer>(new HIPWavefrontTracer(p)); }},
+        {"HIPBatched", [](const Json &p) { return std::unique_ptr<Tracer>(new HIPBatchedTracer(p)); }},
     };
     return r;
 }

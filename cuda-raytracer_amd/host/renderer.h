@@ -44,9 +44,11 @@ struct Tracer {
 
 using TracerCtor = std::function<std::unique_ptr<Tracer>(const Json &props)>;
 
-// The registry: "HIPWavefront" (the gfx950 wavefront integrator) and
-// "CPUMulticore", the reference's only tracer and its default, served by the
-// same device integrator so reference scene files render unchanged.
+// The registry: "HIPWavefront" (the gfx950 wavefront integrator),
+// "HIPBatched" (the same integrator as a one-job kr_batch replayed from a
+// HIP graph) and "CPUMulticore", the reference's only tracer and its
+// default, served by the wavefront integrator so reference scene files
+// render unchanged.
 // Unknown names throw "no such type:" (src/core/basic/factory.hpp:18-21).
 std::unique_ptr<Tracer> create_tracer(const std::string &name, const Json &props);
 std::vector<std::string> tracer_names();

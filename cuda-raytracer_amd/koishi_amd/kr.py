@@ -49,6 +49,7 @@ class SceneDesc(C.Structure):
 
 
 KR_SCENE_NO_AREA_LIGHTS = 0x1
+KR_SCENE_FULL_NODES = 0x2
 
 
 class SceneInfo(C.Structure):
@@ -87,11 +88,32 @@ KR_PROFILE_COUNTERS = 0x2
 
 NRC_RAW, NRC_FEATURES, NRC_PARAMS = 16, 96, 23888
 NRC_TRAIN_ALL = 0xFFFFFFFF  # Nrc(train_log2=...): every path trains
+NRC_FROZEN = 0x1  # Nrc(flags=...): renders query the cache but never train it
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("scene", u32), ("camera", Camera), ("params", Params)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("bvh_width", u32), ("chunk_paths", u32), ("blocks_per_cu", u32), ("flags", u32),
+                ("paths_per_second", C.c_double)]
+
+    def as_dict(self):
+        return {"bvh_width": self.bvh_width, "chunk_paths": self.chunk_paths, "blocks_per_cu": self.blocks_per_cu,
+                "flags": self.flags, "trial_paths_per_second": self.paths_per_second}
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("total_paths", u64), ("total_pixels", u64), ("num_jobs", u32), ("num_scenes", u32),
+                ("graph_nodes", u32), ("chunks", u32), ("tuning", BatchTuning), ("seconds", C.c_double),
+                ("capture_seconds", C.c_double)]
 
 
 class NrcConfig(C.Structure):
     _fields_ = [("box_min", f32 * 3), ("box_max", f32 * 3), ("learning_rate", f32), ("seed", u32),
-                ("batch", u32), ("steps_per_frame", u32), ("train_log2", u32), ("rough_threshold", f32)]
+                ("batch", u32), ("steps_per_frame", u32), ("train_log2", u32), ("rough_threshold", f32),
+                ("flags", u32)]
 
 
 EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "kr_device_close",
@@ -100,7 +122,9 @@ EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "
            "kr_render_reset", "kr_render_read", "kr_render_get_stats", "kr_render_free",
            "kr_render_radiance", "kr_render_set_profiling", "kr_device_bandwidth", "kr_nrc_create", "kr_nrc_free",
            "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
-           "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc", "kr_nrc_read_pool"]
+           "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc", "kr_nrc_read_pool",
+           "kr_batch_create", "kr_batch_autotune", "kr_batch_run", "kr_batch_sync", "kr_batch_reset", "kr_batch_read",
+           "kr_batch_get_info", "kr_batch_free"]
 
 _lib = None
 
@@ -146,6 +170,15 @@ def load(path: str = LIB_PATH):
     lib.kr_nrc_get_stats.argtypes = [vp, C.POINTER(u64), C.POINTER(f32)]
     lib.kr_render_attach_nrc.argtypes = [vp, vp]
     lib.kr_nrc_read_pool.argtypes = [vp, u32, vp, vp, C.POINTER(u32)]
+    lib.kr_batch_create.argtypes = [vp, C.POINTER(SceneDesc), u32, C.POINTER(BatchJob), u32,
+                                    C.POINTER(BatchTuning), C.POINTER(vp)]
+    lib.kr_batch_autotune.argtypes = [vp, u32, C.POINTER(BatchTuning)]
+    lib.kr_batch_run.argtypes = [vp]
+    lib.kr_batch_sync.argtypes = [vp]
+    lib.kr_batch_reset.argtypes = [vp]
+    lib.kr_batch_read.argtypes = [vp, u32, vp, vp]
+    lib.kr_batch_get_info.argtypes = [vp, C.POINTER(BatchInfo)]
+    lib.kr_batch_free.argtypes = [vp]
     for name in EXPORTS:
         if name not in ("kr_version",):
             getattr(lib, name).restype = C.c_int
@@ -303,10 +336,10 @@ class Nrc:
     """kr_nrc handle: the neural radiance cache on one device."""
 
     def __init__(self, dev: Device, box_min, box_max, seed=5, learning_rate=0.0, batch=0, steps_per_frame=0,
-                 train_log2=0, rough_threshold=0.0):
+                 train_log2=0, rough_threshold=0.0, flags=0):
         self.lib = dev.lib
         cfg = NrcConfig((f32 * 3)(*box_min), (f32 * 3)(*box_max), learning_rate, seed, batch, steps_per_frame,
-                        train_log2, rough_threshold)
+                        train_log2, rough_threshold, flags)
         h = C.c_void_p()
         check(self.lib.kr_nrc_create(dev.h, C.byref(cfg), C.byref(h)), "kr_nrc_create")
         self.h = h
@@ -364,4 +397,55 @@ class Nrc:
     def free(self):
         if self.h:
             self.lib.kr_nrc_free(self.h)
+            self.h = None
+
+
+class Batch:
+    """kr_batch handle: many (scene, camera, params) jobs rendered as one
+    wavefront, replayed from one captured HIP graph.
+
+    scenes: objects with .to_desc() (SceneData, host.LoadedScene);
+    jobs: (scene index, Camera, Params) triples."""
+
+    def __init__(self, dev: Device, scenes, jobs, tuning: BatchTuning | None = None):
+        self.lib = dev.lib
+        holders = [sc.to_desc() for sc in scenes]
+        descs = (SceneDesc * len(holders))(*[hd.desc for hd in holders])
+        js = (BatchJob * len(jobs))(*[BatchJob(k, cam, prm) for k, cam, prm in jobs])
+        self.sizes = [(prm.width * prm.height) for _, _, prm in jobs]
+        h = C.c_void_p()
+        check(self.lib.kr_batch_create(dev.h, descs, len(holders), js, len(jobs),
+                                       C.byref(tuning) if tuning is not None else None, C.byref(h)),
+              "kr_batch_create")
+        self.h = h
+
+    def autotune(self, trial_spp: int = 0) -> BatchTuning:
+        t = BatchTuning()
+        check(self.lib.kr_batch_autotune(self.h, trial_spp, C.byref(t)), "kr_batch_autotune")
+        return t
+
+    def run(self):
+        check(self.lib.kr_batch_run(self.h), "kr_batch_run")
+
+    def sync(self):
+        check(self.lib.kr_batch_sync(self.h), "kr_batch_sync")
+
+    def reset(self):
+        check(self.lib.kr_batch_reset(self.h), "kr_batch_reset")
+
+    def read(self, job: int):
+        n = self.sizes[job] if 0 <= job < len(self.sizes) else 0  # out of range: the library reports it
+        rgb = np.zeros((n, 3), np.float64)
+        valid = np.zeros(n, np.uint32)
+        check(self.lib.kr_batch_read(self.h, job, _ptr(rgb), _ptr(valid)), "kr_batch_read")
+        return rgb, valid
+
+    def info(self) -> BatchInfo:
+        i = BatchInfo()
+        check(self.lib.kr_batch_get_info(self.h, C.byref(i)), "kr_batch_get_info")
+        return i
+
+    def free(self):
+        if self.h:
+            self.lib.kr_batch_free(self.h)
             self.h = None

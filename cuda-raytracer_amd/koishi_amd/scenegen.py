@@ -102,7 +102,7 @@ class SceneData:
             arr[i] = shader_desc_from_config(self.shaders[n])
         return names, arr
 
-    def to_desc(self, bvh_width: int = 0, leaf_size: int = 0):
+    def to_desc(self, bvh_width: int = 0, leaf_size: int = 0, flags: int = 0):
         names, sh = self.shader_descs()
         index = {n: i for i, n in enumerate(names)}
         prims = (kr.PrimDesc * (len(self.meshes) + len(self.spheres)))()
@@ -129,6 +129,7 @@ class SceneData:
         desc.light_seed = self.light_seed
         desc.bvh_width = bvh_width
         desc.leaf_size = leaf_size
+        desc.flags = flags
         holder = DescHolder(desc, [pos, nrm, idx, prims, sh])
         return holder
 
@@ -362,6 +363,18 @@ def quad(center, sx, sz, normal_sign=-1.0):
     return v, n, t
 
 
+def rect(p0, e1, e2, normal):
+    """Parallelogram p0, p0+e1, p0+e1+e2, p0+e2 (Y-up), 2 triangles wound so
+    that their geometric normal points along `normal`."""
+    p0, e1, e2, normal = (np.asarray(x, np.float64) for x in (p0, e1, e2, normal))
+    v = np.array([p0, p0 + e1, p0 + e1 + e2, p0 + e2])
+    n = np.tile(normal / np.linalg.norm(normal), (4, 1))
+    t = np.array([[0, 1, 2], [0, 2, 3]], np.int64)
+    if np.dot(np.cross(e1, e2), normal) < 0:
+        t = t[:, [0, 2, 1]]
+    return v, n, t
+
+
 class Builder:
     """Accumulates Y-up meshes, then rotates everything into Z-up."""
 
@@ -460,6 +473,44 @@ def mesh_scene(seed: int = 2, freq: int = 71, width: int = 1024, height: int = 5
     return b.finish((0.0, 1.0, -4.0), (0.0, 0.0, 0.0), 50.0, width, height, spp, max_bounce, "mesh")
 
 
+def room_scene(seed: int = 6, freq: int = 20, width: int = 256, height: int = 192, spp: int = 16,
+               max_bounce: int = 8, turn_deg: float = 25.0) -> SceneData:
+    """Closed room (4 x 3 x 4, Y-up): white floor, ceiling and back wall, red
+    and green side walls, a white wall behind the camera, a 1 x 1 ceiling
+    light of emission 15 and a displaced icosphere (Beckmann 0.4). Indirect
+    light carries a large share of the image, unlike the open workloads.
+    Everything is turned by turn_deg about the vertical, so that no wall
+    normal ends up along +-Y after the Z-up rotation (the reference's shading
+    frame is NaN there)."""
+    c, s_ = math.cos(math.radians(turn_deg)), math.sin(math.radians(turn_deg))
+    turn = np.array([[c, 0.0, s_], [0.0, 1.0, 0.0], [-s_, 0.0, c]])
+    b = Builder()
+    b.shaders["white"] = ["Lambert", {"color": [0.75, 0.75, 0.75]}]
+    b.shaders["red"] = ["Lambert", {"color": [0.7, 0.15, 0.12]}]
+    b.shaders["green"] = ["Lambert", {"color": [0.15, 0.6, 0.15]}]
+    b.shaders["mesh"] = _glossy([0.8, 0.7, 0.5], 0.4)
+    b.shaders["light"] = ["Emission", {"color": [1.0, 1.0, 1.0], "strength": 15.0}]
+
+    def add(name, shader, v, n, t):
+        b.add_mesh(name, shader, np.asarray(v) @ turn.T, np.asarray(n) @ turn.T, t)
+
+    X, Y, Z = 2.0, 3.0, 2.0
+    walls = [("floor", "white", (-X, 0, -Z), (2 * X, 0, 0), (0, 0, 2 * Z), (0, 1, 0)),
+             ("ceiling", "white", (-X, Y, -Z), (2 * X, 0, 0), (0, 0, 2 * Z), (0, -1, 0)),
+             ("back", "white", (-X, 0, Z), (2 * X, 0, 0), (0, Y, 0), (0, 0, -1)),
+             ("front", "white", (-X, 0, -Z), (2 * X, 0, 0), (0, Y, 0), (0, 0, 1)),
+             ("left", "red", (-X, 0, -Z), (0, 0, 2 * Z), (0, Y, 0), (1, 0, 0)),
+             ("right", "green", (X, 0, -Z), (0, 0, 2 * Z), (0, Y, 0), (-1, 0, 0))]
+    for name, shader, p0, e1, e2, nn in walls:
+        add(name, shader, *rect(p0, e1, e2, nn))
+    pos, nrm, tris = displaced_icosphere(freq, seed)
+    add("mesh", "mesh", pos * 0.8 + np.array([0.3, 0.95, 0.4]), nrm, tris)
+    add("light", "light", *quad((0.0, Y - 0.01, 0.0), 1.0, 1.0, -1.0))
+    eye = turn @ np.array([0.0, 1.5, -1.9])
+    look = turn @ np.array([0.0, 1.2, 0.0])
+    return b.finish(tuple(eye), tuple(look), 70.0, width, height, spp, max_bounce, "room")
+
+
 def quat_matrix(rng: SplitMix64):
     """Uniform random rotation (Shoemake, Graphics Gems III, 1992)."""
     u1, u2, u3 = rng.uniform(), rng.uniform(), rng.uniform()
@@ -505,8 +556,34 @@ def large_scene(seed: int = 3, instances: int = 40, freq: int = 36, width: int =
     return b.finish((0.0, 9.0, -20.0), (0.0, 0.0, 0.0), 60.0, width, height, spp, max_bounce, "large")
 
 
+BATCH_RESOLUTIONS = ((256, 256), (512, 512), (1024, 512), (1920, 1080))
+
+
+def batch_workload(seed: int = 4, jobs: int = 16, max_bounce: int = 8):
+    """Batched uneven workload: `jobs` independent scenes from the large-scene
+    generator (scene k seeded 4000 + k), each with a triangle count T
+    log-uniform in [10k, 2M] (instances = clamp(round(T / 25920), 1, 40),
+    icosphere frequency round(sqrt(T / (20 instances)))), a resolution drawn
+    from 256^2, 512^2, 1024x512, 1920x1080 and spp uniform in {16..256};
+    depth 8. Returns (scenes, specs) with specs[k] = dict(scene=k, width,
+    height, spp, max_bounce, seed)."""
+    rng = SplitMix64(seed)
+    scenes, specs = [], []
+    for k in range(jobs):
+        t = math.exp(rng.uniform(math.log(1e4), math.log(2e6)))
+        inst = int(min(40, max(1, round(t / 25920))))
+        freq = int(max(2, round(math.sqrt(t / (20 * inst)))))
+        w, h = BATCH_RESOLUTIONS[int(rng.uniform() * len(BATCH_RESOLUTIONS)) % len(BATCH_RESOLUTIONS)]
+        spp = 16 + int(rng.uniform() * 241) % 241
+        scenes.append(large_scene(seed=4000 + k, instances=inst, freq=freq, width=w, height=h, spp=spp,
+                                  max_bounce=max_bounce))
+        specs.append(dict(scene=k, width=w, height=h, spp=spp, max_bounce=max_bounce, seed=4000 + k))
+    return scenes, specs
+
+
 WORKLOADS = {
     "spheres": sphere_scene,
     "mesh": mesh_scene,
     "large": large_scene,
+    "room": room_scene,
 }

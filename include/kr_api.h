@@ -110,6 +110,9 @@ typedef struct kr_scene_desc {
 /* No area lights are derived: the reference skips light promotion when any
  * shader name has no configuration (src/core/meta/scene.cu:193-216). */
 #define KR_SCENE_NO_AREA_LIGHTS 0x1u
+/* Keep BVH child boxes as full FP32 (7W floats per node) instead of the
+ * default 8-bit quantised boxes (16 + 10W bytes per node). */
+#define KR_SCENE_FULL_NODES 0x2u
 
 typedef struct kr_scene_info {
     uint32_t num_prims, num_triangles, num_spheres, num_lights;
@@ -288,7 +291,10 @@ typedef struct kr_nrc_config {
                                      31); 0 = 4; KR_NRC_TRAIN_ALL: every path  */
     float rough_threshold;        /* GlossyBSDF roughness counted as rough;
                                      0 = 0.3                                   */
+    uint32_t flags;               /* KR_NRC_* bits                             */
 } kr_nrc_config;
+
+#define KR_NRC_FROZEN 0x1u /* renders query the cache but never train it */
 
 int kr_nrc_create(kr_device *dev, const kr_nrc_config *config, kr_nrc **out);
 int kr_nrc_free(kr_nrc *nrc);
@@ -319,6 +325,63 @@ int kr_nrc_get_stats(kr_nrc *nrc, uint64_t *samples_trained, float *last_loss);
  * each kr_render_run the cache takes steps_per_frame Adam steps on the most
  * recent samples. */
 int kr_render_attach_nrc(kr_render *render, kr_nrc *nrc);
+
+/* ---- batched renders ------------------------------------------------------
+ * No counterpart in the reference, which renders one scene per `cr` process
+ * (src/cr.cu:53-60). A batch holds many jobs (scene, camera, params) of
+ * uneven size and renders all their samples as ONE wavefront: every chunk of
+ * paths may mix jobs, each path finds its job by its index, and the whole
+ * sequence of kernels for all chunks is captured once into a HIP graph and
+ * replayed by kr_batch_run with a single graph launch. A job's image is
+ * bit-identical to kr_render_radiance of the same scene, camera and params.
+ * Registered on the host side as the tracer "HIPBatched". */
+typedef struct kr_batch kr_batch;
+
+typedef struct kr_batch_job {
+    uint32_t scene;    /* index into the batch's scene list                  */
+    kr_camera camera;
+    kr_params params;  /* width, height, spp, max_bounce, seed; chunk_paths
+                          and flags are batch-wide (kr_batch_tuning)         */
+} kr_batch_job;
+
+typedef struct kr_batch_tuning {
+    uint32_t bvh_width;     /* 2, 4 or 8                                     */
+    uint32_t chunk_paths;   /* paths in flight per wavefront chunk           */
+    uint32_t blocks_per_cu; /* 256-thread blocks per CU of every launch (1..16) */
+    uint32_t flags;         /* KR_RENDER_* bits                              */
+    double paths_per_second;/* kr_batch_autotune: the winner's trial rate    */
+} kr_batch_tuning;
+
+typedef struct kr_batch_info {
+    uint64_t total_paths;   /* sum of width * height * spp over the jobs     */
+    uint64_t total_pixels;
+    uint32_t num_jobs, num_scenes;
+    uint32_t graph_nodes;   /* kernels captured in the batch's HIP graph     */
+    uint32_t chunks;
+    kr_batch_tuning tuning; /* in force                                      */
+    double seconds;         /* device time of the last kr_batch_run (events) */
+    double capture_seconds; /* host time of the last graph capture + instantiate */
+} kr_batch_info;
+
+/* Copies the scene descriptions (the batch rebuilds their BVHs when the
+ * tuning changes the width) and uploads them. tuning NULL: width 4, 32 M
+ * paths per chunk, 8 blocks per CU. */
+int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint32_t num_scenes, const kr_batch_job *jobs,
+                    uint32_t num_jobs, const kr_batch_tuning *tuning, kr_batch **out);
+/* Autotuner: times trial renders of the first trial_spp samples of every
+ * job (0: enough samples for about 64 M paths) for every BVH width in
+ * {2, 4, 8}, chunk size in {4, 8, 16, 32} M paths and blocks per CU in
+ * {4, 8, 16}, applies the fastest and returns it. */
+int kr_batch_autotune(kr_batch *batch, uint32_t trial_spp, kr_batch_tuning *chosen);
+/* Renders every sample of every job (adds to the accumulators) with one
+ * launch of the captured graph; asynchronous. */
+int kr_batch_run(kr_batch *batch);
+int kr_batch_sync(kr_batch *batch);
+int kr_batch_reset(kr_batch *batch);
+/* Accumulators of one job, as kr_render_read. */
+int kr_batch_read(kr_batch *batch, uint32_t job, double *rgb, uint32_t *valid);
+int kr_batch_get_info(kr_batch *batch, kr_batch_info *info);
+int kr_batch_free(kr_batch *batch);
 
 #ifdef __cplusplus
 }

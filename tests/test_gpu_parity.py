@@ -102,10 +102,14 @@ def test_closest_hit_mesh_scene(device, mesh):
     gs.free()
 
 
-@pytest.mark.parametrize("width", [2, 8])
-def test_closest_hit_bvh_widths(device, mesh, width):
-    gs = kr.GpuScene(device, mesh.to_desc(bvh_width=width))
-    assert gs.info().bvh_width == width
+@pytest.mark.parametrize("width", [2, 4, 8])
+@pytest.mark.parametrize("nodes", ["compressed", "full"])
+def test_closest_hit_bvh_widths(device, mesh, width, nodes):
+    flags = kr.KR_SCENE_FULL_NODES if nodes == "full" else 0
+    gs = kr.GpuScene(device, mesh.to_desc(bvh_width=width, flags=flags))
+    info = gs.info()
+    assert info.bvh_width == width
+    assert info.node_bytes == ((7 * width + 3) // 4 * 16 if nodes == "full" else (16 + 10 * width + 15) // 16 * 16)
     orc = ol.OracleScene(mesh, "f64", traversal=1)
     o, d = _surface_rays(mesh, orc, 30000, 4)
     _compare_closest(gs.trace_closest(o, d), orc.trace_closest(o, d))
