@@ -89,6 +89,78 @@ def rmse_vs_oracle(workload, rgb, valid, width):
             "pixels": int(pix.size), "oracle_samples_per_pixel": int(f["seeds"]) * int(f["spp"])}
 
 
+def bench_cache(args, kr, scene, gs, p, dev, plain_rmse):
+    """Radiance cache on the workload: the plain frame's RMSE against the FP64
+    oracle is the target. The cache (1/16 of paths train, 4 Adam steps of
+    16384 samples per frame = 2^16 samples, lr 1e-2) is trained for a number
+    of warm-up frames, then frames of s = 4 .. spp samples are timed; for
+    each, RMSE vs the oracle and wall-clock are reported next to the plain
+    path at the same s. The wall-clock to the target RMSE is the time of the
+    cheapest cached frame that reaches it (null when the cache's bias floor
+    lies above it)."""
+    import numpy as np
+    pos = np.asarray(scene.positions, np.float64)
+    lo, hi = pos.min(axis=0) - 0.01, pos.max(axis=0) + 0.01
+    nrc = kr.Nrc(dev, lo, hi, train_log2=4, steps_per_frame=4, batch=16384)
+    cam = scene.camera_struct()
+
+    def frame(r, spp):
+        r.reset()
+        t0 = time.perf_counter()
+        r.run(0, spp)
+        r.sync()
+        dt = time.perf_counter() - t0
+        rgb, val = r.read()
+        return dt, rmse_vs_oracle(args.workload, rgb, val, p.width)
+
+    warm = 0
+    t_warm = time.perf_counter()
+    rw = kr.Render(gs, cam, scene.params(seed=7, spp=4))
+    rw.attach_nrc(nrc)
+    for warm in range(1, 33):
+        rw.reset()
+        rw.run(0, 4)
+    rw.sync()
+    t_warm = time.perf_counter() - t_warm
+    rw.free()
+    log(f"[bench] cache warm-up: {warm} frames of 4 spp in {t_warm:.2f}s, loss {nrc.stats()[1]:.3f}")
+    rows = []
+    s = 4
+    while s <= p.spp:
+        res = {"spp": s}
+        for mode in ("plain", "cached"):
+            r = kr.Render(gs, cam, scene.params(seed=11, spp=s))
+            if mode == "cached":
+                r.attach_nrc(nrc)
+            frame(r, s)
+            dts, err = [], None
+            for k in range(2):
+                dt, err = frame(r, s)
+                dts.append(dt)
+            r.free()
+            res[mode] = {"seconds": min(dts), "rmse": err["unclamped"] if err else None,
+                         "rmse_clamped": err["clamped"] if err else None}
+        log(f"[bench] spp {s}: plain {res['plain']['seconds'] * 1e3:.1f} ms rmse {res['plain']['rmse']}, "
+            f"cached {res['cached']['seconds'] * 1e3:.1f} ms rmse {res['cached']['rmse']}")
+        rows.append(res)
+        s *= 2
+    nrc.free()
+    target = plain_rmse
+    reach = [r for r in rows if target is not None and r["cached"]["rmse"] is not None
+             and r["cached"]["rmse"] <= target]
+    plain_time = rows[-1]["plain"]["seconds"] if rows else None
+    cached_time = min((r["cached"]["seconds"] for r in reach), default=None)
+    return {
+        "target_rmse": target,
+        "plain_seconds_to_target": plain_time,
+        "cached_seconds_to_target": cached_time,
+        "speedup": (plain_time / cached_time) if cached_time else None,
+        "warmup_frames": warm, "warmup_seconds": t_warm,
+        "train_fraction": 1 / 16, "samples_per_frame": 4 * 16384,
+        "by_spp": rows,
+    }
+
+
 def bench_batch(args, kr, sg, barrier):
     """Batched uneven workload: 16 seeded scenes of 10k-2M triangles, mixed
     resolutions and 16-256 spp, rendered as one wavefront from one captured
@@ -220,7 +292,13 @@ def main():
     ap.add_argument("--workload", default="mesh", choices=["mesh", "spheres", "large", "batch"])
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--chunk-paths", type=int, default=0)
-    ap.add_argument("--full-nodes", action="store_true", help="FP32 BVH child boxes instead of 8-bit quantised")
+    ap.add_argument("--compressed-nodes", action="store_true",
+                    help="8-bit quantised BVH child boxes instead of FP32")
+    ap.add_argument("--cache", action="store_true",
+                    help="also measure the radiance-cache render: RMSE vs the FP64 oracle and wall-clock "
+                         "against the plain path at equal RMSE")
+    ap.add_argument("--persistent-traversal", action="store_true",
+                    help="traversal kernels refill finished lanes from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-torch", action="store_true", help="skip torch.cuda.synchronize (profiling runs)")
@@ -250,11 +328,12 @@ def main():
     scene = sg.WORKLOADS[args.workload]()
     dev = kr.Device(0)
     gs = kr.GpuScene(dev, scene.to_desc(bvh_width=args.bvh_width,
-                                        flags=kr.KR_SCENE_FULL_NODES if args.full_nodes else 0))
+                                        flags=kr.KR_SCENE_COMPRESSED_NODES if args.compressed_nodes else 0))
     info = gs.info()
     log(f"[bench] scene: {info.num_triangles} tris, {info.num_spheres} spheres, {info.num_lights} lights, "
         f"BVH{info.bvh_width} {info.bvh_nodes} nodes depth {info.bvh_depth}, build {info.build_seconds:.2f}s")
-    p = scene.params(seed=1, chunk_paths=args.chunk_paths)
+    p = scene.params(seed=1, chunk_paths=args.chunk_paths,
+                     flags=kr.KR_RENDER_PERSISTENT_TRAVERSAL if args.persistent_traversal else 0)
     rend = kr.Render(gs, scene.camera_struct(), p)
     paths_per_step = p.width * p.height * p.spp
 
@@ -296,10 +375,15 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_trace_mesh_traffic.json")
     # the committed counter summary is for the default mesh frame only
-    if (args.workload == "mesh" and info.bvh_width == 4 and not args.chunk_paths and not args.full_nodes
-            and os.path.exists(tpath)):
+    if (args.workload == "mesh" and info.bvh_width == 4 and not args.chunk_paths and not args.compressed_nodes
+            and not args.persistent_traversal and os.path.exists(tpath)):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cached = None
+    if args.cache:
+        rend.set_profiling(0)
+        cached = bench_cache(args, kr, scene, gs, p, dev, rmse["unclamped"] if rmse else None)
 
     cpu = None
     if not args.no_cpu:
@@ -335,6 +419,7 @@ def main():
             "wall_clock_to_rmse_s": elapsed / args.steps,
             "kernel_ms_per_step": {"trace": st.trace_ms / args.steps, "shade": st.shade_ms / args.steps,
                                    "shadow": st.shadow_ms / args.steps, "other": st.other_ms / args.steps},
+            "cached": cached,
         },
         "roofline": {
             "bound": "hbm",

@@ -295,9 +295,8 @@ void compress_wide_nodes(const WideBvh &bvh, std::vector<uint32_t> &out) {
             // 254 quanta cover the extent, leaving one for the round-up
             double ext = double(hi[a]) - double(lo[a]);
             int ea = ext > 0 ? int(std::ceil(std::log2(ext / 254.0))) : -125;
-            // one quantum at least one ulp of the coordinates, so that
-            // fma(q, s, p) can reach every float in the box and q = 255
-            // still lies beyond hi after rounding
+            // a quantum finer than one ulp of the coordinates resolves
+            // nothing: keep the grid at least that coarse
             float m = std::max(std::fabs(lo[a]), std::fabs(hi[a]));
             float ulp = std::nextafter(m, std::numeric_limits<float>::infinity()) - m;
             if (ulp > 0) ea = std::max(ea, int(std::ceil(std::log2(double(ulp)))));
@@ -319,6 +318,9 @@ void compress_wide_nodes(const WideBvh &bvh, std::vector<uint32_t> &out) {
                 double rh = std::ceil((double(ch) - double(lo[a])) / double(sc[a]));
                 int ql = int(std::max(0.0, std::min(255.0, rl)));
                 int qh = int(std::max(0.0, std::min(255.0, rh)));
+                // q s is exact and fma rounds once, monotonically, so the
+                // floor / ceil above already contain the box; the loops only
+                // guard the clamped ends
                 while (ql > 0 && qdecode(uint32_t(ql), sc[a], lo[a]) > cl) --ql;
                 while (qh < 255 && qdecode(uint32_t(qh), sc[a], lo[a]) < ch) ++qh;
                 q[(2 * a) * W + c] = uint8_t(ql);

@@ -451,6 +451,174 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
     return false;
 }
 
+// ------------------------------------------------ stepwise traversal --
+// The same two traversals cut into steps (one inner node, or one leaf and
+// the pops after it), so that a persistent kernel can refill lanes whose ray
+// is done with new rays while the rest of the wave keeps traversing
+// (Aila and Laine, "Understanding the Efficiency of Ray Traversal on GPUs",
+// HPG 2009). Results are identical to trace_closest / trace_occluded.
+struct TravState {
+    RayBox rb;
+    ShearRay sr;
+    HitRec h;
+    float limit;  // shadow: segment end (with the tri_hit margin); tmax
+    float tmax;
+    uint32_t cur;
+    bool done;
+    bool occluded;
+};
+
+template <int W, int LDS_DEPTH>
+__device__ __forceinline__ void closest_begin(const DevScene &S, f3 o, f3 d, Stack<LDS_DEPTH> &st, TravState &ts) {
+    ts.h.t = __builtin_inff();
+    ts.h.u = ts.h.v = 0.f;
+    ts.h.code = kMissCode;
+    for (uint32_t s = 0; s < S.nspheres; ++s) {
+        double tsph;
+        if (sphere_hit(S.spheres + 4 * s, o, d, tsph)) {
+            float tf = float(tsph);
+            if (tf < ts.h.t) {
+                ts.h.t = tf;
+                ts.h.code = kSphereBit | s;
+            }
+        }
+    }
+    ts.done = S.ntris == 0;
+    ts.rb = make_raybox(o, d);
+    ts.sr = make_shear(o, d);
+    ts.cur = 0;
+    st.sp = 0;
+}
+
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ void closest_step(const DevScene &S, Stack<LDS_DEPTH> &st, TravState &ts, uint32_t &nvis,
+                                             uint32_t &ntri) {
+    uint32_t cur = ts.cur;
+    if (!(cur & kLeafBit)) {
+        NodeView<W> nv;
+        fetch_node<W, Q>(S.nodes, cur, nv);
+        ++nvis;
+        float key[W];
+        uint32_t ref[W];
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            float tn = box_enter(ts.rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
+                                 ts.h.t);
+            bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
+            key[c] = ok ? tn : __builtin_inff();
+            ref[c] = ok ? nv.ref[c] : kEmpty;
+        }
+#pragma unroll
+        for (int i = 1; i < W; ++i) {
+#pragma unroll
+            for (int j = i; j > 0; --j) {
+                bool sw = key[j] < key[j - 1];
+                float tk = sw ? key[j] : key[j - 1];
+                float tk2 = sw ? key[j - 1] : key[j];
+                uint32_t tr = sw ? ref[j] : ref[j - 1];
+                uint32_t tr2 = sw ? ref[j - 1] : ref[j];
+                key[j - 1] = tk;
+                key[j] = tk2;
+                ref[j - 1] = tr;
+                ref[j] = tr2;
+            }
+        }
+#pragma unroll
+        for (int c = W - 1; c >= 1; --c)
+            if (ref[c] != kEmpty) st.push(ref[c], key[c]);
+        if (ref[0] != kEmpty) {
+            ts.cur = ref[0];
+            return;
+        }
+    } else {
+        uint32_t first = cur & kLeafFirstMask;
+        uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4 *tp = S.tris + 3 * size_t(first + k);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            float tt, b1, b2;
+            ++ntri;
+            if (tri_hit(ts.sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, ts.h.t, tt, b1, b2)) {
+                ts.h.t = tt;
+                ts.h.u = b1;
+                ts.h.v = b2;
+                ts.h.code = __float_as_uint(b.w);
+            }
+        }
+    }
+    while (st.sp > 0) {
+        float tn;
+        st.pop(cur, tn);
+        if (tn <= ts.h.t) {
+            ts.cur = cur;
+            return;
+        }
+    }
+    ts.done = true;
+}
+
+template <int W, int LDS_DEPTH>
+__device__ __forceinline__ void occluded_begin(const DevScene &S, f3 o, f3 d, float tmax, Stack<LDS_DEPTH> &st,
+                                               TravState &ts) {
+    ts.occluded = false;
+    for (uint32_t s = 0; s < S.nspheres; ++s) {
+        double tsph;
+        if (sphere_hit(S.spheres + 4 * s, o, d, tsph) && tsph <= double(tmax)) ts.occluded = true;
+    }
+    ts.done = ts.occluded || S.ntris == 0;
+    ts.rb = make_raybox(o, d);
+    ts.sr = make_shear(o, d);
+    ts.tmax = tmax;
+    ts.limit = tmax * 1.00000012f;
+    ts.cur = 0;
+    st.sp = 0;
+}
+
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ void occluded_step(const DevScene &S, Stack<LDS_DEPTH> &st, TravState &ts) {
+    uint32_t cur = ts.cur;
+    if (!(cur & kLeafBit)) {
+        NodeView<W> nv;
+        fetch_node<W, Q>(S.nodes, cur, nv);
+        uint32_t next = kEmpty;
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            float tn = box_enter(ts.rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
+                                 ts.limit);
+            if (nv.ref[c] != kEmpty && tn != __builtin_inff()) {
+                if (next == kEmpty)
+                    next = nv.ref[c];
+                else
+                    st.push(nv.ref[c], tn);
+            }
+        }
+        if (next != kEmpty) {
+            ts.cur = next;
+            return;
+        }
+    } else {
+        uint32_t first = cur & kLeafFirstMask;
+        uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4 *tp = S.tris + 3 * size_t(first + k);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            float tt, b1, b2;
+            if (tri_hit(ts.sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, ts.limit, tt, b1, b2) &&
+                tt <= ts.tmax) {
+                ts.occluded = true;
+                ts.done = true;
+                return;
+            }
+        }
+    }
+    if (st.sp == 0) {
+        ts.done = true;
+        return;
+    }
+    float tn;
+    st.pop(ts.cur, tn);
+}
+
 // ------------------------------------------------------------- shading --
 // H::fromEular (src/util/hemisphere.hpp:66-69)
 __device__ __forceinline__ f3 from_euler(float sinth, float costh, float phi) {

@@ -49,7 +49,15 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 constexpr int kBlock = 256;
-constexpr int kLdsDepth = 12;
+// Traversal stack entries per lane kept in LDS, and the waves per SIMD the
+// traversal kernels ask the register allocator for (1: its own choice).
+#ifndef KR_LDS_DEPTH
+#define KR_LDS_DEPTH 12
+#endif
+#ifndef KR_TRACE_WAVES
+#define KR_TRACE_WAVES 1
+#endif
+constexpr int kLdsDepth = KR_LDS_DEPTH;
 // traversal stack entries per lane beyond the LDS part, in HBM
 constexpr int kSpillDepth = 96;
 
@@ -110,7 +118,7 @@ struct kr_scene {
     kr_device *dev = nullptr;
     DevScene view{};
     uint32_t width = 4;
-    bool qnodes = true;  // compressed nodes (qnode_bytes) or full FP32 boxes
+    bool qnodes = false;  // compressed nodes (qnode_bytes) or full FP32 boxes
     std::vector<void *> allocs;
     // host copies needed to answer queries
     std::vector<kr_prim_desc> prims;
@@ -181,8 +189,15 @@ enum : int {
     C_SHADOW = 5,
     C_NRCQ = 6,   // cache queries
     C_TRAIN = 7,  // training paths
-    C_NUM = 8
+    C_FETCH = 8,  // persistent closest-hit traversal: next ray to fetch
+    C_FETCH_SH = 9,  // persistent shadow traversal: next segment to fetch
+    C_NUM = 10
 };
+// Each counter sits on its own 128-byte line, so that the wave-aggregated
+// atomics of different queues (fetch, buckets, shadow, next rays) do not
+// serialise on one L2 channel.
+constexpr int kCtrStride = 32;
+#define CTR(i) (kCtrStride * (i))
 enum : int { S_CLOSEST = 0, S_SHADOW = 1, S_INVALID = 2, S_PATHS = 3, S_NODES = 4, S_TRIS = 5, S_NUM = 8 };
 enum : int { K_TRACE = 0, K_SHADE = 1, K_SHADOW = 2, K_OTHER = 3, K_KINDS = 4 };
 
@@ -276,12 +291,6 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t *counter, uint32_t n) 
     return base + incl - n;
 }
 
-__device__ __forceinline__ void wave_count(unsigned long long *counter, bool pred) {
-    unsigned long long mask = __ballot(pred);
-    if (mask && __lane_id() == uint32_t(__ffsll((unsigned long long)mask) - 1))
-        atomicAdd(counter, (unsigned long long)__popcll(mask));
-}
-
 // Job owning global path g: the last k with begin[k] <= g.
 __device__ __forceinline__ uint32_t find_job(const JobTable &T, uint64_t g) {
     uint32_t lo = 0, hi = T.n;
@@ -368,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_copy(const float4 *__restrict__ src,
 // Clears the counters whose bit is set in `mask`.
 __global__ void k_reset(uint32_t *counters, uint32_t mask) {
     int i = threadIdx.x;
-    if (i < C_NUM && ((mask >> i) & 1u)) counters[i] = 0;
+    if (i < C_NUM && ((mask >> i) & 1u)) counters[CTR(i)] = 0;
 }
 
 // With a cache attached, the global path indices fall into groups of
@@ -457,21 +466,22 @@ __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, JobTable T, uin
         if (active) q.flags[j] = flags;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[qi] = P;
+        counters[CTR(qi)] = P;
         if (!BATCH && nr.enabled)
-            counters[C_TRAIN] = uint32_t(((g0 + P - 1) >> nr.train_log2) - (g0 >> nr.train_log2) + 1);
+            counters[CTR(C_TRAIN)] = uint32_t(((g0 + P - 1) >> nr.train_log2) - (g0 >> nr.train_log2) + 1);
     }
 }
 
 template <int W, bool COUNT, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock) void k_trace(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
+__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
                                                   uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
                                                   unsigned long long *stats, int sort, uint32_t *spill,
                                                   size_t spill_lanes) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
     __shared__ float st_[kLdsDepth * kBlock];
     auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
-    uint32_t n = counters[qi];
+    uint32_t n = counters[CTR(qi)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_CLOSEST], (unsigned long long)n);
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         uint32_t i = base + threadIdx.x;
@@ -501,15 +511,83 @@ __global__ __launch_bounds__(kBlock) void k_trace(DevScene S, JobTable T, uint64
             uint32_t prim = (h.code & kSphereBit) ? Sx.sphere_prim[h.code & ~kSphereBit] : Sx.tri_prim[h.code];
             mat = Sx.shaders[Sx.prims[prim].shader].type;
         }
-        wave_count(&stats[S_CLOSEST], active);
-        uint32_t p0 = wave_append(&counters[C_BUCKET + 0], hitp && mat == 0);
+        uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
         if (hitp && mat == 0) b0[p0] = i;
         if (sort) {
-            uint32_t p1 = wave_append(&counters[C_BUCKET + 1], hitp && mat == 1);
+            uint32_t p1 = wave_append(&counters[CTR(C_BUCKET + 1)], hitp && mat == 1);
             if (hitp && mat == 1) b1[p1] = i;
-            uint32_t p2 = wave_append(&counters[C_BUCKET + 2], hitp && mat == 2);
+            uint32_t p2 = wave_append(&counters[CTR(C_BUCKET + 2)], hitp && mat == 2);
             if (hitp && mat == 2) b2[p2] = i;
         }
+    }
+}
+
+// Persistent closest-hit traversal: every lane keeps a ray until it is done
+// and takes a new one from the queue (one atomic per wave) once more than
+// 64 - kMinActive lanes of its wave are idle, so waves do not idle behind
+// their slowest ray. Same outputs and bucket appends as k_trace.
+constexpr int kMinActive = 32;
+
+template <int W, bool COUNT, bool BATCH, bool Q>
+__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace_dyn(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
+                                                      uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
+                                                      unsigned long long *stats, int sort, uint32_t *spill,
+                                                      size_t spill_lanes) {
+    __shared__ uint32_t sref[kLdsDepth * kBlock];
+    __shared__ float st_[kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    const uint32_t n = counters[CTR(qi)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_CLOSEST], (unsigned long long)n);
+    uint32_t i = 0, nvis = 0, ntri = 0;
+    bool has = false, exhausted = false;
+    TravState ts;
+    DevScene Sx = S;
+    while (true) {
+        bool want = !has && !exhausted;
+        uint32_t k = wave_append(&counters[CTR(C_FETCH)], want);
+        if (want) {
+            if (k < n) {
+                i = k;
+                has = true;
+                if (BATCH) Sx = T.rec[find_job(T, g0 + q.slot[i])].S;
+                f3 o{q.ox[i], q.oy[i], q.oz[i]}, d{q.dx[i], q.dy[i], q.dz[i]};
+                closest_begin<W, kLdsDepth>(Sx, o, d, st, ts);
+            } else {
+                exhausted = true;
+            }
+        }
+        if (__ballot(has) == 0ull) break;
+        while (true) {
+            if (has && !ts.done) closest_step<W, kLdsDepth, Q>(Sx, st, ts, nvis, ntri);
+            if (__popcll(__ballot(has && !ts.done)) < kMinActive) break;
+        }
+        bool fin = has && ts.done;
+        if (fin) {
+            hit.t[i] = ts.h.t;
+            hit.u[i] = ts.h.u;
+            hit.v[i] = ts.h.v;
+            hit.code[i] = ts.h.code;
+        }
+        bool hitp = fin && ts.h.code != kMissCode;
+        uint32_t mat = 0;
+        if (hitp && sort) {
+            uint32_t c = ts.h.code;
+            uint32_t prim = (c & kSphereBit) ? Sx.sphere_prim[c & ~kSphereBit] : Sx.tri_prim[c];
+            mat = Sx.shaders[Sx.prims[prim].shader].type;
+        }
+        uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
+        if (hitp && mat == 0) b0[p0] = i;
+        if (sort) {
+            uint32_t p1 = wave_append(&counters[CTR(C_BUCKET + 1)], hitp && mat == 1);
+            if (hitp && mat == 1) b1[p1] = i;
+            uint32_t p2 = wave_append(&counters[CTR(C_BUCKET + 2)], hitp && mat == 2);
+            if (hitp && mat == 2) b2[p2] = i;
+        }
+        if (fin) has = false;
+    }
+    if (COUNT) {
+        atomicAdd(&stats[S_NODES], (unsigned long long)nvis);
+        atomicAdd(&stats[S_TRIS], (unsigned long long)ntri);
     }
 }
 
@@ -533,7 +611,7 @@ struct ShadeArgs {
 // one path vertex.
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
-    uint32_t n0 = a.counters[C_BUCKET + 0], n1 = a.counters[C_BUCKET + 1], n2 = a.counters[C_BUCKET + 2];
+    uint32_t n0 = a.counters[CTR(C_BUCKET + 0)], n1 = a.counters[CTR(C_BUCKET + 1)], n2 = a.counters[CTR(C_BUCKET + 2)];
     uint32_t total = n0 + n1 + n2;
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
@@ -734,7 +812,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             }  // not terminated
         }
         if (a.nr.enabled) {
-            uint32_t pq = wave_append(&a.counters[C_NRCQ], push_query);
+            uint32_t pq = wave_append(&a.counters[CTR(C_NRCQ)], push_query);
             if (push_query) {
                 a.nr.qslot[pq] = slot;
                 a.nr.qbeta[3 * size_t(pq)] = nb.x;
@@ -747,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 dst[3] = make_float4(qrec[12], 0.f, 0.f, 0.f);
             }
         }
-        uint32_t ps = wave_append(&a.counters[C_SHADOW], push_shadow);
+        uint32_t ps = wave_append(&a.counters[CTR(C_SHADOW)], push_shadow);
         if (push_shadow) {
             a.sq.slot[ps] = slot;
             a.sq.ox[ps] = so.x;
@@ -761,7 +839,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             a.sq.cy[ps] = contrib.y;
             a.sq.cz[ps] = contrib.z;
         }
-        uint32_t pr = wave_append(&a.counters[a.qni], push_ray);
+        uint32_t pr = wave_append(&a.counters[CTR(a.qni)], push_ray);
         if (push_ray) {
             a.qn.slot[pr] = slot;
             a.qn.ox[pr] = no.x;
@@ -779,13 +857,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
 }
 
 template <int W, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, JobTable T, uint64_t g0, ShadowQ sq, uint32_t *counters,
+__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_shadow(DevScene S, JobTable T, uint64_t g0, ShadowQ sq, uint32_t *counters,
                                                    float *L, unsigned long long *stats, uint32_t *spill,
                                                    size_t spill_lanes, NrcRender nr, uint32_t bounce) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
     __shared__ float st_[kLdsDepth * kBlock];
     auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
-    uint32_t n = counters[C_SHADOW];
+    uint32_t n = counters[CTR(C_SHADOW)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_SHADOW], (unsigned long long)n);
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         uint32_t i = base + threadIdx.x;
@@ -810,7 +889,61 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene S, JobTable T, uint6
                 }
             }
         }
-        wave_count(&stats[S_SHADOW], active);
+    }
+}
+
+// Persistent shadow traversal (lane refill as in k_trace_dyn); same
+// additions to L and to the training records as k_shadow.
+template <int W, bool BATCH, bool Q>
+__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_shadow_dyn(DevScene S, JobTable T, uint64_t g0, ShadowQ sq,
+                                                       uint32_t *counters, float *L, unsigned long long *stats,
+                                                       uint32_t *spill, size_t spill_lanes, NrcRender nr,
+                                                       uint32_t bounce) {
+    __shared__ uint32_t sref[kLdsDepth * kBlock];
+    __shared__ float st_[kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    const uint32_t n = counters[CTR(C_SHADOW)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_SHADOW], (unsigned long long)n);
+    uint32_t i = 0;
+    bool has = false, exhausted = false;
+    TravState ts;
+    DevScene Sx = S;
+    while (true) {
+        bool want = !has && !exhausted;
+        uint32_t k = wave_append(&counters[CTR(C_FETCH_SH)], want);
+        if (want) {
+            if (k < n) {
+                i = k;
+                has = true;
+                if (BATCH) Sx = T.rec[find_job(T, g0 + sq.slot[i])].S;
+                f3 o{sq.ox[i], sq.oy[i], sq.oz[i]}, d{sq.dx[i], sq.dy[i], sq.dz[i]};
+                occluded_begin<W, kLdsDepth>(Sx, o, d, sq.tmax[i], st, ts);
+            } else {
+                exhausted = true;
+            }
+        }
+        if (__ballot(has) == 0ull) break;
+        while (true) {
+            if (has && !ts.done) occluded_step<W, kLdsDepth, Q>(Sx, st, ts);
+            if (__popcll(__ballot(has && !ts.done)) < kMinActive) break;
+        }
+        bool fin = has && ts.done;
+        if (fin && !ts.occluded) {
+            uint32_t slot = sq.slot[i];
+            L[3 * slot] += sq.cx[i];
+            L[3 * slot + 1] += sq.cy[i];
+            L[3 * slot + 2] += sq.cz[i];
+            if (nr.enabled) {
+                uint32_t t = nr.tslot_of[slot];
+                if (t != kMissCode) {
+                    float *rec = nr.rec + (size_t(t) * nr.nv + bounce) * kRecFloats;
+                    rec[19] += sq.cx[i];
+                    rec[20] += sq.cy[i];
+                    rec[21] += sq.cz[i];
+                }
+            }
+        }
+        if (fin) has = false;
     }
 }
 
@@ -1369,7 +1502,7 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     (void)hipSetDevice(dev->ordinal);
     DevScene &v = s->view;
     int rc = KR_OK;
-    s->qnodes = !(d->flags & KR_SCENE_FULL_NODES);
+    s->qnodes = (d->flags & KR_SCENE_COMPRESSED_NODES) != 0;
     if (s->qnodes) {
         std::vector<uint32_t> qn;
         compress_wide_nodes(bvh, qn);
@@ -1665,7 +1798,7 @@ int render_buffers(kr_render *r, size_t lanes) {
             if (!rc) rc = get(f, P);
     }
     if (!rc) rc = get(&r->L, 3 * P);
-    if (!rc) rc = get(&r->counters, C_NUM);
+    if (!rc) rc = get(&r->counters, size_t(C_NUM) * kCtrStride);
     if (!rc) rc = get(&r->stats, S_NUM);
     if (!rc) rc = get(&r->acc, 3 * size_t(r->npix));
     if (!rc) rc = get(&r->valid, r->npix);
@@ -1715,7 +1848,7 @@ KR_EXPORT int kr_render_reset(kr_render *r) {
     KR_HIP(hipMemsetAsync(r->acc, 0, sizeof(double) * 3 * size_t(r->npix), st));
     KR_HIP(hipMemsetAsync(r->valid, 0, sizeof(uint32_t) * size_t(r->npix), st));
     KR_HIP(hipMemsetAsync(r->stats, 0, sizeof(unsigned long long) * S_NUM, st));
-    KR_HIP(hipMemsetAsync(r->counters, 0, sizeof(uint32_t) * C_NUM, st));
+    KR_HIP(hipMemsetAsync(r->counters, 0, sizeof(uint32_t) * C_NUM * kCtrStride, st));
     KR_HIP(hipStreamSynchronize(st));
     resolve_events(r);
     for (int k = 0; k < K_KINDS; ++k) {
@@ -1801,25 +1934,27 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
                            uint32_t(r->prm.seed >> 32));
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
+    bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
     for (uint32_t b = 0; b <= r->prm.max_bounce; ++b) {
         int qi = cur == 0 ? C_RAY0 : C_RAY1;
         int qni = cur == 0 ? C_RAY1 : C_RAY0;
         {
             // clear buckets, shadow count and the next ray queue's count
             Timed tm(r, K_OTHER);
-            uint32_t mask = (0x7u << C_BUCKET) | (1u << C_SHADOW) | (1u << qni);
+            uint32_t mask = (0x7u << C_BUCKET) | (1u << C_SHADOW) | (1u << qni) | (1u << C_FETCH) | (1u << C_FETCH_SH);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, mask);
         }
         {
             Timed tm(r, K_TRACE);
-            if (count)
-                hipLaunchKernelGGL((k_trace<W, true, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->q[cur],
-                                   qi, r->hit, r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort,
-                                   r->spill, r->spill_lanes);
+            auto trace = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->q[cur], qi, r->hit,
+                                   r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort, r->spill,
+                                   r->spill_lanes);
+            };
+            if (dyn)
+                count ? trace(k_trace_dyn<W, true, BATCH, Q>) : trace(k_trace_dyn<W, false, BATCH, Q>);
             else
-                hipLaunchKernelGGL((k_trace<W, false, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->q[cur],
-                                   qi, r->hit, r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort,
-                                   r->spill, r->spill_lanes);
+                count ? trace(k_trace<W, true, BATCH, Q>) : trace(k_trace<W, false, BATCH, Q>);
         }
         ShadeArgs a;
         a.S = S;
@@ -1849,8 +1984,12 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
         }
         {
             Timed tm(r, K_SHADOW);
-            hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq, r->counters,
-                               r->L, r->stats, r->spill, r->spill_lanes, r->nr, b);
+            if (dyn)
+                hipLaunchKernelGGL((k_shadow_dyn<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq,
+                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b);
+            else
+                hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq,
+                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b);
         }
         cur = 1 - cur;
     }
@@ -2270,7 +2409,7 @@ KR_EXPORT int kr_nrc_get_stats(kr_nrc *n, uint64_t *samples_trained, float *last
 __global__ __launch_bounds__(256) void k_nrc_infer(nrc::Weights W, NrcRender nr, const uint32_t *counters, float *L) {
     using namespace nrc;
     int lane = __lane_id(), s = lane & 15, q = lane >> 4;
-    uint32_t n = counters[C_NRCQ];
+    uint32_t n = counters[CTR(C_NRCQ)];
     uint32_t waves = gridDim.x * (blockDim.x / 64);
     for (uint32_t base = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * 64; base < n; base += waves * 64) {
         v4s in[6][4];
@@ -2298,7 +2437,7 @@ __global__ __launch_bounds__(256) void k_nrc_infer(nrc::Weights W, NrcRender nr,
 // where c_k is everything vertex k added to the path's L.
 __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_t *counters, float *pool_raw,
                                                      float *pool_tgt, uint32_t *pool_total, uint32_t cap) {
-    uint32_t ntrain = min(counters[C_TRAIN], nr.train_cap);
+    uint32_t ntrain = min(counters[CTR(C_TRAIN)], nr.train_cap);
     uint32_t stride = gridDim.x * blockDim.x;
     // whole waves iterate together so that the ring slots can be reserved
     // with one atomic per wave
@@ -2516,7 +2655,7 @@ int batch_set_width(kr_batch *b, uint32_t width) {
         kr_scene_desc d = b->scenes[i].desc;
         d.bvh_width = width;
         // one node format for all scenes: the kernels are compiled per format
-        d.flags = (d.flags & ~KR_SCENE_FULL_NODES) | (b->scenes[0].desc.flags & KR_SCENE_FULL_NODES);
+        d.flags = (d.flags & ~KR_SCENE_COMPRESSED_NODES) | (b->scenes[0].desc.flags & KR_SCENE_COMPRESSED_NODES);
         int rc = kr_scene_upload(b->dev, &d, &b->up[i]);
         if (rc) {
             // leave no half-uploaded set behind: the next call re-uploads all

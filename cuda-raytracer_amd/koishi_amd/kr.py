@@ -21,6 +21,7 @@ KR_MISS = 0xFFFFFFFF
 KR_MEM_HOST = 0
 KR_MEM_DEVICE = 1
 KR_RENDER_NO_MATERIAL_SORT = 0x1
+KR_RENDER_PERSISTENT_TRAVERSAL = 0x2
 
 SHADER_LAMBERT, SHADER_GLOSSY, SHADER_EMISSION = 0, 1, 2
 DIST_BECKMANN, DIST_COSINE = 0, 1
@@ -49,7 +50,7 @@ class SceneDesc(C.Structure):
 
 
 KR_SCENE_NO_AREA_LIGHTS = 0x1
-KR_SCENE_FULL_NODES = 0x2
+KR_SCENE_COMPRESSED_NODES = 0x2
 
 
 class SceneInfo(C.Structure):

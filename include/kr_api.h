@@ -110,9 +110,10 @@ typedef struct kr_scene_desc {
 /* No area lights are derived: the reference skips light promotion when any
  * shader name has no configuration (src/core/meta/scene.cu:193-216). */
 #define KR_SCENE_NO_AREA_LIGHTS 0x1u
-/* Keep BVH child boxes as full FP32 (7W floats per node) instead of the
- * default 8-bit quantised boxes (16 + 10W bytes per node). */
-#define KR_SCENE_FULL_NODES 0x2u
+/* Store BVH child boxes quantised to 8 bits per plane (16 + 10W bytes per
+ * node) instead of the default full FP32 boxes (28W bytes, 16-byte padded).
+ * Traversal decodes them conservatively: same hits, fewer bytes, more ALU. */
+#define KR_SCENE_COMPRESSED_NODES 0x2u
 
 typedef struct kr_scene_info {
     uint32_t num_prims, num_triangles, num_spheres, num_lights;
@@ -148,6 +149,9 @@ typedef struct kr_params {
 } kr_params;
 
 #define KR_RENDER_NO_MATERIAL_SORT 0x1u /* shade with one mixed-material queue */
+#define KR_RENDER_PERSISTENT_TRAVERSAL 0x2u /* traversal kernels refill lanes whose ray
+                                               is done from the queue (persistent
+                                               threads) instead of a fixed share */
 
 /* ---- ray query buffers -------------------------------------------------- */
 

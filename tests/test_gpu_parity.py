@@ -105,7 +105,7 @@ def test_closest_hit_mesh_scene(device, mesh):
 @pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("nodes", ["compressed", "full"])
 def test_closest_hit_bvh_widths(device, mesh, width, nodes):
-    flags = kr.KR_SCENE_FULL_NODES if nodes == "full" else 0
+    flags = kr.KR_SCENE_COMPRESSED_NODES if nodes == "compressed" else 0
     gs = kr.GpuScene(device, mesh.to_desc(bvh_width=width, flags=flags))
     info = gs.info()
     assert info.bvh_width == width
@@ -257,7 +257,7 @@ def test_render_stats_and_profiling(device, mesh):
     gs.free()
 
 
-@pytest.mark.parametrize("workload", ["spheres", "mesh"])
+@pytest.mark.parametrize("workload", ["spheres", "mesh", "large"])
 def test_unbiased_against_converged_oracle(device, workload):
     """Full-size frame vs the committed FP64 oracle image (same lens strata,
     65536 samples per pixel on a fixed 4096-pixel subset): the squared error
