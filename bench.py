@@ -289,7 +289,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mesh", choices=["mesh", "spheres", "large", "batch"])
+    ap.add_argument("--workload", default="mesh", choices=["mesh", "spheres", "large", "room", "batch"])
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--chunk-paths", type=int, default=0)
     ap.add_argument("--compressed-nodes", action="store_true",
@@ -410,7 +410,9 @@ def main():
                          "spheres": "sphere scene: 8 spheres + ground sphere + quad light, 256x256, 16 spp, "
                                     "maxBounce 5",
                          "large": "large scene: 40 displaced icosphere instances 1,036,800 tris, 4 quad "
-                                  "lights, 1920x1080, 64 spp, maxBounce 8"}[args.workload],
+                                  "lights, 1920x1080, 64 spp, maxBounce 8",
+                         "room": "closed room: 6 walls, displaced icosphere 8,000 tris, ceiling light, "
+                                 "1024x768, 64 spp, maxBounce 8"}[args.workload],
             "width": p.width, "height": p.height, "spp": p.spp, "max_bounce": p.max_bounce,
             "triangles": info.num_triangles, "bvh_width": info.bvh_width, "bvh_node_bytes": info.node_bytes,
             "paths_per_step": paths_per_step,

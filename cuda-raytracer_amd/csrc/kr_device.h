@@ -399,6 +399,13 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
 // Any hit on the segment (o, d, tmax): Scene::intersect(Seg)
 // (src/core/meta/scene.hpp:57-67). Triangles block for 0 < t <= tmax,
 // spheres for a ray hit with t <= tmax (src/core/primitive/primitive.hpp:16-21).
+// Scheduled "while-while" with postponed leaves (Aila and Laine, HPG 2009):
+// a lane that reaches a leaf parks it and keeps descending until every lane
+// of the wave holds a leaf, then all lanes test triangles together, so the
+// node and leaf code run in separate wave-wide loops instead of alternating
+// under divergence (measured faster than the alternating loop for shadow
+// segments; the closest-hit traversal keeps the alternating loop, which
+// measured faster there).
 template <int W, int LDS_DEPTH, bool Q = false>
 __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, float tmax, Stack<LDS_DEPTH> &st) {
     for (uint32_t s = 0; s < S.nspheres; ++s) {
@@ -410,10 +417,17 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
     ShearRay sr = make_shear(o, d);
     // tri_hit accepts t < limit; the segment includes its end point.
     float limit = tmax * 1.00000012f;
-    uint32_t cur = 0;
     st.sp = 0;
+    auto pop_next = [&]() -> uint32_t {
+        if (st.sp == 0) return kEmpty;
+        uint32_t r;
+        float tn;
+        st.pop(r, tn);
+        return r;
+    };
+    uint32_t cur = 0, leaf = kEmpty;
     while (true) {
-        if (!(cur & kLeafBit)) {
+        while (cur != kEmpty && !(cur & kLeafBit)) {
             NodeView<W> nv;
             fetch_node<W, Q>(S.nodes, cur, nv);
             uint32_t next = kEmpty;
@@ -428,13 +442,24 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
                         st.push(nv.ref[c], tn);
                 }
             }
-            if (next != kEmpty) {
-                cur = next;
-                continue;
+            cur = next != kEmpty ? next : pop_next();
+            if (cur != kEmpty && (cur & kLeafBit) && leaf == kEmpty) {
+                leaf = cur;
+                cur = pop_next();
             }
-        } else {
-            uint32_t first = cur & kLeafFirstMask;
-            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+            if (!__any(leaf == kEmpty)) break;  // every lane still descending holds a leaf
+        }
+        while (leaf != kEmpty || (cur != kEmpty && (cur & kLeafBit))) {
+            uint32_t lf;
+            if (leaf != kEmpty) {
+                lf = leaf;
+                leaf = kEmpty;
+            } else {
+                lf = cur;
+                cur = pop_next();
+            }
+            uint32_t first = lf & kLeafFirstMask;
+            uint32_t cnt = ((lf >> kLeafCountShift) & 0xFu) + 1u;
             for (uint32_t k = 0; k < cnt; ++k) {
                 const float4 *tp = S.tris + 3 * size_t(first + k);
                 float4 a = tp[0], b = tp[1], c = tp[2];
@@ -444,9 +469,7 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
                     return true;
             }
         }
-        if (st.sp == 0) break;
-        float tn;
-        st.pop(cur, tn);
+        if (cur == kEmpty) break;
     }
     return false;
 }

@@ -473,7 +473,7 @@ def mesh_scene(seed: int = 2, freq: int = 71, width: int = 1024, height: int = 5
     return b.finish((0.0, 1.0, -4.0), (0.0, 0.0, 0.0), 50.0, width, height, spp, max_bounce, "mesh")
 
 
-def room_scene(seed: int = 6, freq: int = 20, width: int = 256, height: int = 192, spp: int = 16,
+def room_scene(seed: int = 6, freq: int = 20, width: int = 1024, height: int = 768, spp: int = 64,
                max_bounce: int = 8, turn_deg: float = 25.0) -> SceneData:
     """Closed room (4 x 3 x 4, Y-up): white floor, ceiling and back wall, red
     and green side walls, a white wall behind the camera, a 1 x 1 ceiling

@@ -189,7 +189,7 @@ def test_cached_render_learns_and_gains(device):
     per-pixel error, with every pixel valid and finite. (The remaining bias
     is the cache's error after 16 short frames of online training.)"""
     from koishi_amd import scenegen as sg
-    s = sg.room_scene()  # closed room: most radiance arrives after the second vertex
+    s = sg.room_scene(width=256, height=192, spp=16)  # closed room: most radiance arrives after the second vertex
     gs = kr.GpuScene(device, s.to_desc())
     p = s.params(seed=2)
     ref = 0
