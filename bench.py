@@ -89,19 +89,24 @@ def rmse_vs_oracle(workload, rgb, valid, width):
             "pixels": int(pix.size), "oracle_samples_per_pixel": int(f["seeds"]) * int(f["spp"])}
 
 
-def bench_cache(args, kr, scene, gs, p, dev, plain_rmse):
-    """Radiance cache on the workload: the plain frame's RMSE against the FP64
-    oracle is the target. The cache (1/16 of paths train, 4 Adam steps of
-    16384 samples per frame = 2^16 samples, lr 1e-2) is trained for a number
-    of warm-up frames, then frames of s = 4 .. spp samples are timed; for
-    each, RMSE vs the oracle and wall-clock are reported next to the plain
-    path at the same s. The wall-clock to the target RMSE is the time of the
-    cheapest cached frame that reaches it (null when the cache's bias floor
-    lies above it)."""
+def bench_cache(args, kr, scene, gs, p, dev):
+    """Radiance cache on the workload against the plain path. The cache
+    (1/16 of paths train, 4 Adam steps of 16384 samples per frame = 2^16
+    samples, lr 1e-2, targets clamped at the brightest emitter's radiance) is
+    trained for a number of 4-spp warm-up frames, then frames of s = 4 .. spp
+    samples are timed for both paths, with RMSE against the FP64 oracle
+    averaged over 3 seeds. The target is the plain path's clamped RMSE at the
+    workload's spp; the cached wall-clock to it is the time of the cheapest
+    cached frame that reaches it (null when the cache's bias lies above)."""
     import numpy as np
     pos = np.asarray(scene.positions, np.float64)
     lo, hi = pos.min(axis=0) - 0.01, pos.max(axis=0) + 0.01
-    nrc = kr.Nrc(dev, lo, hi, train_log2=4, steps_per_frame=4, batch=16384)
+    # training targets clamped at the brightest emitter's radiance: expected
+    # reflected radiance stays below it, larger single-path targets are noise
+    emax = max([max(c.get("color", [1, 1, 1])) * c.get("strength", 2.0)
+                for t, c in ((v[0], v[1] if len(v) > 1 else {}) for v in scene.shaders.values()) if t == "Emission"]
+               + [max(col) * k for _, col, k in scene.point_lights] + [1.0])
+    nrc = kr.Nrc(dev, lo, hi, train_log2=4, steps_per_frame=4, batch=16384, target_clamp=emax)
     cam = scene.camera_struct()
 
     def frame(r, spp):
@@ -113,11 +118,27 @@ def bench_cache(args, kr, scene, gs, p, dev, plain_rmse):
         rgb, val = r.read()
         return dt, rmse_vs_oracle(args.workload, rgb, val, p.width)
 
+    def mse_over_seeds(mode, s):
+        """Best time and the MSE averaged over 3 independent seeds (single
+        64-spp frames of these scenes differ by tens of percent in RMSE)."""
+        dts, e, ec = [], 0.0, 0.0
+        for seed in (11, 12, 13):
+            r = kr.Render(gs, cam, scene.params(seed=seed, spp=s))
+            if mode == "cached":
+                r.attach_nrc(nrc)
+            frame(r, s)
+            dt, err = frame(r, s)
+            r.free()
+            dts.append(dt)
+            e += err["unclamped"] ** 2 / 3
+            ec += err["clamped"] ** 2 / 3
+        return {"seconds": min(dts), "rmse": e ** 0.5, "rmse_clamped": ec ** 0.5}
+
     warm = 0
     t_warm = time.perf_counter()
     rw = kr.Render(gs, cam, scene.params(seed=7, spp=4))
     rw.attach_nrc(nrc)
-    for warm in range(1, 33):
+    for warm in range(1, args.cache_warmup + 1):
         rw.reset()
         rw.run(0, 4)
     rw.sync()
@@ -129,34 +150,38 @@ def bench_cache(args, kr, scene, gs, p, dev, plain_rmse):
     while s <= p.spp:
         res = {"spp": s}
         for mode in ("plain", "cached"):
-            r = kr.Render(gs, cam, scene.params(seed=11, spp=s))
-            if mode == "cached":
-                r.attach_nrc(nrc)
-            frame(r, s)
-            dts, err = [], None
-            for k in range(2):
-                dt, err = frame(r, s)
-                dts.append(dt)
-            r.free()
-            res[mode] = {"seconds": min(dts), "rmse": err["unclamped"] if err else None,
-                         "rmse_clamped": err["clamped"] if err else None}
-        log(f"[bench] spp {s}: plain {res['plain']['seconds'] * 1e3:.1f} ms rmse {res['plain']['rmse']}, "
-            f"cached {res['cached']['seconds'] * 1e3:.1f} ms rmse {res['cached']['rmse']}")
+            res[mode] = mse_over_seeds(mode, s)
+        log(f"[bench] spp {s}: plain {res['plain']['seconds'] * 1e3:.1f} ms rmse {res['plain']['rmse']:.4f} / "
+            f"{res['plain']['rmse_clamped']:.4f} clamped, cached {res['cached']['seconds'] * 1e3:.1f} ms rmse "
+            f"{res['cached']['rmse']:.4f} / {res['cached']['rmse_clamped']:.4f} clamped")
         rows.append(res)
         s *= 2
     nrc.free()
-    target = plain_rmse
-    reach = [r for r in rows if target is not None and r["cached"]["rmse"] is not None
-             and r["cached"]["rmse"] <= target]
+    # the target: the plain frame's clamped RMSE at the workload's spp (what the
+    # reference's 8-bit output shows; the unclamped RMSE of these scenes is
+    # dominated by rare very bright samples and is reported beside it)
+    target = rows[-1]["plain"]["rmse_clamped"] if rows else None
+
+    def bias_fit(mode, key):
+        """Least-squares fit of RMSE^2(N) = b^2 + sigma^2 / N over the frames
+        (bias b, noise sigma); b ~ 0 for an unbiased estimator."""
+        ns = np.array([r["spp"] for r in rows], np.float64)
+        m2 = np.array([r[mode][key] ** 2 for r in rows])
+        A = np.stack([np.ones_like(ns), 1.0 / ns], axis=1)
+        (b2, s2), *_ = np.linalg.lstsq(A, m2, rcond=None)
+        return {"bias": float(np.sign(b2) * np.sqrt(abs(b2))), "sigma": float(np.sqrt(max(s2, 0.0)))}
+    reach = [r for r in rows if target is not None and r["cached"]["rmse_clamped"] <= target]
     plain_time = rows[-1]["plain"]["seconds"] if rows else None
     cached_time = min((r["cached"]["seconds"] for r in reach), default=None)
     return {
-        "target_rmse": target,
+        "target_rmse_clamped": target,
+        "target_clamp": emax,
         "plain_seconds_to_target": plain_time,
         "cached_seconds_to_target": cached_time,
         "speedup": (plain_time / cached_time) if cached_time else None,
         "warmup_frames": warm, "warmup_seconds": t_warm,
         "train_fraction": 1 / 16, "samples_per_frame": 4 * 16384,
+        "bias_fit_clamped": {m: bias_fit(m, "rmse_clamped") for m in ("plain", "cached")} if len(rows) > 1 else None,
         "by_spp": rows,
     }
 
@@ -297,6 +322,8 @@ def main():
     ap.add_argument("--cache", action="store_true",
                     help="also measure the radiance-cache render: RMSE vs the FP64 oracle and wall-clock "
                          "against the plain path at equal RMSE")
+    ap.add_argument("--cache-warmup", type=int, default=64,
+                    help="4-spp training frames before the cached frames are timed")
     ap.add_argument("--persistent-traversal", action="store_true",
                     help="traversal kernels refill finished lanes from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -383,7 +410,7 @@ def main():
     cached = None
     if args.cache:
         rend.set_profiling(0)
-        cached = bench_cache(args, kr, scene, gs, p, dev, rmse["unclamped"] if rmse else None)
+        cached = bench_cache(args, kr, scene, gs, p, dev)
 
     cpu = None
     if not args.no_cpu:

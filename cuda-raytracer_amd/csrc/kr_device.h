@@ -682,29 +682,120 @@ __device__ __forceinline__ float beckmann_f(f3 w, float alpha2) {
 }
 __device__ __forceinline__ float cos_f(f3 w) { return w.z >= 0.f ? kInvPi : 0.f; }
 
-// Shader::execute(compute_f_by_wi_wo) (lambert.hpp:31-33, glossy.hpp:77-79,
-// emission.hpp:117-122).
-__device__ __forceinline__ f3 shader_eval(const ShaderRec &sh, f3 wo, f3 wi) {
-    f3 c{sh.cr[0], sh.cr[1], sh.cr[2]};
-    if (sh.type == KR_SHADER_GLOSSY) {
-        f3 h = normalize(wo + wi);
-        float D = sh.dist == KR_DIST_COSINE ? cos_f(h) : beckmann_f(h, sh.alpha2);
-        return c * D;
-    }
-    return c * cos_f(wi);
+// Fresnel::compute (src/ext/scala/fresnel.hpp:15-30): dielectric
+// reflectance of wo for a surface of index ior.
+__device__ __forceinline__ float fresnel(float ior, f3 wo) {
+    float cosi = wo.z;
+    float e = cosi > 0.f ? 1.f / ior : ior;
+    cosi = fabsf(cosi);
+    float sini = sqrtf(smax(0.f, 1.f - cosi * cosi));
+    float sinr = e * sini;
+    if (sinr >= 1.f) return 1.f;
+    float cosr = sqrtf(smax(0.f, 1.f - sinr * sinr));
+    float cosior = cosi / cosr;
+    float rparl = (cosior - e) / (cosior + e);
+    float rperp = (e * cosior - 1.f) / (e * cosior + 1.f);
+    return .5f * (rparl * rparl + rperp * rperp);
 }
 
-// Shader::execute(sample_wi_f_by_wo) (lambert.hpp:26-30, glossy.hpp:73-76).
-__device__ __forceinline__ void shader_sample(const ShaderRec &sh, f3 wo, float u0, float u1, f3 &wi, f3 &f) {
+// refract (src/vec/vmath.hpp:495-503): Snell through the microfacet normal N.
+__device__ __forceinline__ bool refract_dir(f3 I, f3 N, float eta, f3 &out) {
+    float costh = dot(N, I);
+    float sinth2 = smax(0.f, 1.f - costh * costh);
+    float sinphi2 = sinth2 * eta * eta;
+    if (sinphi2 >= 1.f) return false;
+    float cosphi = sqrtf(1.f - sinphi2);
+    out = normalize((-I) * eta + N * (eta * costh - cosphi));
+    return true;
+}
+
+__device__ __forceinline__ float dist_f(const ShaderRec &sh, f3 w) {
+    return sh.dist == KR_DIST_COSINE ? cos_f(w) : beckmann_f(w, sh.alpha2);
+}
+__device__ __forceinline__ f3 dist_sample(const ShaderRec &sh, float u0, float u1) {
+    return sh.dist == KR_DIST_COSINE ? sample_cos(u0, u1) : beckmann_sample(u0, u1, sh.alpha2);
+}
+
+// Shader::execute(compute_f_by_wi_wo) of a leaf shader: Lambert / Emission
+// (lambert.hpp:31-33, emission.hpp:117-122), GlossyBSDF (glossy.hpp:77-79),
+// GlassBSDF (glass.hpp:27-34, 48-51; uc is its Fresnel pick), and f = 0 for
+// RefractionBSDF, Transparent, Blackhole.
+__device__ __forceinline__ f3 shader_eval(const ShaderRec &sh, f3 wo, f3 wi, float uc = 0.f) {
     f3 c{sh.cr[0], sh.cr[1], sh.cr[2]};
-    if (sh.type == KR_SHADER_GLOSSY) {
-        f3 h = sh.dist == KR_DIST_COSINE ? sample_cos(u0, u1) : beckmann_sample(u0, u1, sh.alpha2);
-        wi = h * (dot(h, wo) * 2.f) - wo;  // reflect (src/vec/vmath.hpp:470-473)
-        f = wo.z * wi.z > 0.f ? c : f3{0.f, 0.f, 0.f};
+    if (sh.type == KR_SHADER_GLOSSY) return c * dist_f(sh, normalize(wo + wi));
+    if (sh.type == KR_SHADER_GLASS) {
+        if (uc < fresnel(sh.ior, wo)) return c * dist_f(sh, normalize(wo + wi));
+        return c * dist_f(sh, normalize(wo + wi * sh.ior));
+    }
+    if (sh.type == KR_SHADER_LAMBERT || sh.type == KR_SHADER_EMISSION) return c * cos_f(wi);
+    return f3{0.f, 0.f, 0.f};
+}
+
+// Refraction branch of RefractionBSDF / GlassBSDF (refraction.hpp:24-34,
+// glass.hpp:37-46); wi stays as it was when the refraction fails.
+__device__ __forceinline__ void refraction_sample(const ShaderRec &sh, f3 wo, float u0, float u1, f3 &wi, f3 &f) {
+    f3 h = dist_sample(sh, u0, u1);
+    float e = wo.z > 0.f ? 1.f / sh.ior : sh.ior;
+    if (wo.z < 0.f) h = -h;
+    f3 r;
+    if (refract_dir(wo, h, e, r)) {
+        wi = r;
+        float den = e < 1.f ? e * e : 1.f;
+        f = f3{sh.cr[0] / den, sh.cr[1] / den, sh.cr[2] / den};
     } else {
+        f = f3{0.f, 0.f, 0.f};
+    }
+}
+
+// Shader::execute(sample_wi_f_by_wo) of a leaf shader (lambert.hpp:26-30,
+// glossy.hpp:73-76, glass.hpp:22-46, refraction.hpp:24-34,
+// transparent.hpp:20-23, blackhole.hpp:14-21). wi is in/out: a failed
+// refraction keeps the caller's value, as the reference's Varyings does.
+__device__ __forceinline__ void shader_sample(const ShaderRec &sh, f3 wo, float u0, float u1, float uc, f3 &wi,
+                                              f3 &f) {
+    f3 c{sh.cr[0], sh.cr[1], sh.cr[2]};
+    if (sh.type == KR_SHADER_LAMBERT || sh.type == KR_SHADER_EMISSION) {
         wi = sample_cos(u0, u1);
         f = c * cos_f(wi) / kInvPi;
+    } else if (sh.type == KR_SHADER_GLOSSY || (sh.type == KR_SHADER_GLASS && uc < fresnel(sh.ior, wo))) {
+        f3 h = dist_sample(sh, u0, u1);
+        wi = h * (dot(h, wo) * 2.f) - wo;  // reflect (src/vec/vmath.hpp:470-473)
+        f = wo.z * wi.z > 0.f ? c : f3{0.f, 0.f, 0.f};
+    } else if (sh.type == KR_SHADER_GLASS || sh.type == KR_SHADER_REFRACTION) {
+        refraction_sample(sh, wo, u0, u1, wi, f);
+    } else if (sh.type == KR_SHADER_TRANSPARENT) {
+        wi = -wo;
+        f = c;
+    } else {  // Blackhole
+        wi = f3{0.f, 0.f, 1.f};
+        f = f3{0.f, 0.f, 0.f};
     }
+}
+
+// Mixed::execute (src/ext/shader/mixed.hpp:22-27) down to a leaf: one draw
+// per Mixed level (b when u < fac), then GlassBSDF's Fresnel pick, all from
+// one Philox block of the choice stream (at most 3 levels + 1 pick).
+constexpr uint32_t kStreamChoice = 1u;
+__device__ __forceinline__ float pick4(const Philox4 &r, int k) {
+    return u01(k == 0 ? r.v[0] : (k == 1 ? r.v[1] : (k == 2 ? r.v[2] : r.v[3])));
+}
+__device__ __forceinline__ uint32_t resolve_leaf(const DevScene &S, uint32_t sid, f3 wo, uint32_t pixel,
+                                                 uint32_t sample, uint32_t block, uint32_t seed_lo, uint32_t seed_hi,
+                                                 float &uc) {
+    uc = 0.f;
+    uint32_t t = S.shaders[sid].type;
+    if (t != KR_SHADER_MIXED && t != KR_SHADER_GLASS) return sid;
+    Philox4 r = philox(pixel, sample, block, kStreamChoice, seed_lo, seed_hi);
+    int k = 0;
+    for (int level = 0; level < 3 && t == KR_SHADER_MIXED; ++level) {
+        const ShaderRec &m = S.shaders[sid];
+        float fc = m.fac_type == KR_SCALA_FRESNEL ? fresnel(m.fac_value, wo) : m.fac_value;
+        float u = pick4(r, k++);
+        sid = u < fc ? m.child1 : m.child0;
+        t = S.shaders[sid].type;
+    }
+    if (t == KR_SHADER_GLASS) uc = pick4(r, k);
+    return sid;
 }
 
 }  // namespace kr

@@ -56,10 +56,19 @@ struct ShaderRec {
     uint32_t type;   // KR_SHADER_*
     uint32_t dist;   // KR_DIST_*
     float alpha2;    // roughness^2
-    float roughness; // GlossyBSDF roughness, 0 for Lambert / Emission
-    float cr[4];     // Lambert color*R; Glossy color; Emission (1,1,1)
+    float roughness; // Beckmann roughness of GlossyBSDF / GlassBSDF / RefractionBSDF, else 0
+    float cr[4];     // Lambert color*R; Emission (1,1,1); the others color
     float emission[4];
+    float ior;       // GlassBSDF / RefractionBSDF
+    uint32_t fac_type;  // Mixed: KR_SCALA_*
+    float fac_value;    // Mixed: Constant value or Fresnel ior
+    uint32_t child0, child1;  // Mixed: shader a, shader b (b when u < fac)
+    uint32_t pad[3];
 };
+
+// Light kinds in LightRec::type: KR_PRIM_MESH / KR_PRIM_SPHERE area lights,
+// and point lights.
+constexpr uint32_t kLightPoint = 2;
 
 struct LightRec {
     uint32_t prim;
@@ -67,7 +76,7 @@ struct LightRec {
     uint32_t table;     // mesh: first entry in the light table (1024 entries)
     float pdf;          // mesh: 1 / total padded area
     float emission[4];  // E = color * strength
-    float center[4];    // sphere: centre.xyz, r2
+    float center[4];    // sphere: centre.xyz, r2; point light: position.xyz
 };
 
 constexpr uint32_t kLightTable = 1024;

@@ -509,7 +509,7 @@ __global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace(DevScene S, Jo
         uint32_t mat = 0;
         if (hitp && sort) {
             uint32_t prim = (h.code & kSphereBit) ? Sx.sphere_prim[h.code & ~kSphereBit] : Sx.tri_prim[h.code];
-            mat = Sx.shaders[Sx.prims[prim].shader].type;
+            mat = min(Sx.shaders[Sx.prims[prim].shader].type, 2u);  // Lambert, Glossy, everything else
         }
         uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
         if (hitp && mat == 0) b0[p0] = i;
@@ -573,7 +573,7 @@ __global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace_dyn(DevScene S
         if (hitp && sort) {
             uint32_t c = ts.h.code;
             uint32_t prim = (c & kSphereBit) ? Sx.sphere_prim[c & ~kSphereBit] : Sx.tri_prim[c];
-            mat = Sx.shaders[Sx.prims[prim].shader].type;
+            mat = min(Sx.shaders[Sx.prims[prim].shader].type, 2u);  // Lambert, Glossy, everything else
         }
         uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
         if (hitp && mat == 0) b0[p0] = i;
@@ -589,6 +589,61 @@ __global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace_dyn(DevScene S
         atomicAdd(&stats[S_NODES], (unsigned long long)nvis);
         atomicAdd(&stats[S_TRIS], (unsigned long long)ntri);
     }
+}
+
+// Scene::intersect's shading normal and primitive of a hit
+// (src/core/meta/scene.hpp:48-53): sphere normal in FP64
+// (src/core/primitive/sphere.hpp:92), smooth mesh normal
+// (src/core/primitive/mesh.hpp:85-91).
+__device__ __forceinline__ void hit_surface(const DevScene &S, f3 o, f3 d, float t, uint32_t code, float bu,
+                                            float bv, f3 &nrm, uint32_t &prim) {
+    if (code & kSphereBit) {
+        uint32_t s = code & ~kSphereBit;
+        const double *sp = S.spheres + 4 * s;
+        prim = S.sphere_prim[s];
+        double hx = double(o.x) + double(t) * double(d.x) - sp[0];
+        double hy = double(o.y) + double(t) * double(d.y) - sp[1];
+        double hz = double(o.z) + double(t) * double(d.z) - sp[2];
+        double inv = 1.0 / sqrt(hx * hx + hy * hy + hz * hz);
+        nrm = f3{float(hx * inv), float(hy * inv), float(hz * inv)};
+    } else {
+        prim = S.tri_prim[code];
+        const float *nn = S.tri_nrm + 9 * size_t(code);
+        float b0w = 1.f - bu - bv;
+        f3 n0{nn[0], nn[1], nn[2]}, n1{nn[3], nn[4], nn[5]}, n2{nn[6], nn[7], nn[8]};
+        nrm = normalize(n0 * b0w + n1 * bu + n2 * bv);
+    }
+}
+
+// Light::sample's point and pdf for shading point p: a point light
+// (pointLight.hpp:20-27; pdf unused), a mesh light's table entry
+// (mesh.hpp:194-200), a sphere's theta-uniform point (sphere.hpp:62-70);
+// area pdfs become solid-angle pdfs (areaLight.hpp:26).
+__device__ __forceinline__ void light_point(const DevScene &S, const LightRec &lr, f3 p, float u0, float u1, f3 &lp,
+                                            float &pdf) {
+    f3 ln{0.f, 0.f, 1.f};
+    pdf = 1.f;
+    if (lr.type == kLightPoint) {
+        lp = f3{lr.center[0], lr.center[1], lr.center[2]};
+        return;
+    }
+    if (lr.type == KR_PRIM_MESH) {
+        uint32_t e = min(uint32_t(floorf(u0 * 1024.f)), 1023u);
+        const float *te = S.light_table + 6 * (size_t(lr.table) + e);
+        lp = f3{te[0], te[1], te[2]};
+        ln = f3{te[3], te[4], te[5]};
+        pdf = lr.pdf;
+    } else {
+        float st0, ct0;
+        sincosf(u0 * kPi, &st0, &ct0);
+        f3 w = from_euler(st0, ct0, u1 * kPi * 2.f);
+        lp = f3{lr.center[0], lr.center[1], lr.center[2]} + w * sqrtf(lr.center[3]);
+        ln = w;
+        pdf = 1.f / (4.f * kPi * lr.center[3]);
+    }
+    f3 wl = p - lp;
+    float d2 = dot(wl, wl);
+    pdf *= d2 / fabsf(dot(ln, normalize(wl)));
 }
 
 struct ShadeArgs {
@@ -642,32 +697,16 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 seed_hi = J.seed_hi;
             }
             uint32_t sample = uint32_t(g / npix), pixel = uint32_t(g % npix);
-            // Scene::intersect frame (src/core/meta/scene.hpp:48-53)
             f3 nrm;
             uint32_t prim;
-            if (code & kSphereBit) {
-                uint32_t s = code & ~kSphereBit;
-                const double *sp = S.spheres + 4 * s;
-                prim = S.sphere_prim[s];
-                // hit.data = normalize(o + t d - c) (src/core/primitive/sphere.hpp:92)
-                double hx = double(o.x) + double(t) * double(d.x) - sp[0];
-                double hy = double(o.y) + double(t) * double(d.y) - sp[1];
-                double hz = double(o.z) + double(t) * double(d.z) - sp[2];
-                double inv = 1.0 / sqrt(hx * hx + hy * hy + hz * hz);
-                nrm = f3{float(hx * inv), float(hy * inv), float(hz * inv)};
-            } else {
-                prim = S.tri_prim[code];
-                const float *nn = S.tri_nrm + 9 * size_t(code);
-                float bu = a.hit.u[i], bv = a.hit.v[i], b0w = 1.f - bu - bv;
-                f3 n0{nn[0], nn[1], nn[2]}, n1{nn[3], nn[4], nn[5]}, n2{nn[6], nn[7], nn[8]};
-                nrm = normalize(n0 * b0w + n1 * bu + n2 * bv);
-            }
+            hit_surface(S, o, d, t, code, a.hit.u[i], a.hit.v[i], nrm, prim);
             f3 fu = normalize(cross(nrm, f3{0.f, 1.f, 0.f}));
             f3 fv = normalize(cross(nrm, fu));
             f3 p = o + d * t;
             f3 md = -d;
             f3 wo = normalize(f3{dot(md, fu), dot(md, fv), dot(md, nrm)});
-            const ShaderRec sh = S.shaders[S.prims[prim].shader];
+            const uint32_t sid = S.prims[prim].shader;
+            const ShaderRec sh = S.shaders[sid];  // the primitive's shader (a Mixed is resolved per call)
             f3 L{0.f, 0.f, 0.f};
             bool touch_L = false;
             // radiance cache: training paths record this vertex; other paths
@@ -678,7 +717,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             bool terminate = false;
             float *rec = nullptr;
             if (a.nr.enabled) {
-                bool dr = sh.type != KR_SHADER_GLOSSY || sh.roughness >= a.nr.rough_threshold;
+                bool dr = sh.type == KR_SHADER_LAMBERT || sh.type == KR_SHADER_EMISSION ||
+                          sh.type == KR_SHADER_MIXED ||
+                          ((sh.type == KR_SHADER_GLOSSY || sh.type == KR_SHADER_GLASS ||
+                            sh.type == KR_SHADER_REFRACTION) &&
+                           sh.dist == KR_DIST_BECKMANN && sh.roughness >= a.nr.rough_threshold);
                 if (!train && dr && ++ndr >= 2u) terminate = true;
                 if (train) {
                     uint32_t ts = a.nr.tslot_of[slot];
@@ -711,6 +754,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
             } else {
             Philox4 r0 = philox(pixel, sample, 2 * a.bounce, 0u, seed_lo, seed_hi);
             Philox4 r1 = philox(pixel, sample, 2 * a.bounce + 1, 0u, seed_lo, seed_hi);
+            // wi of the reference's Varyings: the light direction after NEE, else
+            // zero; a failed refraction leaves it in place
+            f3 wi_v{0.f, 0.f, 0.f};
+            uint32_t eval_leaf = sid;
             if (S.nlights) {
                 // light pick (radiance.cu:32-34) and AreaLight::sample (areaLight.hpp:19-31)
                 float xi = u01(r0.v[0]);
@@ -718,26 +765,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 float lpdf = 1.f / float(S.nlights);
                 const LightRec lr = S.lights[li_idx];
                 float u0 = u01(r0.v[1]), u1 = u01(r0.v[2]);
-                f3 lp, ln;
+                f3 lp;
                 float pdf;
-                if (lr.type == KR_PRIM_MESH) {
-                    uint32_t e = min(uint32_t(floorf(u0 * 1024.f)), 1023u);
-                    const float *te = S.light_table + 6 * (size_t(lr.table) + e);
-                    lp = f3{te[0], te[1], te[2]};
-                    ln = f3{te[3], te[4], te[5]};
-                    pdf = lr.pdf;
-                } else {
-                    // Sphere::sample (sphere.hpp:62-70)
-                    float st0, ct0;
-                    sincosf(u0 * kPi, &st0, &ct0);
-                    f3 w = from_euler(st0, ct0, u1 * kPi * 2.f);
-                    lp = f3{lr.center[0], lr.center[1], lr.center[2]} + w * sqrtf(lr.center[3]);
-                    ln = w;
-                    pdf = 1.f / (4.f * kPi * lr.center[3]);
-                }
-                f3 wl = p - lp;
-                float d2 = dot(wl, wl);
-                pdf *= d2 / fabsf(dot(ln, normalize(wl)));
+                light_point(S, lr, p, u0, u1, lp, pdf);
                 // LocalVaryings::emitSeg (varyings.hpp:27-37)
                 f3 sdv = lp - p;
                 f3 sdir = normalize(sdv);
@@ -746,9 +776,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 float slen = length(sdv - diff * 2.f);
                 f3 wi = normalize(lp - p);
                 wi = normalize(f3{dot(wi, fu), dot(wi, fv), dot(wi, nrm)});
-                f3 f = shader_eval(sh, wo, wi);
+                wi_v = wi;
+                float uce;
+                eval_leaf = resolve_leaf(S, sid, wo, pixel, sample, 2 * a.bounce, seed_lo, seed_hi, uce);
+                f3 f = shader_eval(S.shaders[eval_leaf], wo, wi, uce);
                 f3 E{lr.emission[0], lr.emission[1], lr.emission[2]};
-                f3 li = E / pdf;
+                f3 li = lr.type == kLightPoint ? E : E / pdf;
                 f3 c = beta * f * li * fabsf(wi.z) / lpdf;
                 if (!finite3(c)) {
                     // The sample is rejected whether or not the segment is
@@ -768,8 +801,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                     contrib = c;
                 }
             }
-            if (sh.type == KR_SHADER_EMISSION && S.nlights) {
-                f3 e = beta * f3{sh.emission[0], sh.emission[1], sh.emission[2]};
+            if (S.nlights && S.shaders[eval_leaf].type == KR_SHADER_EMISSION) {
+                const ShaderRec &es = S.shaders[eval_leaf];
+                f3 e = beta * f3{es.emission[0], es.emission[1], es.emission[2]};
                 L = L + e;
                 touch_L = true;
                 if (rec) {
@@ -787,8 +821,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 a.L[3 * slot + 2] += L.z;
             }
             // BSDF sample (radiance.cu:56-60)
-            f3 wi, f;
-            shader_sample(sh, wo, u01(r0.v[3]), u01(r1.v[0]), wi, f);
+            float ucs;
+            uint32_t sample_leaf = resolve_leaf(S, sid, wo, pixel, sample, 2 * a.bounce + 1, seed_lo, seed_hi, ucs);
+            f3 wi = wi_v, f;
+            shader_sample(S.shaders[sample_leaf], wo, u01(r0.v[3]), u01(r1.v[0]), ucs, wi, f);
             beta = beta * (f * fabsf(wi.z));
             f3 gd = normalize(fu * wi.x + fv * wi.y + nrm * wi.z);
             f3 go = p + gd * kSpawnEps;
@@ -802,6 +838,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(ShadeArgs a) {
                 else
                     beta = beta / (1.f - qq);
             }
+            // a ray with a NaN direction or origin (NaN frame, failed refraction
+            // with no light sampled) hits nothing, in the reference's traversal
+            // as in this one: end the path here instead of tracing it
+            if (alive && !(finite3(gd) && finite3(go))) alive = false;
             if (alive) {
                 push_ray = true;
                 no = go;
@@ -888,6 +928,97 @@ __global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_shadow(DevScene S, J
                     }
                 }
             }
+        }
+    }
+}
+
+// NormalKernel / VisibilityKernel / HalfVectorKernel at the first hit
+// (src/core/kernel/normal.hpp:17-28, visibility.hpp:17-35,
+// halfVector.hpp:17-37). Visibility queues a shadow segment to light 0 with
+// contribution 1 (k_shadow adds it when unoccluded); the others write L.
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_shade_debug(ShadeArgs a, uint32_t kernel) {
+    uint32_t n0 = a.counters[CTR(C_BUCKET + 0)], n1 = a.counters[CTR(C_BUCKET + 1)],
+             n2 = a.counters[CTR(C_BUCKET + 2)];
+    uint32_t total = n0 + n1 + n2;
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < total; base += stride) {
+        uint32_t k = base + threadIdx.x;
+        bool active = k < total, push_shadow = false;
+        uint32_t slot = 0;
+        f3 so{0, 0, 0}, sd{0, 0, 0};
+        float stmax = 0.f;
+        if (active) {
+            uint32_t i = k < n0 ? a.b0[k] : (k < n0 + n1 ? a.b1[k - n0] : a.b2[k - n0 - n1]);
+            slot = a.q.slot[i];
+            f3 o{a.q.ox[i], a.q.oy[i], a.q.oz[i]}, d{a.q.dx[i], a.q.dy[i], a.q.dz[i]};
+            float t = a.hit.t[i];
+            uint32_t code = a.hit.code[i];
+            uint64_t g = a.g0 + slot;
+            DevScene S = a.S;
+            uint32_t npix = a.npix, seed_lo = a.seed_lo, seed_hi = a.seed_hi;
+            if (BATCH) {
+                const JobRec &J = a.T.rec[find_job(a.T, g)];
+                S = J.S;
+                g -= J.begin;
+                npix = J.npix;
+                seed_lo = J.seed_lo;
+                seed_hi = J.seed_hi;
+            }
+            uint32_t sample = uint32_t(g / npix), pixel = uint32_t(g % npix);
+            f3 nrm;
+            uint32_t prim;
+            hit_surface(S, o, d, t, code, a.hit.u[i], a.hit.v[i], nrm, prim);
+            f3 L{0.f, 0.f, 0.f};
+            if (kernel == KR_KERNEL_NORMAL) {
+                L = nrm * .5f + f3{.5f, .5f, .5f};
+            } else if (S.nlights) {
+                f3 fu = normalize(cross(nrm, f3{0.f, 1.f, 0.f}));
+                f3 fv = normalize(cross(nrm, fu));
+                f3 p = o + d * t;
+                f3 md = -d;
+                f3 wo = normalize(f3{dot(md, fu), dot(md, fv), dot(md, nrm)});
+                Philox4 r0 = philox(pixel, sample, 0u, 0u, seed_lo, seed_hi);
+                const LightRec lr = S.lights[0];
+                f3 lp;
+                float pdf;
+                light_point(S, lr, p, u01(r0.v[1]), u01(r0.v[2]), lp, pdf);
+                f3 wi = normalize(lp - p);
+                wi = normalize(f3{dot(wi, fu), dot(wi, fv), dot(wi, nrm)});
+                if (kernel == KR_KERNEL_VISIBILITY) {
+                    f3 E{lr.emission[0], lr.emission[1], lr.emission[2]};
+                    f3 li = lr.type == kLightPoint ? E : E / pdf;
+                    if (!(li.x == 0.f && li.y == 0.f && li.z == 0.f)) {
+                        // LocalVaryings::emitSeg (varyings.hpp:27-37)
+                        f3 sdv = lp - p;
+                        sd = normalize(sdv);
+                        f3 diff = normalize(sd) * kSpawnEps;
+                        so = p + diff;
+                        stmax = length(sdv - diff * 2.f);
+                        push_shadow = true;
+                    }
+                } else {
+                    float v = acosf(normalize(wi + wo).z) * 2.f / kPi;
+                    L = f3{v, v, v};
+                }
+            }
+            a.L[3 * slot] = L.x;
+            a.L[3 * slot + 1] = L.y;
+            a.L[3 * slot + 2] = L.z;
+        }
+        uint32_t ps = wave_append(&a.counters[CTR(C_SHADOW)], push_shadow);
+        if (push_shadow) {
+            a.sq.slot[ps] = slot;
+            a.sq.ox[ps] = so.x;
+            a.sq.oy[ps] = so.y;
+            a.sq.oz[ps] = so.z;
+            a.sq.dx[ps] = sd.x;
+            a.sq.dy[ps] = sd.y;
+            a.sq.dz[ps] = sd.z;
+            a.sq.tmax[ps] = stmax;
+            a.sq.cx[ps] = 1.f;
+            a.sq.cy[ps] = 1.f;
+            a.sq.cz[ps] = 1.f;
         }
     }
 }
@@ -1393,22 +1524,48 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
         r.type = s.type;
         r.dist = s.distribution;
         r.alpha2 = s.roughness * s.roughness;
-        r.roughness = s.type == KR_SHADER_GLOSSY ? s.roughness : 0.f;
+        bool micro = s.type == KR_SHADER_GLOSSY || s.type == KR_SHADER_GLASS || s.type == KR_SHADER_REFRACTION;
+        r.roughness = micro ? s.roughness : 0.f;
+        r.ior = s.ior;
         if (s.type == KR_SHADER_LAMBERT) {
             for (int c = 0; c < 3; ++c) r.cr[c] = s.color[c] * s.R[c];
-        } else if (s.type == KR_SHADER_GLOSSY) {
-            for (int c = 0; c < 3; ++c) r.cr[c] = s.color[c];
         } else if (s.type == KR_SHADER_EMISSION) {
             for (int c = 0; c < 3; ++c) {
                 r.cr[c] = 1.f;
                 r.emission[c] = s.color[c] * s.strength;
             }
+        } else if (micro || s.type == KR_SHADER_TRANSPARENT || s.type == KR_SHADER_BLACKHOLE) {
+            for (int c = 0; c < 3; ++c) r.cr[c] = s.color[c];
+        } else if (s.type == KR_SHADER_MIXED) {
+            if (s.child[0] >= d->num_shaders || s.child[1] >= d->num_shaders)
+                return fail(KR_E_INVALID, "Mixed shader child index out of range");
+            if (s.fac_type != KR_SCALA_CONSTANT && s.fac_type != KR_SCALA_FRESNEL)
+                return fail(KR_E_INVALID, "unknown Mixed fac type");
+            r.fac_type = s.fac_type;
+            r.fac_value = s.fac_value;
+            r.child0 = s.child[0];
+            r.child1 = s.child[1];
         } else {
             return fail(KR_E_INVALID, "unknown shader type");
         }
-        if (s.type == KR_SHADER_GLOSSY && s.distribution != KR_DIST_BECKMANN && s.distribution != KR_DIST_COSINE)
+        if (micro && s.distribution != KR_DIST_BECKMANN && s.distribution != KR_DIST_COSINE)
             return fail(KR_E_INVALID, "unknown distribution type");
+        if ((s.type == KR_SHADER_GLASS || s.type == KR_SHADER_REFRACTION) && !(s.ior > 0.f))
+            return fail(KR_E_INVALID, "GlassBSDF / RefractionBSDF need ior > 0");
         shd[i] = r;
+    }
+    // Mixed trees: at most 3 Mixed levels below any shader (one Philox block
+    // of choices per execute call), which also rules out cycles
+    for (uint32_t i = 0; i < d->num_shaders; ++i) {
+        std::vector<std::pair<uint32_t, int>> todo{{i, 0}};
+        while (!todo.empty()) {
+            auto [k, depth] = todo.back();
+            todo.pop_back();
+            if (d->shaders[k].type != KR_SHADER_MIXED) continue;
+            if (depth >= 3) return fail(KR_E_UNSUPPORTED, "Mixed shaders nest deeper than 3 levels");
+            todo.push_back({d->shaders[k].child[0], depth + 1});
+            todo.push_back({d->shaders[k].child[1], depth + 1});
+        }
     }
     auto *s = new kr_scene();
     s->dev = dev;
@@ -1416,7 +1573,23 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     s->prims.assign(d->prims, d->prims + d->num_prims);
     s->tri_prim = tri_prim;
     s->sphere_prim = sph_prim;
-    // lights: every Emission primitive, in primitive order (scene.cu:200-216)
+    // point lights first, in file order (scene.cu:158-168)
+    if (d->num_point_lights && !d->point_lights) {
+        delete s;
+        return fail(KR_E_INVALID, "point light array missing");
+    }
+    for (uint32_t i = 0; i < d->num_point_lights; ++i) {
+        LightRec lr{};
+        lr.prim = KR_MISS;
+        lr.type = kLightPoint;
+        for (int c = 0; c < 3; ++c) {
+            lr.center[c] = d->point_lights[i].position[c];
+            lr.emission[c] = d->point_lights[i].emission[c];
+        }
+        s->lights.push_back(lr);
+    }
+    // then every Emission primitive as an area light, in primitive order
+    // (scene.cu:200-216)
     for (uint32_t i = 0; i < d->num_prims && !(d->flags & KR_SCENE_NO_AREA_LIGHTS); ++i) {
         const kr_prim_desc &p = d->prims[i];
         const kr_shader_desc &sh = d->shaders[p.shader];
@@ -1764,6 +1937,7 @@ int check_view(const kr_camera *cam, const kr_params *prm) {
     if (prm->width == 0 || prm->height == 0 || prm->spp == 0) return fail(KR_E_INVALID, "empty render");
     if (uint64_t(prm->width) * prm->height > (1ull << 31)) return fail(KR_E_INVALID, "resolution too large");
     if (cam->lens != KR_LENS_PINHOLE && cam->lens != KR_LENS_ORTHOGRAPHIC) return fail(KR_E_INVALID, "unknown lens");
+    if (prm->kernel > KR_KERNEL_HALFVECTOR) return fail(KR_E_INVALID, "unknown kernel");
     return KR_OK;
 }
 
@@ -1935,7 +2109,9 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
     bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
-    for (uint32_t b = 0; b <= r->prm.max_bounce; ++b) {
+    const uint32_t kern = r->prm.kernel;
+    const uint32_t rounds = kern == KR_KERNEL_RADIANCE ? r->prm.max_bounce + 1 : 1;
+    for (uint32_t b = 0; b < rounds; ++b) {
         int qi = cur == 0 ? C_RAY0 : C_RAY1;
         int qni = cur == 0 ? C_RAY1 : C_RAY0;
         {
@@ -1980,7 +2156,10 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
         a.T = T;
         {
             Timed tm(r, K_SHADE);
-            hipLaunchKernelGGL((k_shade<BATCH>), dim3(grid), dim3(kBlock), 0, st, a);
+            if (kern == KR_KERNEL_RADIANCE)
+                hipLaunchKernelGGL((k_shade<BATCH>), dim3(grid), dim3(kBlock), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_shade_debug<BATCH>), dim3(grid), dim3(kBlock), 0, st, a, kern);
         }
         {
             Timed tm(r, K_SHADOW);
@@ -1993,7 +2172,7 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
         }
         cur = 1 - cur;
     }
-    if (!BATCH && r->nr.enabled) nrc_after_chunk(r);
+    if (!BATCH && r->nr.enabled && kern == KR_KERNEL_RADIANCE) nrc_after_chunk(r);
     {
         Timed tm(r, K_OTHER);
         if (!BATCH) {
@@ -2436,7 +2615,8 @@ __global__ __launch_bounds__(256) void k_nrc_infer(nrc::Weights W, NrcRender nr,
 // the rest of the path carried back to it, (sum_{k>=j} c_k - beta_j E_j) / beta_j,
 // where c_k is everything vertex k added to the path's L.
 __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_t *counters, float *pool_raw,
-                                                     float *pool_tgt, uint32_t *pool_total, uint32_t cap) {
+                                                     float *pool_tgt, uint32_t *pool_total, uint32_t cap,
+                                                     float clamp) {
     uint32_t ntrain = min(counters[CTR(C_TRAIN)], nr.train_cap);
     uint32_t stride = gridDim.x * blockDim.x;
     // whole waves iterate together so that the ring slots can be reserved
@@ -2471,6 +2651,11 @@ __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_
             if (!(bx > 0.f && by > 0.f && bz > 0.f)) continue;
             float tx = (sx - rec[22]) / bx, ty = (sy - rec[23]) / by, tz = (sz - rec[24]) / bz;
             if (!(isfinite(tx) && isfinite(ty) && isfinite(tz))) continue;
+            if (clamp > 0.f) {
+                tx = fminf(tx, clamp);
+                ty = fminf(ty, clamp);
+                tz = fminf(tz, clamp);
+            }
             uint32_t r = slot++ % cap;
             float4 *dst = reinterpret_cast<float4 *>(pool_raw + size_t(r) * nrc::kRaw);
             dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
@@ -2492,7 +2677,7 @@ void nrc_after_chunk(kr_render *r) {
         hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_view(n), r->nr, r->counters, r->L);
         int egrid = int(std::min<size_t>((size_t(r->nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
         hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, r->nr, r->counters, n->pool_raw, n->pool_tgt,
-                           n->pool_total, n->pool_cap);
+                           n->pool_total, n->pool_cap, n->cfg.target_clamp);
     }
 }
 
@@ -2747,6 +2932,8 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
         if (J.scene >= num_scenes) return fail(KR_E_INVALID, "job scene index out of range");
         int rc = check_view(&J.camera, &J.params);
         if (rc) return rc;
+        if (J.params.kernel != KR_KERNEL_RADIANCE)
+            return fail(KR_E_UNSUPPORTED, "batched jobs render the Radiance kernel only");
         total += uint64_t(J.params.width) * J.params.height * J.params.spp;
         pix += uint64_t(J.params.width) * J.params.height;
         mb = std::max(mb, J.params.max_bounce);

@@ -21,6 +21,15 @@ void kr_check(int rc, const char *what) {
     throw std::logic_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + buf);
 }
 
+// Factory<Kernel> names (src/core/kernel/kernel.impl.cu:10-15)
+uint32_t kernel_type(const std::string &name) {
+    if (name == "Radiance") return KR_KERNEL_RADIANCE;
+    if (name == "Normal") return KR_KERNEL_NORMAL;
+    if (name == "Visibility") return KR_KERNEL_VISIBILITY;
+    if (name == "HalfVector") return KR_KERNEL_HALFVECTOR;
+    throw std::logic_error("no such type: " + name);
+}
+
 // RAII over the kr_* handles
 struct DeviceScene {
     kr_device *dev = nullptr;
@@ -44,11 +53,7 @@ public:
     void execute(Image &image, const kr_camera &camera, const LoadedScene &scene, uint32_t spp,
                  const RenderOptions &opt) override {
         const std::string kernel = opt.kernel.empty() ? scene.kernel : opt.kernel;
-        if (kernel != "Radiance") {
-            if (kernel == "Normal" || kernel == "Visibility" || kernel == "HalfVector")
-                throw std::logic_error("kernel " + kernel + " is not provided by this build (Radiance only)");
-            throw std::logic_error("no such type: " + kernel);
-        }
+        uint32_t ktype = kernel_type(kernel);
         DeviceScene ds;
         kr_check(kr_device_open(opt.device, &ds.dev), "kr_device_open");
         kr_scene_desc d = scene.desc(opt.bvh_width);
@@ -64,6 +69,7 @@ public:
         p.spp = spp;
         p.max_bounce = scene.max_bounce;
         p.seed = opt.seed;
+        p.kernel = ktype;
         kr_check(kr_render_create(ds.scene, &camera, &p, &ds.render), "kr_render_create");
         KINFO("tracer", "Tracing start");
         auto t0 = std::chrono::steady_clock::now();
@@ -92,7 +98,8 @@ public:
     void execute(Image &image, const kr_camera &camera, const LoadedScene &scene, uint32_t spp,
                  const RenderOptions &opt) override {
         const std::string kernel = opt.kernel.empty() ? scene.kernel : opt.kernel;
-        if (kernel != "Radiance") throw std::logic_error("no such type: " + kernel);
+        if (kernel_type(kernel) != KR_KERNEL_RADIANCE)
+            throw std::logic_error("tracer HIPBatched renders the Radiance kernel only");
         struct Handles {
             kr_device *dev = nullptr;
             kr_batch *batch = nullptr;
@@ -153,7 +160,7 @@ std::vector<std::string> tracer_names() {
     return v;
 }
 
-std::vector<std::string> kernel_names() { return {"Radiance"}; }
+std::vector<std::string> kernel_names() { return {"Radiance", "Normal", "Visibility", "HalfVector"}; }
 
 void Image::dump_png(const std::string &path) const {
     std::vector<uint8_t> bytes(3 * size_t(w) * h);

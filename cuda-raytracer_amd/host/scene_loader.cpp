@@ -187,10 +187,45 @@ std::vector<ObjMesh> load_obj(const std::string &path, std::vector<std::string> 
     return out;
 }
 
-kr_shader_desc shader_from_config(const Json &cfg) {
+namespace {
+// "distribution": Config of a SphericalDistribution (distribution.impl.cu:8-11)
+void parse_distribution(const Json &props, kr_shader_desc &d) {
+    const Json &dist = props.at("distribution");  // required (glossy.hpp:62-63)
+    std::string dn;
+    const Json *dp = nullptr;
+    if (dist.is_string()) {
+        dn = dist.str();
+    } else {
+        if (!dist.is_array() || dist.size() < 1) throw std::logic_error("distribution must be [type, {props}]");
+        dn = dist[0].str();
+        if (dist.size() > 1) dp = &dist[1];
+    }
+    if (dn == "Beckmann") {
+        d.distribution = KR_DIST_BECKMANN;
+        d.roughness = dp ? getf(*dp, "roughness", .5f) : .5f;
+    } else if (dn == "Cosine") {
+        d.distribution = KR_DIST_COSINE;
+    } else {
+        throw std::logic_error("no such type: " + dn);
+    }
+}
+
+float required_ior(const Json &props) { return float(props.at("ior").number()); }  // get<float>(props, "ior")
+
+constexpr int kMaxMixedDepth = 3;
+
+// Fills table[slot] from cfg; a Mixed's two inline shaders are appended to
+// the table (Factory<Shader>::create of "shaders"[0], [1], mixed.hpp:12-18).
+void fill_shader(const Json &cfg, std::vector<kr_shader_desc> &table, size_t slot, int depth) {
+    if (cfg.is_string()) {  // a bare name, e.g. "Blackhole"
+        Json arr = Json::array();
+        arr.push(cfg);
+        fill_shader(arr, table, slot, depth);
+        return;
+    }
     if (!cfg.is_array() || cfg.size() < 1 || !cfg[0].is_string())
         throw std::logic_error("shader config must be [type, {props}]");
-    const std::string &type = cfg[0].str();
+    const std::string type = cfg[0].str();
     Json empty = Json::object();
     const Json &props = cfg.size() > 1 ? cfg[1] : empty;
     kr_shader_desc d{};
@@ -203,34 +238,61 @@ kr_shader_desc shader_from_config(const Json &cfg) {
         d.type = KR_SHADER_LAMBERT;
     } else if (type == "GlossyBSDF") {
         d.type = KR_SHADER_GLOSSY;
-        const Json &dist = props.at("distribution");  // required (glossy.hpp:62-63)
-        std::string dn;
-        const Json *dp = nullptr;
-        if (dist.is_string()) {
-            dn = dist.str();
-        } else {
-            if (!dist.is_array() || dist.size() < 1) throw std::logic_error("distribution must be [type, {props}]");
-            dn = dist[0].str();
-            if (dist.size() > 1) dp = &dist[1];
-        }
-        if (dn == "Beckmann") {
-            d.distribution = KR_DIST_BECKMANN;
-            d.roughness = dp ? getf(*dp, "roughness", .5f) : .5f;
-        } else if (dn == "Cosine") {
-            d.distribution = KR_DIST_COSINE;
-        } else {
-            throw std::logic_error("no such type: " + dn);
-        }
+        parse_distribution(props, d);
     } else if (type == "Emission") {
         d.type = KR_SHADER_EMISSION;
-    } else if (type == "Blackhole" || type == "RefractionBSDF" || type == "Mixed" || type == "Transparent" ||
-               type == "GlassBSDF") {
-        throw std::logic_error("shader type " + type + " is not provided by this build (Radiance path: Lambert, "
-                               "GlossyBSDF, Emission)");
+    } else if (type == "GlassBSDF") {
+        d.type = KR_SHADER_GLASS;
+        d.ior = required_ior(props);
+        parse_distribution(props, d);
+    } else if (type == "RefractionBSDF") {
+        d.type = KR_SHADER_REFRACTION;
+        d.ior = required_ior(props);
+        parse_distribution(props, d);
+    } else if (type == "Transparent") {
+        d.type = KR_SHADER_TRANSPARENT;
+    } else if (type == "Blackhole") {
+        d.type = KR_SHADER_BLACKHOLE;
+    } else if (type == "Mixed") {
+        d.type = KR_SHADER_MIXED;
+        if (depth >= kMaxMixedDepth)
+            throw std::logic_error("Mixed shaders nest deeper than " + std::to_string(kMaxMixedDepth) +
+                                   " levels; this build does not provide that");
+        const Json &fac = props.at("fac");
+        if (!fac.is_array() || fac.size() < 1) throw std::logic_error("fac must be [type, {props}]");
+        const std::string fn = fac[0].str();
+        const Json &fp = fac.size() > 1 ? fac[1] : empty;
+        if (fn == "Constant") {
+            d.fac_type = KR_SCALA_CONSTANT;
+            d.fac_value = float(fp.at("value").number());
+        } else if (fn == "Fresnel") {
+            d.fac_type = KR_SCALA_FRESNEL;
+            d.fac_value = required_ior(fp);
+        } else {
+            throw std::logic_error("no such type: " + fn);
+        }
+        const Json &sh = props.at("shaders");
+        if (!sh.is_array() || sh.size() < 2) throw std::logic_error("Mixed needs \"shaders\": [a, b]");
+        table[slot] = d;
+        for (int k = 0; k < 2; ++k) {
+            size_t c = table.size();
+            table.push_back(kr_shader_desc{});
+            table[slot].child[k] = uint32_t(c);
+            fill_shader(sh[k], table, c, depth + 1);
+        }
+        return;
     } else {
         throw std::logic_error("no such type: " + type);
     }
-    return d;
+    table[slot] = d;
+}
+}  // namespace
+
+kr_shader_desc shader_from_config(const Json &cfg) {
+    std::vector<kr_shader_desc> table(1);
+    fill_shader(cfg, table, 0, 0);
+    if (table.size() > 1) throw std::logic_error("Mixed shaders need a shader table (load the scene instead)");
+    return table[0];
 }
 
 namespace {
@@ -314,9 +376,16 @@ LoadedScene load_scene_json(const Json &doc, const std::string &base_dir) {
                 p.shader = uint32_t(it - names.begin());
                 out.prims.push_back(p);
             } else if (name.find("light") != std::string::npos || name.find("Light") != std::string::npos) {
-                if (name == "pointLight")
-                    throw std::logic_error("pointLight is not provided by this build (area lights only)");
-                throw std::logic_error("unknown light type: " + name);
+                if (name != "pointLight") throw std::logic_error("unknown light type: " + name);
+                // PointLight ctor (src/core/light/pointLight.hpp:12-17)
+                kr_point_light pl{};
+                f3(props.at("position"), pl.position, "position");
+                const float one[3] = {1, 1, 1};
+                float col[3];
+                get3(props, "color", one, col);
+                float strength = getf(props, "strength", 2.f);
+                for (int k = 0; k < 3; ++k) pl.emission[k] = col[k] * strength;
+                out.point_lights.push_back(pl);
             } else {
                 throw std::logic_error("unknown asset type: " + name);
             }
@@ -334,7 +403,7 @@ LoadedScene load_scene_json(const Json &doc, const std::string &base_dir) {
         KLOG("Looking for shader:", names[i]);
         const Json *cfg = table ? table->find(names[i]) : nullptr;
         if (cfg) {
-            out.shaders[i] = shader_from_config(*cfg);
+            fill_shader(*cfg, out.shaders, i, 0);  // Mixed children go after the named shaders
             out.shader_defined[i] = true;
             KLOG(" =", i);
         } else {
@@ -370,6 +439,8 @@ kr_scene_desc LoadedScene::desc(uint32_t bvh_width, uint32_t leaf_size) const {
     d.bvh_width = bvh_width;
     d.leaf_size = leaf_size;
     d.flags = lights_enabled ? 0u : KR_SCENE_NO_AREA_LIGHTS;
+    d.point_lights = point_lights.empty() ? nullptr : point_lights.data();
+    d.num_point_lights = uint32_t(point_lights.size());
     return d;
 }
 

@@ -31,7 +31,9 @@ struct LoadedScene {
     std::vector<kr_prim_desc> prims;
     std::vector<std::string> shader_names;   // index = shader id
     std::vector<kr_shader_desc> shaders;
-    std::vector<bool> shader_defined;        // false: no config in "shaders"
+    std::vector<bool> shader_defined;        // false: no config in "shaders" (named shaders only;
+                                             // Mixed children follow them in `shaders`)
+    std::vector<kr_point_light> point_lights; // "pointLight" assets in file order
     bool lights_enabled = true;              // false when a shader config is missing
     std::vector<kr_camera> cameras;
     std::string tracer = "CPUMulticore";

@@ -24,6 +24,9 @@ KR_RENDER_NO_MATERIAL_SORT = 0x1
 KR_RENDER_PERSISTENT_TRAVERSAL = 0x2
 
 SHADER_LAMBERT, SHADER_GLOSSY, SHADER_EMISSION = 0, 1, 2
+SHADER_GLASS, SHADER_REFRACTION, SHADER_TRANSPARENT, SHADER_BLACKHOLE, SHADER_MIXED = 3, 4, 5, 6, 7
+SCALA_CONSTANT, SCALA_FRESNEL = 0, 1
+KERNEL_RADIANCE, KERNEL_NORMAL, KERNEL_VISIBILITY, KERNEL_HALFVECTOR = 0, 1, 2, 3
 DIST_BECKMANN, DIST_COSINE = 0, 1
 PRIM_MESH, PRIM_SPHERE = 0, 1
 LENS_PINHOLE, LENS_ORTHOGRAPHIC = 0, 1
@@ -33,7 +36,8 @@ u32, f32, u64 = C.c_uint32, C.c_float, C.c_uint64
 
 class ShaderDesc(C.Structure):
     _fields_ = [("type", u32), ("distribution", u32), ("color", f32 * 3), ("R", f32 * 3),
-                ("roughness", f32), ("strength", f32)]
+                ("roughness", f32), ("strength", f32), ("ior", f32), ("fac_type", u32), ("fac_value", f32),
+                ("child", u32 * 2)]
 
 
 class PrimDesc(C.Structure):
@@ -46,7 +50,12 @@ class SceneDesc(C.Structure):
                 ("indices", C.POINTER(u32)), ("num_triangles", u32),
                 ("prims", C.POINTER(PrimDesc)), ("num_prims", u32),
                 ("shaders", C.POINTER(ShaderDesc)), ("num_shaders", u32),
-                ("light_seed", u64), ("bvh_width", u32), ("leaf_size", u32), ("flags", u32)]
+                ("light_seed", u64), ("bvh_width", u32), ("leaf_size", u32), ("flags", u32),
+                ("point_lights", C.c_void_p), ("num_point_lights", u32)]
+
+
+class PointLight(C.Structure):
+    _fields_ = [("position", f32 * 3), ("emission", f32 * 3)]
 
 
 KR_SCENE_NO_AREA_LIGHTS = 0x1
@@ -65,7 +74,7 @@ class Camera(C.Structure):
 
 class Params(C.Structure):
     _fields_ = [("width", u32), ("height", u32), ("spp", u32), ("max_bounce", u32), ("seed", u64),
-                ("flags", u32), ("chunk_paths", u32)]
+                ("flags", u32), ("chunk_paths", u32), ("kernel", u32)]
 
 
 class Rays(C.Structure):
@@ -114,7 +123,7 @@ class BatchInfo(C.Structure):
 class NrcConfig(C.Structure):
     _fields_ = [("box_min", f32 * 3), ("box_max", f32 * 3), ("learning_rate", f32), ("seed", u32),
                 ("batch", u32), ("steps_per_frame", u32), ("train_log2", u32), ("rough_threshold", f32),
-                ("flags", u32)]
+                ("flags", u32), ("target_clamp", f32)]
 
 
 EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "kr_device_close",
@@ -337,10 +346,10 @@ class Nrc:
     """kr_nrc handle: the neural radiance cache on one device."""
 
     def __init__(self, dev: Device, box_min, box_max, seed=5, learning_rate=0.0, batch=0, steps_per_frame=0,
-                 train_log2=0, rough_threshold=0.0, flags=0):
+                 train_log2=0, rough_threshold=0.0, flags=0, target_clamp=0.0):
         self.lib = dev.lib
         cfg = NrcConfig((f32 * 3)(*box_min), (f32 * 3)(*box_max), learning_rate, seed, batch, steps_per_frame,
-                        train_log2, rough_threshold, flags)
+                        train_log2, rough_threshold, flags, target_clamp)
         h = C.c_void_p()
         check(self.lib.kr_nrc_create(dev.h, C.byref(cfg), C.byref(h)), "kr_nrc_create")
         self.h = h

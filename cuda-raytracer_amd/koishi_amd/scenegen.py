@@ -81,6 +81,7 @@ class SceneData:
     max_bounce: int
     name: str = "scene"
     light_seed: int = 0x4B4F49534849  # fixed key of the light sample tables
+    point_lights: list = field(default_factory=list)  # [(position xyz, color rgb, strength)]
     _keep: list = field(default_factory=list, repr=False)
 
     # ---- flat descriptor -------------------------------------------------
@@ -96,14 +97,19 @@ class SceneData:
         return names
 
     def shader_descs(self):
+        """Named shaders first (prims index them), then the inline children
+        of Mixed shaders, as the C++ loader lays them out."""
         names = self.shader_names()
-        arr = (kr.ShaderDesc * max(1, len(names)))()
+        table = [None] * len(names)
         for i, n in enumerate(names):
-            arr[i] = shader_desc_from_config(self.shaders[n])
-        return names, arr
+            fill_shader(self.shaders[n], table, i)
+        arr = (kr.ShaderDesc * max(1, len(table)))()
+        for i, d in enumerate(table):
+            arr[i] = d
+        return names, arr, len(table)
 
     def to_desc(self, bvh_width: int = 0, leaf_size: int = 0, flags: int = 0):
-        names, sh = self.shader_descs()
+        names, sh, nshaders = self.shader_descs()
         index = {n: i for i, n in enumerate(names)}
         prims = (kr.PrimDesc * (len(self.meshes) + len(self.spheres)))()
         k = 0
@@ -125,12 +131,17 @@ class SceneData:
         desc.prims = prims
         desc.num_prims = k
         desc.shaders = sh
-        desc.num_shaders = len(names)
+        desc.num_shaders = nshaders
         desc.light_seed = self.light_seed
         desc.bvh_width = bvh_width
         desc.leaf_size = leaf_size
         desc.flags = flags
-        holder = DescHolder(desc, [pos, nrm, idx, prims, sh])
+        pls = (kr.PointLight * max(1, len(self.point_lights)))()
+        for i, (pos_, col, strength) in enumerate(self.point_lights):
+            pls[i] = kr.PointLight((kr.f32 * 3)(*pos_), (kr.f32 * 3)(*[c * strength for c in col]))
+        desc.point_lights = C.cast(pls, C.c_void_p) if self.point_lights else None
+        desc.num_point_lights = len(self.point_lights)
+        holder = DescHolder(desc, [pos, nrm, idx, prims, sh, pls])
         return holder
 
     def camera_struct(self):
@@ -160,6 +171,9 @@ class SceneData:
             assets.append(["import", {"path": obj_path}])
         for cen, r, s in self.spheres:
             assets.append(["sphere", {"o": [float(x) for x in cen], "r": float(r), "shader": s}])
+        for pos_, col, strength in self.point_lights:
+            assets.append(["pointLight", {"position": [float(x) for x in pos_], "color": [float(x) for x in col],
+                                          "strength": float(strength)}])
         doc = {
             "scene": {
                 "camera": [self.camera],
@@ -182,14 +196,30 @@ class DescHolder:
         self.keep = keep
 
 
-def shader_desc_from_config(cfg) -> kr.ShaderDesc:
-    """Reference shader config [type, props] -> kr_shader_desc, with the
-    reference's defaults (lambert.hpp:12-19, glossy.hpp:60-66,
-    emission.hpp:107-114, beckmann.hpp:186-192)."""
+def _distribution(props, d):
+    dist = props["distribution"]
+    dname = dist if isinstance(dist, str) else dist[0]
+    dprops = {} if isinstance(dist, str) or len(dist) < 2 else dist[1]
+    if dname == "Beckmann":
+        d.distribution = kr.DIST_BECKMANN
+        d.roughness = float(dprops.get("roughness", 0.5))
+    elif dname == "Cosine":
+        d.distribution = kr.DIST_COSINE
+    else:
+        raise ValueError("no such type: " + dname)
+
+
+def fill_shader(cfg, table: list, slot: int, depth: int = 0):
+    """Reference shader config [type, props] -> kr_shader_desc at table[slot]
+    with the reference's defaults (lambert.hpp:12-19, glossy.hpp:60-66,
+    emission.hpp:107-114, beckmann.hpp:186-192, glass.hpp:12-19,
+    refraction.hpp:12-19, mixed.hpp:12-18); a Mixed's two inline shaders are
+    appended to the table."""
+    if isinstance(cfg, str):
+        cfg = [cfg]
     typ, props = cfg[0], (cfg[1] if len(cfg) > 1 else {})
     d = kr.ShaderDesc()
-    col = props.get("color", [1, 1, 1])
-    d.color = (kr.f32 * 3)(*col)
+    d.color = (kr.f32 * 3)(*props.get("color", [1, 1, 1]))
     d.R = (kr.f32 * 3)(*props.get("R", [1, 1, 1]))
     d.roughness = 0.5
     d.strength = float(props.get("strength", 2.0))
@@ -197,21 +227,47 @@ def shader_desc_from_config(cfg) -> kr.ShaderDesc:
         d.type = kr.SHADER_LAMBERT
     elif typ == "GlossyBSDF":
         d.type = kr.SHADER_GLOSSY
-        dist = props["distribution"]
-        dname = dist if isinstance(dist, str) else dist[0]
-        dprops = {} if isinstance(dist, str) or len(dist) < 2 else dist[1]
-        if dname == "Beckmann":
-            d.distribution = kr.DIST_BECKMANN
-            d.roughness = float(dprops.get("roughness", 0.5))
-        elif dname == "Cosine":
-            d.distribution = kr.DIST_COSINE
-        else:
-            raise ValueError("no such type: " + dname)
+        _distribution(props, d)
     elif typ == "Emission":
         d.type = kr.SHADER_EMISSION
+    elif typ in ("GlassBSDF", "RefractionBSDF"):
+        d.type = kr.SHADER_GLASS if typ == "GlassBSDF" else kr.SHADER_REFRACTION
+        d.ior = float(props["ior"])
+        _distribution(props, d)
+    elif typ == "Transparent":
+        d.type = kr.SHADER_TRANSPARENT
+    elif typ == "Blackhole":
+        d.type = kr.SHADER_BLACKHOLE
+    elif typ == "Mixed":
+        if depth >= 3:
+            raise ValueError("Mixed shaders nest deeper than 3 levels")
+        d.type = kr.SHADER_MIXED
+        fname, fprops = props["fac"][0], (props["fac"][1] if len(props["fac"]) > 1 else {})
+        if fname == "Constant":
+            d.fac_type, d.fac_value = kr.SCALA_CONSTANT, float(fprops["value"])
+        elif fname == "Fresnel":
+            d.fac_type, d.fac_value = kr.SCALA_FRESNEL, float(fprops["ior"])
+        else:
+            raise ValueError("no such type: " + fname)
+        table[slot] = d
+        for k in range(2):
+            c = len(table)
+            table.append(None)
+            table[slot].child[k] = c
+            fill_shader(props["shaders"][k], table, c, depth + 1)
+        return
     else:
         raise ValueError("no such type: " + typ)
-    return d
+    table[slot] = d
+
+
+def shader_desc_from_config(cfg) -> kr.ShaderDesc:
+    """A leaf shader config -> kr_shader_desc (Mixed needs a table: fill_shader)."""
+    table = [None]
+    fill_shader(cfg, table, 0)
+    if len(table) > 1:
+        raise ValueError("Mixed shaders need a shader table")
+    return table[0]
 
 
 def fmt(x) -> str:
@@ -384,6 +440,7 @@ class Builder:
         self.nt = 0
         self.spheres = []
         self.shaders = {}
+        self.point_lights = []
 
     def add_mesh(self, name, shader, pos, nrm, tris):
         self.pos.append(np.asarray(pos, np.float64))
@@ -395,6 +452,9 @@ class Builder:
 
     def add_sphere(self, center, r, shader):
         self.spheres.append((center, r, shader))
+
+    def add_point_light(self, position, color, strength):
+        self.point_lights.append((position, color, strength))
 
     def finish(self, camera_eye, camera_look, fovx, width, height, spp, max_bounce, name) -> SceneData:
         if self.pos:
@@ -414,8 +474,10 @@ class Builder:
         tgt /= np.linalg.norm(tgt)
         cam = {"lens": "pinhole", "fovx": float(fovx), "position": [float(x) for x in np.float32(eye)],
                "target": [float(x) for x in np.float32(tgt)], "upaxis": [0.0, 0.0, 1.0]}
+        pls = [([float(x) for x in rot_yup_to_zup(np.array(p_)).astype(np.float32)], list(c), float(k))
+               for p_, c, k in self.point_lights]
         return SceneData(pos, nrm, idx, self.meshes, spheres, self.shaders, cam, width, height, spp, max_bounce,
-                         name=name)
+                         name=name, point_lights=pls)
 
 
 def _rand_color(rng):
@@ -511,6 +573,46 @@ def room_scene(seed: int = 6, freq: int = 20, width: int = 1024, height: int = 7
     return b.finish(tuple(eye), tuple(look), 70.0, width, height, spp, max_bounce, "room")
 
 
+def materials_scene(width: int = 160, height: int = 96, spp: int = 16, max_bounce: int = 8) -> SceneData:
+    """Every shader type, light type and Mixed fac of the Radiance path on
+    one ground: GlassBSDF, RefractionBSDF (Beckmann and Cosine), Transparent,
+    Blackhole, Mixed with a Constant and a Fresnel fac (nested two levels,
+    with an Emission leaf), a Mixed displaced icosphere, one quad area light
+    and two point lights."""
+    b = Builder()
+    b.shaders["ground"] = ["Lambert", {"color": [0.6, 0.6, 0.55]}]
+    b.shaders["light"] = ["Emission", {"color": [1.0, 0.9, 0.8], "strength": 12.0}]
+    b.shaders["glass"] = ["GlassBSDF", {"ior": 1.5, "color": [0.95, 0.95, 1.0],
+                                        "distribution": ["Beckmann", {"roughness": 0.15}]}]
+    b.shaders["water"] = ["RefractionBSDF", {"ior": 1.33, "color": [0.8, 0.9, 1.0],
+                                             "distribution": ["Beckmann", {"roughness": 0.3}]}]
+    b.shaders["frosted"] = ["RefractionBSDF", {"ior": 1.2, "distribution": ["Cosine", {}]}]
+    b.shaders["clear"] = ["Transparent", {"color": [0.7, 0.9, 0.7]}]
+    b.shaders["void"] = ["Blackhole", {}]
+    b.shaders["mix"] = ["Mixed", {"fac": ["Constant", {"value": 0.3}],
+                                  "shaders": [["Lambert", {"color": [0.7, 0.3, 0.2]}],
+                                              _glossy([0.9, 0.9, 0.9], 0.2)]}]
+    b.shaders["coated"] = ["Mixed", {"fac": ["Fresnel", {"ior": 1.6}],
+                                     "shaders": [["Lambert", {"color": [0.2, 0.4, 0.7]}],
+                                                 ["Mixed", {"fac": ["Constant", {"value": 0.5}],
+                                                            "shaders": [["GlassBSDF", {"ior": 1.4, "distribution":
+                                                                                       ["Beckmann", {"roughness": 0.05}]}],
+                                                                        ["Emission", {"strength": 3.0}]]}]]}]
+    v, n, t = quad((0.0, 0.0, 0.0), 14.0, 14.0, 1.0)
+    b.add_mesh("ground", "ground", v, n, t)
+    v, n, t = quad((0.0, 4.0, 0.0), 1.5, 1.5, -1.0)
+    b.add_mesh("light", "light", v, n, t)
+    pos, nrm, tris = displaced_icosphere(12, 9)
+    b.add_mesh("blob", "coated", pos * 0.7 + np.array([0.0, 0.8, 1.6]), nrm, tris)
+    for c, r, sh in (((-2.4, 0.6, 0.0), 0.6, "glass"), ((-1.0, 0.5, -0.6), 0.5, "water"),
+                     ((0.3, 0.45, -1.0), 0.45, "frosted"), ((1.5, 0.5, -0.4), 0.5, "clear"),
+                     ((2.6, 0.4, 0.6), 0.4, "void"), ((1.2, 0.55, 1.2), 0.55, "mix")):
+        b.add_sphere(c, r, sh)
+    b.add_point_light((2.0, 3.0, -2.0), [1.0, 1.0, 1.0], 1.5)
+    b.add_point_light((-3.0, 2.0, 1.0), [0.4, 0.6, 1.0], 0.8)
+    return b.finish((0.0, 2.2, -6.5), (0.0, 0.6, 0.0), 55.0, width, height, spp, max_bounce, "materials")
+
+
 def quat_matrix(rng: SplitMix64):
     """Uniform random rotation (Shoemake, Graphics Gems III, 1992)."""
     u1, u2, u3 = rng.uniform(), rng.uniform(), rng.uniform()
@@ -586,4 +688,5 @@ WORKLOADS = {
     "mesh": mesh_scene,
     "large": large_scene,
     "room": room_scene,
+    "materials": materials_scene,
 }

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KR_API_VERSION 1
+#define KR_API_VERSION 2
 
 enum kr_status {
     KR_OK = 0,
@@ -49,13 +49,25 @@ typedef struct kr_render kr_render;
 
 /* ---- scene description (flat, host memory) ---------------------------- */
 
-/* Shader type tags. Replace Factory<Shader> names
+/* Shader type tags. Replace the Factory<Shader> names
  * (src/ext/shader/shader.impl.cu:15-24): "Lambert", "GlossyBSDF",
- * "Emission". */
+ * "Emission", "GlassBSDF", "RefractionBSDF", "Transparent", "Blackhole",
+ * "Mixed". */
 enum kr_shader_type {
     KR_SHADER_LAMBERT = 0,
     KR_SHADER_GLOSSY = 1,
-    KR_SHADER_EMISSION = 2
+    KR_SHADER_EMISSION = 2,
+    KR_SHADER_GLASS = 3,
+    KR_SHADER_REFRACTION = 4,
+    KR_SHADER_TRANSPARENT = 5,
+    KR_SHADER_BLACKHOLE = 6,
+    KR_SHADER_MIXED = 7
+};
+
+/* Scala<float> tags for Mixed's "fac" (src/ext/scala/scala.impl.cu:9-16). */
+enum kr_scala_type {
+    KR_SCALA_CONSTANT = 0, /* "Constant": value                              */
+    KR_SCALA_FRESNEL = 1   /* "Fresnel": dielectric reflectance of wo, ior   */
 };
 
 /* Distribution tags for GlossyBSDF's "distribution" entry. Replace
@@ -72,6 +84,13 @@ typedef struct kr_shader_desc {
     float R[3];            /* Lambert "R", default 1                            */
     float roughness;       /* Beckmann "roughness", default 0.5 (alpha2 = r^2) */
     float strength;        /* Emission "strength", default 2                    */
+    float ior;             /* GlassBSDF / RefractionBSDF "ior"                  */
+    uint32_t fac_type;     /* Mixed: kr_scala_type of "fac"                     */
+    float fac_value;       /* Mixed: Constant "value" or Fresnel "ior"          */
+    uint32_t child[2];     /* Mixed: "shaders" [a, b] as indices into the
+                              shader array; b is taken when u < fac
+                              (src/ext/shader/mixed.hpp:22-27). Nesting depth
+                              at most 3 Mixed levels.                         */
 } kr_shader_desc;
 
 enum kr_prim_type {
@@ -105,7 +124,18 @@ typedef struct kr_scene_desc {
     uint32_t bvh_width;       /* 2, 4 or 8; 0 = library default (4)            */
     uint32_t leaf_size;       /* max triangles per leaf; 0 = default (4)       */
     uint32_t flags;           /* KR_SCENE_* bits                               */
+    const struct kr_point_light *point_lights; /* "pointLight" assets, in file order */
+    uint32_t num_point_lights;
 } kr_scene_desc;
+
+/* A "pointLight" asset (src/core/light/pointLight.hpp:10-36): li is the
+ * emission color * strength with no distance falloff, occluded by the scene.
+ * Point lights come first in the light list, in file order, then the area
+ * lights (src/core/meta/scene.cu:158-168, 200-216). */
+typedef struct kr_point_light {
+    float position[3];
+    float emission[3]; /* color * strength */
+} kr_point_light;
 
 /* No area lights are derived: the reference skips light promotion when any
  * shader name has no configuration (src/core/meta/scene.cu:193-216). */
@@ -146,7 +176,19 @@ typedef struct kr_params {
     uint64_t seed;       /* counter-based RNG key                               */
     uint32_t flags;      /* KR_RENDER_* bits                                    */
     uint32_t chunk_paths;/* paths in flight per wavefront chunk; 0 = default   */
+    uint32_t kernel;     /* kr_kernel_type: "Radiance" (default) or one of the
+                            one-hit debug kernels                             */
 } kr_params;
+
+/* Factory<Kernel> names (src/core/kernel/kernel.impl.cu:10-15). */
+enum kr_kernel_type {
+    KR_KERNEL_RADIANCE = 0,   /* RadianceKernel (src/core/kernel/radiance.cu)       */
+    KR_KERNEL_NORMAL = 1,     /* n/2 + 1/2 at the first hit (normal.hpp:17-28)       */
+    KR_KERNEL_VISIBILITY = 2, /* 1 if light 0 is visible from the first hit
+                                 (visibility.hpp:17-35)                              */
+    KR_KERNEL_HALFVECTOR = 3  /* acos(cos theta of the half vector to light 0) 2/pi
+                                 (halfVector.hpp:17-37)                              */
+};
 
 #define KR_RENDER_NO_MATERIAL_SORT 0x1u /* shade with one mixed-material queue */
 #define KR_RENDER_PERSISTENT_TRAVERSAL 0x2u /* traversal kernels refill lanes whose ray
@@ -296,6 +338,9 @@ typedef struct kr_nrc_config {
     float rough_threshold;        /* GlossyBSDF roughness counted as rough;
                                      0 = 0.3                                   */
     uint32_t flags;               /* KR_NRC_* bits                             */
+    float target_clamp;           /* > 0: training targets are clamped to this
+                                     radiance per channel (e.g. the brightest
+                                     emitter's), 0: unclamped                   */
 } kr_nrc_config;
 
 #define KR_NRC_FROZEN 0x1u /* renders query the cache but never train it */

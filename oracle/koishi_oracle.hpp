@@ -113,9 +113,14 @@ inline float u01(uint32_t x) { return float(x >> 8) * (1.0f / 16777216.0f); }
 constexpr uint32_t kStreamPath = 0u;
 constexpr uint32_t kStreamLight = 0x4C495445u;  // light sample tables
 
-// Per-path random numbers: dimension d of bounce b is word (d & 3) of
-// Philox(ctr = {pixel, sample, 2*b + (d >> 2), kStreamPath}, key = seed).
-// Dimensions: 0 light pick, 1-2 light point, 3-5 BSDF sample3, 6 roulette.
+constexpr uint32_t kStreamChoice = 1u;          // shader choices (Mixed, Glass)
+
+// Per-path random numbers: dimension d < 8 of bounce b is word (d & 3) of
+// Philox(ctr = {pixel, sample, 2*b + (d >> 2), kStreamPath}, key = seed);
+// d = 8..15 is word (d & 3) of Philox({pixel, sample, 2*b + ((d - 8) >> 2),
+// kStreamChoice}). Dimensions: 0 light pick, 1-2 light point, 3-5 BSDF
+// sample3, 6 roulette, 8-11 the shader choices (Mixed levels, then Glass's
+// Fresnel pick) of compute_f_by_wi_wo, 12-15 those of sample_wi_f_by_wo.
 struct PathRng {
     uint32_t pixel, sample;
     uint32_t key[2];
@@ -126,13 +131,15 @@ struct PathRng {
         key[1] = uint32_t(seed >> 32);
     }
     float get(int bounce, int dim) {
-        int block = 2 * bounce + (dim >> 2);
+        bool choice = dim >= 8;
+        int d = choice ? dim - 8 : dim;
+        int block = (2 * bounce + (d >> 2)) * 2 + (choice ? 1 : 0);
         if (block != cached) {
-            uint32_t ctr[4] = {pixel, sample, uint32_t(block), kStreamPath};
+            uint32_t ctr[4] = {pixel, sample, uint32_t(2 * bounce + (d >> 2)), choice ? kStreamChoice : kStreamPath};
             philox4x32_10(ctr, key, buf);
             cached = block;
         }
-        return u01(buf[dim & 3]);
+        return u01(buf[d & 3]);
     }
 };
 
@@ -378,6 +385,8 @@ struct Prim {
 template <class R> struct Light {
     uint32_t prim;
     V3<R> emission;
+    bool point = false;  // PointLight (src/core/light/pointLight.hpp:10-36)
+    V3<R> pos{0, 0, 0};
 };
 
 struct Counters {
@@ -423,6 +432,16 @@ template <class R> struct Scene {
 
     bool build(const kr_scene_desc &d) {
         shaders.assign(d.shaders, d.shaders + d.num_shaders);
+        // "pointLight" assets join the light list while the assets are read,
+        // before area lights are promoted (src/core/meta/scene.cu:158-168)
+        for (uint32_t i = 0; i < d.num_point_lights; ++i) {
+            Light<R> L;
+            L.prim = ~0u;
+            L.point = true;
+            L.pos = mk<R>(d.point_lights[i].position);
+            L.emission = mk<R>(d.point_lights[i].emission);
+            lights.push_back(L);
+        }
         for (uint32_t i = 0; i < d.num_prims; ++i) {
             const kr_prim_desc &p = d.prims[i];
             if (p.shader >= d.num_shaders) return false;
@@ -445,7 +464,10 @@ template <class R> struct Scene {
             const kr_shader_desc &sh = shaders[prims[i].shader];
             if (sh.type != KR_SHADER_EMISSION) continue;
             V3<R> e = mk<R>(sh.color) * R(sh.strength);
-            lights.push_back({i, e});
+            Light<R> L;
+            L.prim = i;
+            L.emission = e;
+            lights.push_back(L);
             if (prims[i].type == KR_PRIM_MESH) gen_samples(meshes[prims[i].index], uint32_t(lights.size() - 1), d.light_seed);
         }
         return true;
@@ -647,27 +669,107 @@ template <class R> inline bool scene_intersect(const Scene<R> &s, const Ray<R> &
     return true;
 }
 
-// Shader::execute(compute_f_by_wi_wo): Lambert (lambert.hpp:31-33), Glossy
-// (glossy.hpp:77-79), Emission = emission + white Lambert (emission.hpp:117-122).
-template <class R> inline void shader_eval(const kr_shader_desc &sh, Vary<R> &vy) {
+// Scala<float>::compute for Mixed's fac: Constant (constant.hpp:15-20) or
+// Fresnel (fresnel.hpp:15-30), the dielectric reflectance of wo.
+template <class R> inline R fresnel(R ior, const Vary<R> &vy) {
+    R cosi = vy.wo.z;
+    R e = cosi > R(0) ? R(1) / ior : ior;
+    cosi = std::fabs(cosi);
+    R sini = std::sqrt(smax(R(0), R(1) - cosi * cosi));
+    R sinr = e * sini;
+    if (sinr >= R(1)) return R(1);
+    R cosr = std::sqrt(smax(R(0), R(1) - sinr * sinr));
+    R cosior = cosi / cosr;
+    R rparl = (cosior - e) / (cosior + e);
+    R rperp = (e * cosior - R(1)) / (e * cosior + R(1));
+    return R(.5f) * (rparl * rparl + rperp * rperp);
+}
+template <class R> inline R scala(const kr_shader_desc &sh, const Vary<R> &vy) {
+    return sh.fac_type == KR_SCALA_FRESNEL ? fresnel(R(sh.fac_value), vy) : R(sh.fac_value);
+}
+
+// Mixed::execute (src/ext/shader/mixed.hpp:22-27) down to a leaf shader:
+// one draw per Mixed level, b when u < fac. `dim` is the next choice
+// dimension of this execute call and advances with every draw.
+template <class R>
+inline uint32_t resolve_shader(const std::vector<kr_shader_desc> &shaders, uint32_t idx, const Vary<R> &vy,
+                               PathRng &rng, int bounce, int &dim) {
+    while (shaders[idx].type == KR_SHADER_MIXED) {
+        const kr_shader_desc &m = shaders[idx];
+        R f = scala(m, vy);
+        R u = R(rng.get(bounce, dim++));
+        idx = u < f ? m.child[1] : m.child[0];
+    }
+    return idx;
+}
+
+template <class R> inline R dist_f(const kr_shader_desc &sh, V3<R> w) {
+    R a2 = R(sh.roughness) * R(sh.roughness);
+    return sh.distribution == KR_DIST_COSINE ? cos_f(w) : beckmann_f(w, a2);
+}
+template <class R> inline V3<R> dist_sample(const kr_shader_desc &sh, R u0, R u1) {
+    R a2 = R(sh.roughness) * R(sh.roughness);
+    return sh.distribution == KR_DIST_COSINE ? sample_cos(u0, u1) : beckmann_sample(u0, u1, a2);
+}
+
+// refract (src/vec/vmath.hpp:495-503): Snell through the microfacet normal N.
+template <class R> inline bool refract(V3<R> I, V3<R> &out, V3<R> N, R eta) {
+    R costh = dot(N, I);
+    R sinth2 = smax(R(0), R(1) - costh * costh);
+    R sinphi2 = sinth2 * eta * eta;
+    if (sinphi2 >= R(1)) return false;
+    R cosphi = std::sqrt(R(1) - sinphi2);
+    out = normalize(eta * (-I) + (eta * costh - cosphi) * N);
+    return true;
+}
+
+// Refraction branch of RefractionBSDF / GlassBSDF sample_wi_f_by_wo
+// (refraction.hpp:24-34, glass.hpp:37-46); wi is left as it was when the
+// refraction fails (f = 0).
+template <class R> inline void refraction_sample(const kr_shader_desc &sh, Vary<R> &vy, R u0, R u1) {
+    V3<R> h = dist_sample(sh, u0, u1);
+    R ior = R(sh.ior);
+    R e = vy.wo.z > R(0) ? R(1) / ior : ior;
+    if (vy.wo.z < R(0)) h = -h;
+    if (refract(vy.wo, vy.wi, h, e))
+        vy.f = mk<R>(sh.color) / (e < R(1) ? e * e : R(1));
+    else
+        vy.f = V3<R>{0, 0, 0};
+}
+
+// Shader::execute(compute_f_by_wi_wo) of a leaf: Lambert (lambert.hpp:31-33),
+// Glossy (glossy.hpp:77-79), Emission = emission + white Lambert
+// (emission.hpp:117-122), GlassBSDF (glass.hpp:27-34, 48-51: reflection when
+// u < Fresnel(ior)), RefractionBSDF / Transparent / Blackhole: f = 0
+// (refraction.hpp:35-38, transparent.hpp:24-26, blackhole.hpp:18-21).
+// uc: the Glass branch uniform (its draw follows the Mixed draws).
+template <class R> inline void shader_eval(const kr_shader_desc &sh, Vary<R> &vy, R uc) {
     if (sh.type == KR_SHADER_EMISSION) {
         vy.emission = mk<R>(sh.color) * R(sh.strength);
         vy.f = V3<R>{1, 1, 1} * cos_f(vy.wi);
     } else if (sh.type == KR_SHADER_LAMBERT) {
         vy.f = mk<R>(sh.color) * mk<R>(sh.R) * cos_f(vy.wi);
+    } else if (sh.type == KR_SHADER_GLOSSY) {
+        vy.f = mk<R>(sh.color) * dist_f(sh, normalize(vy.wo + vy.wi));
+    } else if (sh.type == KR_SHADER_GLASS) {
+        if (uc < fresnel(R(sh.ior), vy))
+            vy.f = mk<R>(sh.color) * dist_f(sh, normalize(vy.wo + vy.wi));
+        else
+            vy.f = mk<R>(sh.color) * dist_f(sh, normalize(vy.wo + vy.wi * R(sh.ior)));
     } else {
-        V3<R> h = normalize(vy.wo + vy.wi);
-        R a2 = R(sh.roughness) * R(sh.roughness);
-        R d = sh.distribution == KR_DIST_COSINE ? cos_f(h) : beckmann_f(h, a2);
-        vy.f = mk<R>(sh.color) * d;
+        vy.f = V3<R>{0, 0, 0};
     }
 }
 
-// Shader::execute(sample_wi_f_by_wo) with u = sample3(). Lambert: wi from
-// CosDistribution with pdf 1/pi, f = color R f(wi) / pdf (lambert.hpp:26-30).
-// Glossy: wi = reflect(wo, h), f = color when wo, wi lie on one side
-// (glossy.hpp:73-76, H::isSame hemisphere.hpp:14-17).
-template <class R> inline void shader_sample(const kr_shader_desc &sh, Vary<R> &vy, R u0, R u1, R u2) {
+// Shader::execute(sample_wi_f_by_wo) of a leaf with u = sample3(). Lambert:
+// wi from CosDistribution with pdf 1/pi, f = color R f(wi) / pdf
+// (lambert.hpp:26-30). Glossy: wi = reflect(wo, h), f = color when wo, wi lie
+// on one side (glossy.hpp:73-76, H::isSame hemisphere.hpp:14-17). GlassBSDF:
+// that reflection when u < Fresnel(ior), else the refraction branch
+// (glass.hpp:22-46). Transparent: wi = -wo, f = color (transparent.hpp:20-23).
+// Blackhole: wi = (0, 0, 1), f = 0 (blackhole.hpp:14-21).
+template <class R>
+inline void shader_sample(const kr_shader_desc &sh, Vary<R> &vy, R u0, R u1, R u2, R uc) {
     (void)u2;
     if (sh.type == KR_SHADER_EMISSION || sh.type == KR_SHADER_LAMBERT) {
         V3<R> c = sh.type == KR_SHADER_EMISSION ? V3<R>{1, 1, 1} : mk<R>(sh.color) * mk<R>(sh.R);
@@ -675,11 +777,26 @@ template <class R> inline void shader_sample(const kr_shader_desc &sh, Vary<R> &
         vy.wi = sample_cos(u0, u1);
         R pdf = Const<R>::inv_pi();
         vy.f = c * cos_f(vy.wi) / pdf;
-    } else {
-        R a2 = R(sh.roughness) * R(sh.roughness);
-        V3<R> h = sh.distribution == KR_DIST_COSINE ? sample_cos(u0, u1) : beckmann_sample(u0, u1, a2);
+    } else if (sh.type == KR_SHADER_GLOSSY) {
+        V3<R> h = dist_sample(sh, u0, u1);
         vy.wi = reflect(vy.wo, h);
         vy.f = vy.wo.z * vy.wi.z > 0 ? mk<R>(sh.color) : V3<R>{0, 0, 0};
+    } else if (sh.type == KR_SHADER_GLASS) {
+        if (uc < fresnel(R(sh.ior), vy)) {
+            V3<R> h = dist_sample(sh, u0, u1);
+            vy.wi = reflect(vy.wo, h);
+            vy.f = vy.wo.z * vy.wi.z > 0 ? mk<R>(sh.color) : V3<R>{0, 0, 0};
+        } else {
+            refraction_sample(sh, vy, u0, u1);
+        }
+    } else if (sh.type == KR_SHADER_REFRACTION) {
+        refraction_sample(sh, vy, u0, u1);
+    } else if (sh.type == KR_SHADER_TRANSPARENT) {
+        vy.wi = -vy.wo;
+        vy.f = mk<R>(sh.color);
+    } else {
+        vy.wi = V3<R>{0, 0, 1};
+        vy.f = V3<R>{0, 0, 0};
     }
 }
 
@@ -688,6 +805,13 @@ template <class R> inline void shader_sample(const kr_shader_desc &sh, Vary<R> &
 template <class R>
 inline V3<R> light_sample(const Scene<R> &s, const Light<R> &L, const Vary<R> &res, R u0, R u1, V3<R> &li,
                           uint64_t *shadow_count) {
+    if (L.point) {
+        // PointLight::sample (src/core/light/pointLight.hpp:20-27): no pdf, no falloff
+        Seg<R> seg = res.emit_seg(L.pos);
+        if (shadow_count) ++*shadow_count;
+        li = s.occluded(seg) ? V3<R>{0, 0, 0} : L.emission;
+        return res.local(normalize(L.pos - res.p));
+    }
     const Prim &p = s.prims[L.prim];
     LV<R> lv;
     R pdf;
@@ -719,6 +843,25 @@ struct PathStats {
     Counters trav;
 };
 
+// NormalKernel / VisibilityKernel / HalfVectorKernel (src/core/kernel/
+// normal.hpp:17-28, visibility.hpp:17-35, halfVector.hpp:17-37): one hit,
+// light 0 sampled with dimensions 1-2 of bounce 0. With no light the
+// reference indexes an empty list; here the pixel stays black.
+template <class R>
+inline V3<R> debug_kernel(uint32_t kernel, const Scene<R> &s, Ray<R> ray, PathRng &rng, PathStats *st) {
+    Vary<R> vy;
+    if (st) st->closest++;
+    if (!scene_intersect(s, ray, vy, st ? &st->trav : nullptr)) return V3<R>{0, 0, 0};
+    if (kernel == KR_KERNEL_NORMAL) return vy.n * R(.5f) + V3<R>{R(.5f), R(.5f), R(.5f)};
+    if (s.lights.empty()) return V3<R>{0, 0, 0};
+    V3<R> li;
+    vy.wi = light_sample(s, s.lights[0], vy, R(rng.get(0, 1)), R(rng.get(0, 2)), li, st ? &st->shadow : nullptr);
+    if (kernel == KR_KERNEL_VISIBILITY)
+        return (li.x == R(0) && li.y == R(0) && li.z == R(0)) ? V3<R>{0, 0, 0} : V3<R>{1, 1, 1};
+    R v = std::acos(normalize(vy.wi + vy.wo).z) * R(2) / Const<R>::pi();
+    return V3<R>{v, v, v};
+}
+
 // RadianceKernel::execute (src/core/kernel/radiance.cu:14-81).
 template <class R>
 inline V3<R> radiance(const Scene<R> &s, Ray<R> ray, PathRng &rng, uint32_t max_bounce, PathStats *st) {
@@ -728,19 +871,25 @@ inline V3<R> radiance(const Scene<R> &s, Ray<R> ray, PathRng &rng, uint32_t max_
     for (uint32_t bounce = 0;; ++bounce) {
         if (st) st->closest++;
         if (!(scene_intersect(s, ray, vy, st ? &st->trav : nullptr) && bounce <= max_bounce)) break;
-        const kr_shader_desc &sh = s.shaders[vy.shader];
+        int b = int(bounce);
         if (nL) {
             V3<R> li;
-            R xi = R(rng.get(int(bounce), 0));
+            R xi = R(rng.get(b, 0));
             uint32_t idx = smin(uint32_t(std::floor(xi * R(nL))), nL - 1);
             R lpdf = R(1) / R(nL);
-            vy.wi = light_sample(s, s.lights[idx], vy, R(rng.get(int(bounce), 1)), R(rng.get(int(bounce), 2)), li,
+            vy.wi = light_sample(s, s.lights[idx], vy, R(rng.get(b, 1)), R(rng.get(b, 2)), li,
                                  st ? &st->shadow : nullptr);
-            shader_eval(sh, vy);
+            int dim = 8;
+            const kr_shader_desc &sh = s.shaders[resolve_shader(s.shaders, vy.shader, vy, rng, b, dim)];
+            R uc = sh.type == KR_SHADER_GLASS ? R(rng.get(b, dim++)) : R(0);
+            shader_eval(sh, vy, uc);
             L += beta * vy.f * li * std::fabs(dot(vy.wi, V3<R>{0, 0, 1})) / lpdf;
         }
         L += beta * vy.emission;
-        shader_sample(sh, vy, R(rng.get(int(bounce), 3)), R(rng.get(int(bounce), 4)), R(rng.get(int(bounce), 5)));
+        int dim = 12;
+        const kr_shader_desc &sh = s.shaders[resolve_shader(s.shaders, vy.shader, vy, rng, b, dim)];
+        R uc = sh.type == KR_SHADER_GLASS ? R(rng.get(b, dim++)) : R(0);
+        shader_sample(sh, vy, R(rng.get(b, 3)), R(rng.get(b, 4)), R(rng.get(b, 5)), uc);
         beta *= vy.f * std::fabs(dot(vy.wi, V3<R>{0, 0, 1}));
         ray = vy.emit_ray(vy.global(vy.wi));
         R rr = smax(beta.x, smax(beta.y, beta.z));
@@ -782,7 +931,9 @@ inline double render(const Scene<R> &s, const kr_camera &cam, const kr_params &p
                 uint32_t nv = 0;
                 for (uint32_t k = spp_begin; k != spp_begin + spp_count; ++k) {
                     PathRng rng(px, k, prm.seed);
-                    V3<R> r = radiance(s, lens.sample(x, y, k), rng, prm.max_bounce, st);
+                    V3<R> r = prm.kernel == KR_KERNEL_RADIANCE
+                                  ? radiance(s, lens.sample(x, y, k), rng, prm.max_bounce, st)
+                                  : debug_kernel(prm.kernel, s, lens.sample(x, y, k), rng, st);
                     if (!hasnan(r) && !hasinf(r)) {
                         nv++;
                         acc[0] += double(r.x);

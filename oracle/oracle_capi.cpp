@@ -183,22 +183,37 @@ double OR_FN(oracle_beckmann_f)(const double *w, double alpha2) { return double(
 void OR_FN(oracle_reflect)(const double *i, const double *n, double *out) { put(out, reflect(dv(i), dv(n))); }
 
 // Shader sample/eval in the local frame: f and emission out.
+// Leaf shaders only; uc is the Glass branch uniform. wi_in is the wi a
+// failed refraction leaves in place.
 void OR_FN(oracle_shader_sample)(const kr_shader_desc *sh, const double *wo, double u0, double u1, double u2,
-                                 double *wi, double *f) {
+                                 double uc, const double *wi_in, double *wi, double *f) {
     Vary<R> vy;
     vy.wo = dv(wo);
-    shader_sample(*sh, vy, R(u0), R(u1), R(u2));
+    if (wi_in) vy.wi = dv(wi_in);
+    shader_sample(*sh, vy, R(u0), R(u1), R(u2), R(uc));
     put(wi, vy.wi);
     put(f, vy.f);
 }
-void OR_FN(oracle_shader_eval)(const kr_shader_desc *sh, const double *wo, const double *wi, double *f,
+void OR_FN(oracle_shader_eval)(const kr_shader_desc *sh, const double *wo, const double *wi, double uc, double *f,
                                double *emission) {
     Vary<R> vy;
     vy.wo = dv(wo);
     vy.wi = dv(wi);
-    shader_eval(*sh, vy);
+    shader_eval(*sh, vy, R(uc));
     put(f, vy.f);
     put(emission, vy.emission);
+}
+
+double OR_FN(oracle_fresnel)(double ior, const double *wo) {
+    Vary<R> vy;
+    vy.wo = dv(wo);
+    return double(fresnel(R(ior), vy));
+}
+int OR_FN(oracle_refract)(const double *i, const double *n, double eta, double *out) {
+    V3<R> o{0, 0, 0};
+    bool ok = refract(dv(i), o, dv(n), R(eta));
+    put(out, o);
+    return ok ? 1 : 0;
 }
 
 // Frame of a hit: u = normalize(n x (0,1,0)), v = normalize(n x u); local/global.

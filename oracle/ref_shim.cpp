@@ -14,6 +14,7 @@
 #include <core/primitive/sphere.hpp>
 #include <ext/distribution/beckmann.hpp>
 #include <ext/distribution/cosine.hpp>
+#include <ext/scala/fresnel.hpp>
 #include <util/hemisphere.hpp>
 
 using namespace koishi;
@@ -112,6 +113,22 @@ double ref_cos_f(const double *w) {
     Properties props;
     ext::CosDistribution dist(props);
     return dist.f(solid(f3(w))).x;
+}
+
+int ref_refract(const double *i, const double *n, double eta, double *out) {
+    solid r(float3{0, 0, 1});
+    bool ok = refract(solid(f3(i)), r, solid(f3(n)), float(eta));
+    put(out, ok ? float3(r) : float3{0, 0, 0});
+    return ok ? 1 : 0;
+}
+
+double ref_fresnel(double ior, const double *wo) {
+    char buf[32];
+    HybridAllocator al(buf, sizeof buf);
+    Varyings vy;
+    vy.wo = solid(f3(wo));
+    ext::Fresnel fr(Properties{{"ior", float(ior)}});
+    return fr.compute(vy, al);
 }
 
 void ref_reflect(const double *i, const double *n, double *out) {

@@ -60,8 +60,12 @@ def oracle():
             g("oracle_beckmann_f").argtypes = [PD, D]
             g("oracle_beckmann_f").restype = D
             g("oracle_reflect").argtypes = [PD, PD, PD]
-            g("oracle_shader_sample").argtypes = [C.POINTER(kr.ShaderDesc), PD, D, D, D, PD, PD]
-            g("oracle_shader_eval").argtypes = [C.POINTER(kr.ShaderDesc), PD, PD, PD, PD]
+            g("oracle_shader_sample").argtypes = [C.POINTER(kr.ShaderDesc), PD, D, D, D, D, PD, PD, PD]
+            g("oracle_shader_eval").argtypes = [C.POINTER(kr.ShaderDesc), PD, PD, D, PD, PD]
+            g("oracle_fresnel").argtypes = [D, PD]
+            g("oracle_fresnel").restype = D
+            g("oracle_refract").argtypes = [PD, PD, D, PD]
+            g("oracle_refract").restype = C.c_int
             g("oracle_frame").argtypes = [PD, PD, PD, PD, PD, PD]
             g("oracle_emit_seg").argtypes = [PD, PD, PD, PD, PD]
         lib.oracle_philox.argtypes = [VP, VP, VP]

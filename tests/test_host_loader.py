@@ -105,7 +105,12 @@ def test_obj_polygons_negative_indices_and_generated_normals(tmp_path):
     (lambda t: _base_doc(t, shaders={"floor": ["Velvet", {}], "lamp": ["Emission", {}]}), "no such type: Velvet"),
     (lambda t: _base_doc(t, shaders={"floor": ["GlossyBSDF", {}], "lamp": ["Emission", {}]}),
      'No such key named "distribution"'),
-    (lambda t: _base_doc(t, shaders={"floor": ["GlassBSDF", {}], "lamp": ["Emission", {}]}), "not provided"),
+    (lambda t: _base_doc(t, shaders={"floor": ["GlassBSDF", {}], "lamp": ["Emission", {}]}), 'No such key named "ior"'),
+    (lambda t: _base_doc(t, shaders={"floor": ["Mixed", {"fac": ["Constant", {"value": 0.5}],
+                                                          "shaders": [["Lambert", {}]]}],
+                                     "lamp": ["Emission", {}]}), "Mixed needs"),
+    (lambda t: _base_doc(t, shaders={"floor": ["Mixed", {"fac": ["Noise", {}], "shaders": [["Lambert", {}]] * 2}],
+                                     "lamp": ["Emission", {}]}), "no such type: Noise"),
     (lambda t: {"scene": {"camera": [{"lens": "fisheye"}], "assets": []}}, "unknown lens type: fisheye"),
     (lambda t: {"scene": {"assets": [["sphere", {"o": [0, 0, 0], "r": 1}]]}}, 'No such key named "shader"'),
     (lambda t: "{\"scene\": [1, 2,,]}", "json parse error at line 1"),

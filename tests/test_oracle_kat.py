@@ -319,3 +319,125 @@ def test_reflect_identities():
         assert np.linalg.norm(r) == pytest.approx(1.0, abs=1e-12)
         assert r @ h == pytest.approx(wo @ h, abs=1e-12)
         assert np.allclose(r + wo, 2 * (wo @ h) * h, atol=1e-12)
+
+
+# ------------------------------------------ refraction, Fresnel, new BSDFs --
+def _pd(a):
+    return a.ctypes.data_as(ol.PD)
+
+
+def _refract(real, i, n, eta):
+    out = np.zeros(3)
+    ok = ol.fn("oracle_refract", real)(d3(i), d3(n), eta, _pd(out))
+    return bool(ok), out
+
+
+@pytest.mark.parametrize("real", REALS)
+def test_refract_and_fresnel_match_reference_known_answers(real):
+    # test/math.cu:6-29: refract through n = z at 1/1.33 and back, Fresnel(1.4)
+    wo = np.array([1.0, 0.0, 1.0]) / math.sqrt(2.0)
+    ok, wi = _refract(real, wo, [0, 0, 1], 1 / 1.33)
+    assert ok and np.allclose(wi, [-0.531659, 0.0, -0.846958], atol=1e-4)
+    ok, wi2 = _refract(real, wi, [0, 0, -1], 1.33)
+    assert ok and np.allclose(wi2, [0.707107, 0.0, 0.707107], atol=1e-4)
+    assert ol.fn("oracle_fresnel", real)(1.4, d3(wo)) == pytest.approx(0.0365785, abs=1e-4)
+
+
+def test_refract_fresnel_total_internal_reflection_and_limits():
+    # beyond the critical angle refract fails and Fresnel reflects everything
+    ok, _ = _refract("f64", [math.sin(1.2), 0, -math.cos(1.2)], [0, 0, -1], 1.5)
+    assert not ok
+    assert ol.fn("oracle_fresnel", "f64")(1.5, d3([math.sin(1.2), 0, -math.cos(1.2)])) == 1.0
+    # normal incidence: ((n - 1) / (n + 1))^2
+    assert ol.fn("oracle_fresnel", "f64")(1.5, d3([0, 0, 1])) == pytest.approx((0.5 / 2.5) ** 2, rel=1e-9)
+    # Snell: n1 sin(i) = n2 sin(t) for a refracted direction
+    for th in (0.1, 0.5, 0.9):
+        ok, t = _refract("f64", [math.sin(th), 0, math.cos(th)], [0, 0, 1], 1 / 1.5)
+        assert ok and math.sin(th) == pytest.approx(1.5 * abs(t[0]), rel=1e-9) and t[2] < 0
+
+
+def test_refract_fresnel_vs_reference_headers():
+    ref = ol.ref()
+    if ref is None:
+        pytest.skip("reference shim not built")
+    ref.ref_refract.argtypes = [ol.PD, ol.PD, D, ol.PD]
+    ref.ref_refract.restype = C.c_int
+    ref.ref_fresnel.argtypes = [D, ol.PD]
+    ref.ref_fresnel.restype = D
+    rng = np.random.default_rng(7)
+    for _ in range(400):
+        i = rng.normal(size=3)
+        i /= np.linalg.norm(i)
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        eta = float(rng.choice([1 / 1.5, 1.5, 1 / 1.33, 1.33]))
+        a = np.zeros(3)
+        oka = ref.ref_refract(d3(i), d3(n), eta, _pd(a))
+        okb, b = _refract("f32", i, n, eta)
+        assert bool(oka) == okb
+        if okb:
+            assert np.allclose(a, b, atol=2e-6)
+        ior = float(rng.uniform(1.1, 2.4))
+        assert ref.ref_fresnel(ior, d3(i)) == pytest.approx(ol.fn("oracle_fresnel", "f32")(ior, d3(i)), abs=2e-6)
+
+
+def _shader(typ, **kw):
+    from koishi_amd import kr
+    d = kr.ShaderDesc()
+    d.type = typ
+    d.color = (kr.f32 * 3)(*kw.get("color", (0.8, 0.6, 0.4)))
+    d.R = (kr.f32 * 3)(1, 1, 1)
+    d.roughness = kw.get("roughness", 0.5)
+    d.distribution = kw.get("dist", 0)
+    d.ior = kw.get("ior", 1.5)
+    return d
+
+
+def _sample(real, sh, wo, u0, u1, uc, wi_in=(9.0, 9.0, 9.0)):
+    wi, f = np.zeros(3), np.zeros(3)
+    ol.fn("oracle_shader_sample", real)(C.byref(sh), d3(wo), u0, u1, 0.0, uc, d3(wi_in), _pd(wi), _pd(f))
+    return wi, f
+
+
+def test_transparent_blackhole_refraction_glass_closed_forms():
+    from koishi_amd import kr
+    wo = np.array([0.3, -0.2, 0.9])
+    wo /= np.linalg.norm(wo)
+    # Transparent (transparent.hpp:20-23): straight through with f = color
+    wi, f = _sample("f64", _shader(kr.SHADER_TRANSPARENT), wo, 0.3, 0.7, 0.0)
+    assert np.allclose(wi, -wo) and np.allclose(f, [0.8, 0.6, 0.4], atol=1e-7)
+    # Blackhole (blackhole.hpp:14-21): wi = z, f = 0
+    wi, f = _sample("f64", _shader(kr.SHADER_BLACKHOLE), wo, 0.3, 0.7, 0.0)
+    assert np.allclose(wi, [0, 0, 1]) and not f.any()
+    # Beckmann with u0 = 1 samples the macro normal exactly (tan^2 = -a^2 ln 1 = 0):
+    # RefractionBSDF then refracts through z (Snell) with f = color / e^2 (e < 1)
+    ok, t = _refract("f64", wo, [0, 0, 1], 1 / 1.5)
+    wi, f = _sample("f64", _shader(kr.SHADER_REFRACTION), wo, 1.0, 0.25, 0.0)
+    assert ok and np.allclose(wi, t, atol=1e-12)
+    assert np.allclose(f, np.array([0.8, 0.6, 0.4]) / (1 / 1.5) ** 2, rtol=1e-6)
+    # GlassBSDF: reflection when uc < Fresnel(ior) (glass.hpp:22-46), refraction otherwise
+    fr = ol.fn("oracle_fresnel", "f64")(1.5, d3(wo))
+    wi, f = _sample("f64", _shader(kr.SHADER_GLASS), wo, 1.0, 0.25, fr * 0.5)
+    assert np.allclose(wi, [-wo[0], -wo[1], wo[2]], atol=1e-12) and np.allclose(f, [0.8, 0.6, 0.4], atol=1e-7)
+    wi, f = _sample("f64", _shader(kr.SHADER_GLASS), wo, 1.0, 0.25, min(1.0, fr * 1.5 + 0.01))
+    assert np.allclose(wi, t, atol=1e-12)
+    # total internal reflection leaves wi as it was, with f = 0
+    wo_in = np.array([math.sin(1.2), 0, -math.cos(1.2)])
+    wi, f = _sample("f64", _shader(kr.SHADER_REFRACTION), wo_in, 1.0, 0.25, 0.0, wi_in=(0.1, 0.2, 0.3))
+    assert np.allclose(wi, [0.1, 0.2, 0.3]) and not f.any()
+
+
+def test_glass_eval_and_zero_eval_shaders():
+    from koishi_amd import kr
+    wo = np.array([0.2, 0.1, 0.95])
+    wo /= np.linalg.norm(wo)
+    wi = np.array([-0.4, 0.3, 0.8])
+    wi /= np.linalg.norm(wi)
+    f, e = np.zeros(3), np.zeros(3)
+    # Cosine distribution: D(h) = 1/pi for h in the upper hemisphere
+    sh = _shader(kr.SHADER_GLASS, dist=kr.DIST_COSINE)
+    ol.fn("oracle_shader_eval", "f64")(C.byref(sh), d3(wo), d3(wi), 0.0, _pd(f), _pd(e))
+    assert np.allclose(f, np.array([0.8, 0.6, 0.4]) / math.pi, rtol=1e-6)
+    for typ in (kr.SHADER_REFRACTION, kr.SHADER_TRANSPARENT, kr.SHADER_BLACKHOLE):
+        ol.fn("oracle_shader_eval", "f64")(C.byref(_shader(typ)), d3(wo), d3(wi), 0.5, _pd(f), _pd(e))
+        assert not f.any() and not e.any()
