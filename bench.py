@@ -109,6 +109,9 @@ def bench_cache(args, kr, scene, gs, p, dev):
     nrc = kr.Nrc(dev, lo, hi, train_log2=4, steps_per_frame=4, batch=16384, target_clamp=emax)
     cam = scene.camera_struct()
 
+    fx = np.load(os.path.join(ROOT, "tests", "golden", f"oracle_{args.workload}.npz"))
+    fpix, fref = fx["pixels"].astype(np.int64), fx["mean"]
+
     def frame(r, spp):
         r.reset()
         t0 = time.perf_counter()
@@ -116,23 +119,32 @@ def bench_cache(args, kr, scene, gs, p, dev):
         r.sync()
         dt = time.perf_counter() - t0
         rgb, val = r.read()
-        return dt, rmse_vs_oracle(args.workload, rgb, val, p.width)
+        return dt, rgb[fpix] / np.maximum(val[fpix], 1)[:, None]
 
-    def mse_over_seeds(mode, s):
-        """Best time and the MSE averaged over 3 independent seeds (single
-        64-spp frames of these scenes differ by tens of percent in RMSE)."""
-        dts, e, ec = [], 0.0, 0.0
+    def over_seeds(mode, s):
+        """Best time, RMSE against the oracle averaged over 3 independent seeds
+        (as MSE), and the bias: the error of the 3-seed mean minus its own
+        noise, b^2 = |mean - ref|^2 - var / 3 (var from the spread of the
+        seeds), unclamped and clamped to [0, 1]."""
+        dts, imgs = [], []
         for seed in (11, 12, 13):
             r = kr.Render(gs, cam, scene.params(seed=seed, spp=s))
             if mode == "cached":
                 r.attach_nrc(nrc)
             frame(r, s)
-            dt, err = frame(r, s)
+            dt, img = frame(r, s)
             r.free()
             dts.append(dt)
-            e += err["unclamped"] ** 2 / 3
-            ec += err["clamped"] ** 2 / 3
-        return {"seconds": min(dts), "rmse": e ** 0.5, "rmse_clamped": ec ** 0.5}
+            imgs.append(img)
+        out = {"seconds": min(dts)}
+        for key, f in (("", lambda x: x), ("_clamped", lambda x: np.clip(x, 0.0, 1.0))):
+            im = np.stack([f(x) for x in imgs])
+            ref = f(fref)
+            out["rmse" + key] = float(np.sqrt(((im - ref) ** 2).mean()))
+            var = im.var(axis=0, ddof=1).mean()
+            b2 = ((im.mean(axis=0) - ref) ** 2).mean() - var / 3
+            out["bias" + key] = float(np.sign(b2) * np.sqrt(abs(b2)))
+        return out
 
     warm = 0
     t_warm = time.perf_counter()
@@ -150,7 +162,7 @@ def bench_cache(args, kr, scene, gs, p, dev):
     while s <= p.spp:
         res = {"spp": s}
         for mode in ("plain", "cached"):
-            res[mode] = mse_over_seeds(mode, s)
+            res[mode] = over_seeds(mode, s)
         log(f"[bench] spp {s}: plain {res['plain']['seconds'] * 1e3:.1f} ms rmse {res['plain']['rmse']:.4f} / "
             f"{res['plain']['rmse_clamped']:.4f} clamped, cached {res['cached']['seconds'] * 1e3:.1f} ms rmse "
             f"{res['cached']['rmse']:.4f} / {res['cached']['rmse_clamped']:.4f} clamped")
@@ -162,14 +174,6 @@ def bench_cache(args, kr, scene, gs, p, dev):
     # dominated by rare very bright samples and is reported beside it)
     target = rows[-1]["plain"]["rmse_clamped"] if rows else None
 
-    def bias_fit(mode, key):
-        """Least-squares fit of RMSE^2(N) = b^2 + sigma^2 / N over the frames
-        (bias b, noise sigma); b ~ 0 for an unbiased estimator."""
-        ns = np.array([r["spp"] for r in rows], np.float64)
-        m2 = np.array([r[mode][key] ** 2 for r in rows])
-        A = np.stack([np.ones_like(ns), 1.0 / ns], axis=1)
-        (b2, s2), *_ = np.linalg.lstsq(A, m2, rcond=None)
-        return {"bias": float(np.sign(b2) * np.sqrt(abs(b2))), "sigma": float(np.sqrt(max(s2, 0.0)))}
     reach = [r for r in rows if target is not None and r["cached"]["rmse_clamped"] <= target]
     plain_time = rows[-1]["plain"]["seconds"] if rows else None
     cached_time = min((r["cached"]["seconds"] for r in reach), default=None)
@@ -181,7 +185,8 @@ def bench_cache(args, kr, scene, gs, p, dev):
         "speedup": (plain_time / cached_time) if cached_time else None,
         "warmup_frames": warm, "warmup_seconds": t_warm,
         "train_fraction": 1 / 16, "samples_per_frame": 4 * 16384,
-        "bias_fit_clamped": {m: bias_fit(m, "rmse_clamped") for m in ("plain", "cached")} if len(rows) > 1 else None,
+        "bias_at_full_spp": {m: {"unclamped": rows[-1][m]["bias"], "clamped": rows[-1][m]["bias_clamped"]}
+                             for m in ("plain", "cached")} if rows else None,
         "by_spp": rows,
     }
 

@@ -49,21 +49,15 @@ struct Weights {
     float box_min[3], box_scale[3];
 };
 
-__device__ __forceinline__ uint16_t to_bf16(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x7FFFFFu)) return uint16_t((u >> 16) | 0x40u);  // NaN
-    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
-    return uint16_t(u >> 16);
-}
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+// FP32 -> BF16, round to nearest even (gfx950's v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 __device__ __forceinline__ v4s pack4(float a, float b, float c, float d) {
-    v4s v;
-    v[0] = short(to_bf16(a));
-    v[1] = short(to_bf16(b));
-    v[2] = short(to_bf16(c));
-    v[3] = short(to_bf16(d));
-    return v;
+    v4bf v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+    return __builtin_bit_cast(v4s, v);
 }
 
 __device__ __forceinline__ void angles(float x, float y, float z, float &th, float &ph) {
@@ -82,19 +76,26 @@ __device__ __forceinline__ RawRec load_raw(const float *raw) {
     return RawRec{a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x};
 }
 
-// Feature `idx` of a vertex record: sin / cos(2^k pi p) of the normalised
-// position (idx < 72, ordered [sin xyz, cos xyz] per k), the outgoing
-// direction and normal as (theta/pi, phi/2pi + 1/2), roughness, albedo,
-// zero padding. Selects instead of indexed loads keep it in registers.
+// Position feature idx < 72 of the normalised position (px, py, pz):
+// sin / cos(2^k pi p_c) with k = idx / 6, j = idx % 6, c = j % 3, cos for
+// j >= 3. 2^k pi p = 2 pi (2^(k-1) p), and both the revolution count
+// 2^(k-1) p and its fractional part are exact in FP32, so the hardware's
+// v_sin / v_cos (which take revolutions) see an exactly reduced argument.
+__device__ __forceinline__ float pos_feature(int idx, float px, float py, float pz) {
+    int k = idx / 6, j = idx - 6 * k;
+    int c = j >= 3 ? j - 3 : j;
+    float p = c == 0 ? px : (c == 1 ? py : pz);
+    float rev = __builtin_amdgcn_fractf(ldexpf(p, k - 1));
+    return j >= 3 ? __builtin_amdgcn_cosf(rev) : __builtin_amdgcn_sinf(rev);
+}
+
+// Feature `idx` of a vertex record: the position features above, the
+// outgoing direction and normal as (theta/pi, phi/2pi + 1/2), roughness,
+// albedo, zero padding.
 __device__ __forceinline__ float feature(int idx, const RawRec &r, const Weights &W) {
-    if (idx < 72) {
-        int k = idx / 6, j = idx - 6 * k;
-        int c = j >= 3 ? j - 3 : j;
-        float p = c == 0 ? (r.px - W.box_min[0]) * W.box_scale[0]
-                         : (c == 1 ? (r.py - W.box_min[1]) * W.box_scale[1] : (r.pz - W.box_min[2]) * W.box_scale[2]);
-        float a = float(1 << k) * 3.14159265358979f * p;
-        return j < 3 ? sinf(a) : cosf(a);
-    }
+    if (idx < 72)
+        return pos_feature(idx, (r.px - W.box_min[0]) * W.box_scale[0], (r.py - W.box_min[1]) * W.box_scale[1],
+                           (r.pz - W.box_min[2]) * W.box_scale[2]);
     if (idx < 76) {
         bool nrm = idx >= 74;
         float x = nrm ? r.nx : r.wx, y = nrm ? r.ny : r.wy, z = nrm ? r.nz : r.wz;
@@ -111,27 +112,43 @@ __device__ __forceinline__ float feature(int idx, const RawRec &r, const Weights
 
 // Input fragments of a wave's 64 samples: in[kt][nt] for kt < 6, nt < 4.
 // raw_of(n) gives the record of sample n of the wave (nullptr: padding,
-// encoded as zeros).
+// encoded as zeros). Lane (s, q) holds features 16 kt + 4 q .. + 3: for
+// kt < 4 and for kt = 4, q < 2 these are position features; kt = 4, q = 2
+// the two direction pairs; kt = 4, q = 3 roughness and albedo; kt = 5 is
+// padding. Each lane evaluates only its own features.
 template <class RawOf>
 __device__ __forceinline__ void encode_inputs(v4s in[6][4], RawOf raw_of, const Weights &W) {
     int lane = __lane_id(), s = lane & 15, q = lane >> 4;
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
         const float *raw = raw_of(nt * 16 + s);
-        RawRec r{};
-        if (raw) r = load_raw(raw);
+        float f[20];
 #pragma unroll
-        for (int kt = 0; kt < 6; ++kt) {
-            float f0 = 0.f, f1 = 0.f, f2 = 0.f, f3 = 0.f;
-            if (raw) {
-                int b = kt * 16 + 4 * q;
-                f0 = feature(b, r, W);
-                f1 = feature(b + 1, r, W);
-                f2 = feature(b + 2, r, W);
-                f3 = feature(b + 3, r, W);
+        for (int i = 0; i < 20; ++i) f[i] = 0.f;
+        if (raw) {
+            RawRec r = load_raw(raw);
+            float px = (r.px - W.box_min[0]) * W.box_scale[0], py = (r.py - W.box_min[1]) * W.box_scale[1],
+                  pz = (r.pz - W.box_min[2]) * W.box_scale[2];
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) f[4 * kt + i] = pos_feature(kt * 16 + 4 * q + i, px, py, pz);
+            if (q < 2) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) f[16 + i] = pos_feature(64 + 4 * q + i, px, py, pz);
+            } else if (q == 2) {
+                angles(r.wx, r.wy, r.wz, f[16], f[17]);
+                angles(r.nx, r.ny, r.nz, f[18], f[19]);
+            } else {
+                f[16] = r.rough;
+                f[17] = r.ar;
+                f[18] = r.ag;
+                f[19] = r.ab;
             }
-            in[kt][nt] = pack4(f0, f1, f2, f3);
         }
+#pragma unroll
+        for (int kt = 0; kt < 5; ++kt) in[kt][nt] = pack4(f[4 * kt], f[4 * kt + 1], f[4 * kt + 2], f[4 * kt + 3]);
+        in[5][nt] = v4s{0, 0, 0, 0};
     }
 }
 

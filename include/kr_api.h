@@ -396,7 +396,7 @@ typedef struct kr_batch_job {
 typedef struct kr_batch_tuning {
     uint32_t bvh_width;     /* 2, 4 or 8                                     */
     uint32_t chunk_paths;   /* paths in flight per wavefront chunk           */
-    uint32_t blocks_per_cu; /* 256-thread blocks per CU of every launch (1..16) */
+    uint32_t blocks_per_cu; /* 256-thread blocks per CU of every launch (1..64) */
     uint32_t flags;         /* KR_RENDER_* bits                              */
     double paths_per_second;/* kr_batch_autotune: the winner's trial rate    */
 } kr_batch_tuning;
@@ -414,13 +414,13 @@ typedef struct kr_batch_info {
 
 /* Copies the scene descriptions (the batch rebuilds their BVHs when the
  * tuning changes the width) and uploads them. tuning NULL: width 4, 32 M
- * paths per chunk, 8 blocks per CU. */
+ * paths per chunk, 64 blocks per CU. */
 int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint32_t num_scenes, const kr_batch_job *jobs,
                     uint32_t num_jobs, const kr_batch_tuning *tuning, kr_batch **out);
 /* Autotuner: times trial renders of the first trial_spp samples of every
  * job (0: enough samples for about 64 M paths) for every BVH width in
  * {2, 4, 8}, chunk size in {4, 8, 16, 32} M paths and blocks per CU in
- * {4, 8, 16}, applies the fastest and returns it. */
+ * {16, 32, 64}, applies the fastest and returns it. */
 int kr_batch_autotune(kr_batch *batch, uint32_t trial_spp, kr_batch_tuning *chosen);
 /* Renders every sample of every job (adds to the accumulators) with one
  * launch of the captured graph; asynchronous. */

@@ -76,7 +76,7 @@ def test_batch_autotune_picks_a_valid_tuning(device):
     assert t.bvh_width in (2, 4, 8)
     assert t.chunk_paths in (4 << 20, 8 << 20, 16 << 20, 32 << 20) or t.chunk_paths <= sum(
         p.width * p.height * p.spp for _, _, p in jobs)
-    assert t.blocks_per_cu in (4, 8, 16) and t.paths_per_second > 0
+    assert t.blocks_per_cu in (16, 32, 64) and t.paths_per_second > 0
     info = b.info()
     assert info.tuning.bvh_width == t.bvh_width
     # accumulators were cleared after the trials; the tuned batch still
@@ -99,7 +99,7 @@ def test_batch_rejects_bad_input(device):
     with pytest.raises(kr.KrError):
         kr.Batch(device, scenes, jobs, kr.BatchTuning(3, 0, 8, 0, 0.0))
     with pytest.raises(kr.KrError):
-        kr.Batch(device, scenes, jobs, kr.BatchTuning(4, 0, 17, 0, 0.0))
+        kr.Batch(device, scenes, jobs, kr.BatchTuning(4, 0, 65, 0, 0.0))
     b = kr.Batch(device, scenes[:1], jobs[:1])
     with pytest.raises(kr.KrError):
         b.read(1)
