@@ -51,19 +51,26 @@ int fail(int code, const std::string &msg) {
 constexpr int kBlock = 256;
 // Traversal stack entries per lane kept in LDS, and the waves per SIMD the
 // traversal kernels and the shading kernel ask the register allocator for
-// (1: its own choice).
+// (1: its own choice). At 6 waves the closest-hit kernel fits 80 VGPRs
+// without spilling (its own choice, 82, allows 5); with a 10-entry LDS stack
+// (20 KB per block) that measured 5 % faster on the 100k-triangle frame than
+// 12 entries at 5 waves, and equal on the 1M-triangle frame.
 #ifndef KR_LDS_DEPTH
-#define KR_LDS_DEPTH 12
+#define KR_LDS_DEPTH 10
 #endif
 #ifndef KR_TRACE_WAVES
-#define KR_TRACE_WAVES 1
+#define KR_TRACE_WAVES 6
 #endif
 #ifndef KR_SHADE_WAVES
 #define KR_SHADE_WAVES 1
 #endif
 constexpr int kLdsDepth = KR_LDS_DEPTH;
-// traversal stack entries per lane beyond the LDS part, in HBM
-constexpr int kSpillDepth = 96;
+// The 6-wave hint applies to the plain BVH2 / BVH4 traversals; the BVH8,
+// batch and compressed-node variants need more registers and would spill.
+constexpr int trace_waves(int w, bool batch, bool q) { return (w <= 4 && !batch && !q) ? KR_TRACE_WAVES : 1; }
+// traversal stack entries per lane beyond the LDS part, in HBM (108 in all:
+// BVH8 trees to depth 14, BVH4 to depth 34)
+constexpr int kSpillDepth = 108 - kLdsDepth;
 
 // Host Philox4x32-10 for the light tables (same function as the device one).
 void philox_host(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
@@ -477,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, JobTable T, uin
 }
 
 template <int W, bool COUNT, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
+__global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
                                                   uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
                                                   unsigned long long *stats, int sort, uint32_t *spill,
                                                   size_t spill_lanes) {
@@ -533,7 +540,7 @@ __global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace(DevScene S, Jo
 constexpr int kMinActive = 32;
 
 template <int W, bool COUNT, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_trace_dyn(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
+__global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace_dyn(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
                                                       uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
                                                       unsigned long long *stats, int sort, uint32_t *spill,
                                                       size_t spill_lanes) {
@@ -901,7 +908,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
 }
 
 template <int W, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_shadow(DevScene S, JobTable T, uint64_t g0, ShadowQ sq, uint32_t *counters,
+__global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(DevScene S, JobTable T, uint64_t g0, ShadowQ sq, uint32_t *counters,
                                                    float *L, unsigned long long *stats, uint32_t *spill,
                                                    size_t spill_lanes, NrcRender nr, uint32_t bounce) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
@@ -1030,7 +1037,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_debug(ShadeArgs a, uint32_t ke
 // Persistent shadow traversal (lane refill as in k_trace_dyn); same
 // additions to L and to the training records as k_shadow.
 template <int W, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock, KR_TRACE_WAVES) void k_shadow_dyn(DevScene S, JobTable T, uint64_t g0, ShadowQ sq,
+__global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow_dyn(DevScene S, JobTable T, uint64_t g0, ShadowQ sq,
                                                        uint32_t *counters, float *L, unsigned long long *stats,
                                                        uint32_t *spill, size_t spill_lanes, NrcRender nr,
                                                        uint32_t bounce) {
@@ -1469,12 +1476,12 @@ int upload(kr_scene *s, const T *host, size_t n, const T **dst) {
 }
 
 // Blocks per CU of a render's grid-stride launches (batches choose their
-// own, see kr_batch_tuning). The traversal kernels fit about 6 blocks per CU
-// (LDS stack), so most blocks queue behind resident ones: a block that
+// own, see kr_batch_tuning). The traversal kernels fit 6 blocks per CU
+// (registers), so most blocks queue behind resident ones: a block that
 // finishes early is replaced, which balances the uneven cost of rays. 64
 // blocks per CU was measured 21 % faster than 8 on the 100k-triangle frame
-// and 13 % on the 1M-triangle frame; the price is the spill stack, 768 B per
-// lane of the grid (3.2 GB on 256 CUs).
+// and 13 % on the 1M-triangle frame; the price is the spill stack, 784 B per
+// lane of the grid (3.3 GB on 256 CUs).
 #ifndef KR_BLOCKS_PER_CU
 #define KR_BLOCKS_PER_CU 64
 #endif

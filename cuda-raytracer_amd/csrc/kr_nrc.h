@@ -2,16 +2,23 @@
 // ReLU MLP with BF16 weights and activations and FP32 accumulation, trained
 // online with Adam (FP32 master weights).
 //
-// Every layer is evaluated transposed, Y^T = W^T X^T, with
-// v_mfma_f32_16x16x16_bf16 on a wave of 64 samples (4 tiles of 16). In that
-// form the 16x16 output tile a lane holds (4 consecutive output features of
-// one sample) is exactly the B-operand fragment the next layer needs, so
-// activations stay in registers from the input encoding to the output.
+// Every layer is evaluated transposed, Y^T = W^T X^T, on a wave of 64
+// samples (4 tiles of 16). In that form the 16x16 output tile a lane holds
+// (4 consecutive output features of one sample) is exactly the B-operand
+// fragment the next layer needs, so activations stay in registers from the
+// input encoding to the output.
 //
 // Fragment layout of v_mfma_f32_16x16x16_bf16 (wave64), lane l = 16 q + s:
 //   A[m][k]: lane holds A[s][4q + r], r = 0..3
 //   B[k][n]: lane holds B[4q + r][s]
 //   D[m][n]: lane holds D[4q + r][s]
+// The forward pass uses gfx950's v_mfma_f32_16x16x32_bf16 (twice the rate):
+// its A and B fragments hold 8 k values per lane, and a lane's 8 slots are
+// contracted against the same 8 slots of the other operand. Filling slots
+// 0-3 from 16-wide k tile 2p and slots 4-7 from tile 2p + 1, for A and B
+// alike, assigns every k of the 32 to exactly one (lane group, slot) pair in
+// both operands, so the sum is the same contraction without moving data
+// between lanes.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -40,7 +47,18 @@ constexpr int bw_off(int l) {
 constexpr int kNumWeights = bw_off(kLayers);
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// One 16x16x32 step over k tiles (2p, 2p + 1): acc += A B with the pairing
+// described above.
+__device__ __forceinline__ v4f mfma_pair(v4s a0, v4s a1, v4s b0, v4s b1, v4f acc) {
+    v8s a = __builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7);
+    v8s b = __builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), acc, 0,
+                                                   0, 0);
+}
 
 struct Weights {
     const uint16_t *wt;  // per layer W^T [out][in], bf16
@@ -153,9 +171,10 @@ __device__ __forceinline__ void encode_inputs(v4s in[6][4], RawOf raw_of, const 
 }
 
 // Hidden layer: Y(mt, nt) = ReLU(sum_kt W^T(mt, kt) X^T(kt, nt) + b), packed
-// to BF16 fragments for the next layer. KT = in/16, MT = out/16.
+// to BF16 fragments for the next layer. KT = in/16 (even), MT = out/16.
 template <int KT, int MT>
 __device__ __forceinline__ void hidden_layer(const v4s x[KT][4], v4s y[MT][4], const uint16_t *wt, const float *bias) {
+    static_assert(KT % 2 == 0, "k tiles are contracted in pairs");
     int lane = __lane_id(), s = lane & 15, q = lane >> 4;
     constexpr int IN = KT * 16;
 #pragma unroll
@@ -169,7 +188,7 @@ __device__ __forceinline__ void hidden_layer(const v4s x[KT][4], v4s y[MT][4], c
         for (int nt = 0; nt < 4; ++nt) {
             v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int kt = 0; kt < KT; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], x[kt][nt], acc, 0, 0, 0);
+            for (int p = 0; p < KT / 2; ++p) acc = mfma_pair(a[2 * p], a[2 * p + 1], x[2 * p][nt], x[2 * p + 1][nt], acc);
             y[mt][nt] = pack4(fmaxf(acc[0] + bb.x, 0.f), fmaxf(acc[1] + bb.y, 0.f), fmaxf(acc[2] + bb.z, 0.f),
                               fmaxf(acc[3] + bb.w, 0.f));
         }
@@ -187,7 +206,7 @@ __device__ __forceinline__ void output_layer(const v4s x[4][4], v4f out[4], cons
     for (int nt = 0; nt < 4; ++nt) {
         v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < 4; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kt], x[kt][nt], acc, 0, 0, 0);
+        for (int p = 0; p < 2; ++p) acc = mfma_pair(a[2 * p], a[2 * p + 1], x[2 * p][nt], x[2 * p + 1][nt], acc);
         acc[0] += bb.x;
         acc[1] += bb.y;
         acc[2] += bb.z;
@@ -199,13 +218,20 @@ __device__ __forceinline__ void output_layer(const v4s x[4][4], v4f out[4], cons
 // Forward of a wave's 64 samples; out[nt] in lanes q = 0 holds the RGB
 // prediction of sample nt*16 + s.
 __device__ __forceinline__ void forward(const v4s in[6][4], v4f out[4], const Weights &W) {
+    // The empty asm makes the weight pointers opaque on every call, so the
+    // compiler cannot hoist all 48 KB of weight fragments out of a caller's
+    // sample loop into registers (which leaves one wave per SIMD, or spills);
+    // they are reloaded from L1/L2 per layer instead.
+    const uint16_t *wt = W.wt;
+    const float *ms = W.master;
+    asm volatile("" : "+s"(wt), "+s"(ms));
     v4s h[4][4], g[4][4];
-    hidden_layer<6, 4>(in, h, W.wt + bw_off(0), W.master + b_off(0));
-    hidden_layer<4, 4>(h, g, W.wt + bw_off(1), W.master + b_off(1));
-    hidden_layer<4, 4>(g, h, W.wt + bw_off(2), W.master + b_off(2));
-    hidden_layer<4, 4>(h, g, W.wt + bw_off(3), W.master + b_off(3));
-    hidden_layer<4, 4>(g, h, W.wt + bw_off(4), W.master + b_off(4));
-    output_layer(h, out, W.wt + bw_off(5), W.master + b_off(5));
+    hidden_layer<6, 4>(in, h, wt + bw_off(0), ms + b_off(0));
+    hidden_layer<4, 4>(h, g, wt + bw_off(1), ms + b_off(1));
+    hidden_layer<4, 4>(g, h, wt + bw_off(2), ms + b_off(2));
+    hidden_layer<4, 4>(h, g, wt + bw_off(3), ms + b_off(3));
+    hidden_layer<4, 4>(g, h, wt + bw_off(4), ms + b_off(4));
+    output_layer(h, out, wt + bw_off(5), ms + b_off(5));
 }
 
 }  // namespace nrc
