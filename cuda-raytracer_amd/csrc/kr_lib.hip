@@ -174,6 +174,15 @@ struct LensDev {
     float n[3], u[3], v[3], du[3], dv[3], o[3];
     uint32_t lens, pw, w, h;
 };
+// Profiler slices (core::Profiler over RadianceKernel::Slice): the paths of
+// the pixels in [x0, x1) x [y0, y1) record their first `depth` vertices.
+constexpr int kProfFloats = 14;  // o, d, f, L, theta, shader id (uint32 bits)
+struct ProfView {
+    int enabled;
+    uint32_t x0, y0, x1, y1, w, npix, spp, depth;
+    float *rec;  // [slice][depth][kProfFloats]
+    float *fin;  // [slice][4]: path radiance, loop counter (uint32 bits)
+};
 
 // One job of a batch, as the kernels see it. A batch's path space is the
 // concatenation of its jobs' sample spaces: job k owns global path indices
@@ -262,6 +271,10 @@ struct kr_render {
     kr_nrc *nrc = nullptr;
     NrcRender nr{};
     std::vector<void *> nrc_allocs;
+    // profiler slices
+    ProfView pv{};
+    uint64_t prof_slices = 0;
+    std::vector<void *> prof_allocs;
 };
 
 namespace {
@@ -313,6 +326,27 @@ __device__ __forceinline__ uint32_t find_job(const JobTable &T, uint64_t g) {
             hi = mid;
     }
     return lo;
+}
+
+// Slice of global path g of a render (Profiler::at, kernel.hpp:62-68:
+// ((y - y0) * (x1 - x0) + (x - x0)) * spp + k), or -1 outside the area.
+__device__ __forceinline__ int64_t prof_slice(const ProfView &pv, uint64_t g) {
+    uint32_t k = uint32_t(g / pv.npix), pixel = uint32_t(g % pv.npix);
+    uint32_t x = pixel % pv.w, y = pixel / pv.w;
+    if (x < pv.x0 || x >= pv.x1 || y < pv.y0 || y >= pv.y1) return -1;
+    return (int64_t(y - pv.y0) * (pv.x1 - pv.x0) + (x - pv.x0)) * pv.spp + k;
+}
+
+// An unoccluded light sample's contribution, added to the L of the
+// profiler record of its vertex (the reference adds it before recording).
+__device__ __forceinline__ void prof_add_light(const ProfView &pv, uint64_t g, uint32_t bounce, float cx, float cy,
+                                               float cz) {
+    int64_t sl = prof_slice(pv, g);
+    if (sl < 0 || bounce >= pv.depth) return;
+    float *pr = pv.rec + (size_t(sl) * pv.depth + bounce) * kProfFloats;
+    pr[9] += cx;
+    pr[10] += cy;
+    pr[11] += cz;
 }
 
 template <int LDS>
@@ -421,19 +455,22 @@ __device__ __forceinline__ void camera_ray(const LensDev &lens, uint32_t s, uint
 }
 
 // Camera paths of a chunk: slot j is global path g0 + j. BATCH: the path's
-// job (and its lens and local index) comes from the job table.
+// job (and its lens and local index) comes from the job table. With a
+// profiler only the paths of its area are traced (CPUMulticoreTracer traces
+// only the profiled area, src/core/tracer/cpuMulticore.cu:17): the queue is
+// then compacted and the other slots keep zero radiance.
 template <bool BATCH>
 __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, JobTable T, uint64_t g0, uint32_t P, uint32_t npix,
                                                    RayQ q, float *L, uint32_t *counters, int qi, NrcRender nr,
-                                                   uint32_t seed_lo, uint32_t seed_hi) {
+                                                   uint32_t seed_lo, uint32_t seed_hi, ProfView pv) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < P; base += stride) {
         uint32_t j = base + threadIdx.x;
         bool active = j < P;
         bool train = false;
+        f3 o{0.f, 0.f, 0.f}, d{0.f, 0.f, 0.f};
         if (active) {
             uint64_t g = g0 + j;
-            f3 o, d;
             uint32_t s, px;
             if (BATCH) {
                 const JobRec &J = T.rec[find_job(T, g)];
@@ -446,16 +483,6 @@ __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, JobTable T, uin
                 px = uint32_t(g % npix);
                 camera_ray(lens, s, px, o, d);
             }
-            q.slot[j] = j;
-            q.ox[j] = o.x;
-            q.oy[j] = o.y;
-            q.oz[j] = o.z;
-            q.dx[j] = d.x;
-            q.dy[j] = d.y;
-            q.dz[j] = d.z;
-            q.bx[j] = 1.f;
-            q.by[j] = 1.f;
-            q.bz[j] = 1.f;
             L[3 * j] = 0.f;
             L[3 * j + 1] = 0.f;
             L[3 * j + 2] = 0.f;
@@ -474,10 +501,28 @@ __global__ __launch_bounds__(kBlock) void k_camera(LensDev lens, JobTable T, uin
             if (train) flags = 1u;
             nr.tslot_of[j] = train ? ts : kMissCode;
         }
-        if (active) q.flags[j] = flags;
+        bool emit = active;
+        uint32_t pos = j;
+        if (!BATCH && pv.enabled) {
+            emit = active && prof_slice(pv, g0 + j) >= 0;
+            pos = wave_append(&counters[CTR(qi)], emit);
+        }
+        if (emit) {
+            q.slot[pos] = j;
+            q.ox[pos] = o.x;
+            q.oy[pos] = o.y;
+            q.oz[pos] = o.z;
+            q.dx[pos] = d.x;
+            q.dy[pos] = d.y;
+            q.dz[pos] = d.z;
+            q.bx[pos] = 1.f;
+            q.by[pos] = 1.f;
+            q.bz[pos] = 1.f;
+            q.flags[pos] = flags;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        counters[CTR(qi)] = P;
+        if (BATCH || !pv.enabled) counters[CTR(qi)] = P;
         if (!BATCH && nr.enabled)
             counters[CTR(C_TRAIN)] = uint32_t(((g0 + P - 1) >> nr.train_log2) - (g0 >> nr.train_log2) + 1);
     }
@@ -671,6 +716,7 @@ struct ShadeArgs {
     uint32_t npix, bounce, max_bounce, seed_lo, seed_hi;
     NrcRender nr;
     JobTable T;  // batch renders: per-path scene, pixel count, depth and seed
+    ProfView pv;
 };
 
 // RadianceKernel::execute loop body (src/core/kernel/radiance.cu:26-71) for
@@ -768,6 +814,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
             // wi of the reference's Varyings: the light direction after NEE, else
             // zero; a failed refraction leaves it in place
             f3 wi_v{0.f, 0.f, 0.f};
+            f3 f_nee{0.f, 0.f, 0.f};  // Varyings::f after the light sample (zero without lights)
             uint32_t eval_leaf = sid;
             if (S.nlights) {
                 // light pick (radiance.cu:32-34) and AreaLight::sample (areaLight.hpp:19-31)
@@ -791,6 +838,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
                 float uce;
                 eval_leaf = resolve_leaf(S, sid, wo, pixel, sample, 2 * a.bounce, seed_lo, seed_hi, uce);
                 f3 f = shader_eval(S.shaders[eval_leaf], wo, wi, uce);
+                f_nee = f;
                 f3 E{lr.emission[0], lr.emission[1], lr.emission[2]};
                 f3 li = lr.type == kLightPoint ? E : E / pdf;
                 f3 c = beta * f * li * fabsf(wi.z) / lpdf;
@@ -831,6 +879,27 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
                 a.L[3 * slot + 1] += L.y;
                 a.L[3 * slot + 2] += L.z;
             }
+            // profiler record of this vertex (radiance.cu:38-50); k_shadow adds
+            // the light sample's contribution to its L once it is unoccluded
+            int64_t psl = a.pv.enabled ? prof_slice(a.pv, a.g0 + slot) : -1;
+            if (psl >= 0 && a.bounce < a.pv.depth) {
+                float *pr = a.pv.rec + (size_t(psl) * a.pv.depth + a.bounce) * kProfFloats;
+                pr[0] = o.x;
+                pr[1] = o.y;
+                pr[2] = o.z;
+                pr[3] = d.x;
+                pr[4] = d.y;
+                pr[5] = d.z;
+                pr[6] = f_nee.x;
+                pr[7] = f_nee.y;
+                pr[8] = f_nee.z;
+                pr[9] = a.L[3 * slot];
+                pr[10] = a.L[3 * slot + 1];
+                pr[11] = a.L[3 * slot + 2];
+                // acos(wo . reflect(wi, n)), reflect(wi, n) = 2 (n . wi) n - wi, n = +z
+                pr[12] = acosf(dot(wo, f3{-wi_v.x, -wi_v.y, wi_v.z}));
+                pr[13] = __uint_as_float(sid);
+            }
             // BSDF sample (radiance.cu:56-60)
             float ucs;
             uint32_t sample_leaf = resolve_leaf(S, sid, wo, pixel, sample, 2 * a.bounce + 1, seed_lo, seed_hi, ucs);
@@ -840,15 +909,21 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
             f3 gd = normalize(fu * wi.x + fv * wi.y + nrm * wi.z);
             f3 go = p + gd * kSpawnEps;
             // Russian roulette (radiance.cu:62-68)
-            bool alive = a.bounce < max_bounce;
+            bool alive = a.bounce < max_bounce, rr_kill = false;
             float rr = smax(beta.x, smax(beta.y, beta.z));
             if (rr < 1.f && a.bounce > 3) {
                 float qq = smax(0.05f, 1.f - rr);
-                if (u01(r1.v[2]) < qq)
+                if (u01(r1.v[2]) < qq) {
                     alive = false;
-                else
+                    rr_kill = true;
+                } else {
                     beta = beta / (1.f - qq);
+                }
             }
+            // the profiler's loop counter: a path ended by Russian roulette
+            // breaks before ++bounce (radiance.cu:66), any other goes on to
+            // the next intersect, which ends the loop
+            if (psl >= 0) a.pv.fin[4 * size_t(psl) + 3] = __uint_as_float(rr_kill ? a.bounce : a.bounce + 1);
             // a ray with a NaN direction or origin (NaN frame, failed refraction
             // with no light sampled) hits nothing, in the reference's traversal
             // as in this one: end the path here instead of tracing it
@@ -910,7 +985,8 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
 template <int W, bool BATCH, bool Q>
 __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(DevScene S, JobTable T, uint64_t g0, ShadowQ sq, uint32_t *counters,
                                                    float *L, unsigned long long *stats, uint32_t *spill,
-                                                   size_t spill_lanes, NrcRender nr, uint32_t bounce) {
+                                                   size_t spill_lanes, NrcRender nr, uint32_t bounce,
+                                                   ProfView pv) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
     __shared__ float st_[kLdsDepth * kBlock];
     auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
@@ -938,6 +1014,7 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(Dev
                         rec[21] += sq.cz[i];
                     }
                 }
+                if (pv.enabled) prof_add_light(pv, g0 + slot, bounce, sq.cx[i], sq.cy[i], sq.cz[i]);
             }
         }
     }
@@ -1040,7 +1117,7 @@ template <int W, bool BATCH, bool Q>
 __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow_dyn(DevScene S, JobTable T, uint64_t g0, ShadowQ sq,
                                                        uint32_t *counters, float *L, unsigned long long *stats,
                                                        uint32_t *spill, size_t spill_lanes, NrcRender nr,
-                                                       uint32_t bounce) {
+                                                       uint32_t bounce, ProfView pv) {
     __shared__ uint32_t sref[kLdsDepth * kBlock];
     __shared__ float st_[kLdsDepth * kBlock];
     auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
@@ -1084,8 +1161,20 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow_dyn
                     rec[21] += sq.cz[i];
                 }
             }
+            if (pv.enabled) prof_add_light(pv, g0 + slot, bounce, sq.cx[i], sq.cy[i], sq.cz[i]);
         }
         if (fin) has = false;
+    }
+}
+
+// Final path radiance of the chunk's profiled paths (Slice::L, radiance.cu:74-78).
+__global__ __launch_bounds__(kBlock) void k_prof_final(ProfView pv, const float *L, uint64_t g0, uint32_t P) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < P; j += gridDim.x * blockDim.x) {
+        int64_t sl = prof_slice(pv, g0 + j);
+        if (sl < 0) continue;
+        pv.fin[4 * size_t(sl)] = L[3 * size_t(j)];
+        pv.fin[4 * size_t(sl) + 1] = L[3 * size_t(j) + 1];
+        pv.fin[4 * size_t(sl) + 2] = L[3 * size_t(j) + 2];
     }
 }
 
@@ -1920,6 +2009,7 @@ KR_EXPORT int kr_render_free(kr_render *r) {
     }
     for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
     for (void *p : r->nrc_allocs) (void)hipFree(p);
+    for (void *p : r->prof_allocs) (void)hipFree(p);
     delete r;
     return KR_OK;
 }
@@ -2044,6 +2134,10 @@ KR_EXPORT int kr_render_reset(kr_render *r) {
     KR_HIP(hipMemsetAsync(r->valid, 0, sizeof(uint32_t) * size_t(r->npix), st));
     KR_HIP(hipMemsetAsync(r->stats, 0, sizeof(unsigned long long) * S_NUM, st));
     KR_HIP(hipMemsetAsync(r->counters, 0, sizeof(uint32_t) * C_NUM * kCtrStride, st));
+    if (r->pv.enabled) {
+        KR_HIP(hipMemsetAsync(r->pv.rec, 0, sizeof(float) * kProfFloats * r->prof_slices * r->pv.depth, st));
+        KR_HIP(hipMemsetAsync(r->pv.fin, 0, sizeof(float) * 4 * r->prof_slices, st));
+    }
     KR_HIP(hipStreamSynchronize(st));
     resolve_events(r);
     for (int k = 0; k < K_KINDS; ++k) {
@@ -2117,6 +2211,9 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
     hipStream_t st = s->dev->stream;
     const DevScene &S = s->view;
     JobTable T = bv ? bv->T : JobTable{nullptr, nullptr, 0};
+    // the profiler (single renders only); the one-hit kernels record no vertices
+    ProfView pv = BATCH ? ProfView{} : r->pv;
+    if (r->prm.kernel != KR_KERNEL_RADIANCE) pv.depth = 0;
     int grid = r->grid;
     int cam_grid = int(std::min<size_t>((P + kBlock - 1) / kBlock, size_t(grid)));
     int cur = 0;
@@ -2126,7 +2223,7 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, 0xFFFFFFFFu);
         hipLaunchKernelGGL((k_camera<BATCH>), dim3(cam_grid), dim3(kBlock), 0, st, r->lens, T, g0, P, r->npix,
                            r->q[0], r->L, r->counters, C_RAY0, r->nr, uint32_t(r->prm.seed),
-                           uint32_t(r->prm.seed >> 32));
+                           uint32_t(r->prm.seed >> 32), pv);
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
     bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
@@ -2175,6 +2272,7 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
         a.seed_hi = uint32_t(r->prm.seed >> 32);
         a.nr = r->nr;
         a.T = T;
+        a.pv = pv;
         {
             Timed tm(r, K_SHADE);
             if (kern == KR_KERNEL_RADIANCE)
@@ -2186,14 +2284,18 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
             Timed tm(r, K_SHADOW);
             if (dyn)
                 hipLaunchKernelGGL((k_shadow_dyn<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq,
-                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b);
+                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b, pv);
             else
                 hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq,
-                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b);
+                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b, pv);
         }
         cur = 1 - cur;
     }
     if (!BATCH && r->nr.enabled && kern == KR_KERNEL_RADIANCE) nrc_after_chunk(r);
+    if (pv.enabled) {
+        Timed tm(r, K_OTHER);
+        hipLaunchKernelGGL(k_prof_final, dim3(cam_grid), dim3(kBlock), 0, st, pv, r->L, g0, P);
+    }
     {
         Timed tm(r, K_OTHER);
         if (!BATCH) {
@@ -2263,6 +2365,89 @@ KR_EXPORT int kr_render_read(kr_render *r, double *rgb, uint32_t *valid) {
     hipStream_t st = r->scene->dev->stream;
     if (rgb) KR_HIP(hipMemcpyAsync(rgb, r->acc, sizeof(double) * 3 * size_t(r->npix), hipMemcpyDeviceToHost, st));
     if (valid) KR_HIP(hipMemcpyAsync(valid, r->valid, sizeof(uint32_t) * r->npix, hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
+    resolve_events(r);
+    return KR_OK;
+}
+
+static_assert(sizeof(kr_slice_record) == kProfFloats * sizeof(float), "slice record layout");
+
+KR_EXPORT int kr_render_set_profiler(kr_render *r, const kr_profiler_config *cfg) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    if (cfg && r->nrc) return fail(KR_E_UNSUPPORTED, "a render with a radiance cache takes no profiler");
+    (void)hipSetDevice(r->scene->dev->ordinal);
+    KR_HIP(hipStreamSynchronize(r->scene->dev->stream));
+    for (void *p : r->prof_allocs) (void)hipFree(p);
+    r->prof_allocs.clear();
+    r->pv = ProfView{};
+    r->prof_slices = 0;
+    if (!cfg) return KR_OK;
+    // Profiler::Profiler (kernel.cu:34-35): area = max(0, min(area, (w, h, w, h)))
+    uint32_t w = r->prm.width, h = r->prm.height;
+    uint32_t x0 = std::min(cfg->area[0], w), y0 = std::min(cfg->area[1], h);
+    uint32_t x1 = std::min(cfg->area[2], w), y1 = std::min(cfg->area[3], h);
+    if (x1 < x0 || y1 < y0) return fail(KR_E_INVALID, "profiler area has x1 < x0 or y1 < y0");
+    uint64_t slices = uint64_t(x1 - x0) * (y1 - y0) * r->prm.spp;
+    ProfView pv{};
+    pv.enabled = 1;
+    pv.x0 = x0;
+    pv.y0 = y0;
+    pv.x1 = x1;
+    pv.y1 = y1;
+    pv.w = w;
+    pv.npix = r->npix;
+    pv.spp = r->prm.spp;
+    pv.depth = cfg->depth;
+    size_t nrec = std::max<size_t>(1, size_t(slices) * cfg->depth * kProfFloats);
+    size_t nfin = std::max<size_t>(1, size_t(slices) * 4);
+    int rc = dalloc(&pv.rec, nrec);
+    if (!rc) r->prof_allocs.push_back(pv.rec);
+    if (!rc) rc = dalloc(&pv.fin, nfin);
+    if (!rc) r->prof_allocs.push_back(pv.fin);
+    if (rc) {
+        for (void *p : r->prof_allocs) (void)hipFree(p);
+        r->prof_allocs.clear();
+        return rc;
+    }
+    hipStream_t st = r->scene->dev->stream;
+    KR_HIP(hipMemsetAsync(pv.rec, 0, sizeof(float) * nrec, st));
+    KR_HIP(hipMemsetAsync(pv.fin, 0, sizeof(float) * nfin, st));
+    KR_HIP(hipStreamSynchronize(st));
+    r->pv = pv;
+    r->prof_slices = slices;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_profiler_size(kr_render *r, uint64_t *num_slices, uint32_t *depth) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    if (num_slices) *num_slices = r->prof_slices;
+    if (depth) *depth = r->pv.enabled ? r->pv.depth : 0;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_render_read_slices(kr_render *r, uint64_t first, uint64_t count, float *L, uint32_t *bounce,
+                                    kr_slice_record *recs) {
+    if (!r) return fail(KR_E_INVALID, "NULL render");
+    if (!r->pv.enabled) return fail(KR_E_INVALID, "render has no profiler");
+    if (first > r->prof_slices || count > r->prof_slices - first)
+        return fail(KR_E_INVALID, "slice range exceeds the profiler's slices");
+    if (count == 0) return KR_OK;
+    hipStream_t st = r->scene->dev->stream;
+    if (L || bounce) {
+        std::vector<float> fin(4 * size_t(count));
+        KR_HIP(hipMemcpyAsync(fin.data(), r->pv.fin + 4 * size_t(first), sizeof(float) * fin.size(),
+                              hipMemcpyDeviceToHost, st));
+        KR_HIP(hipStreamSynchronize(st));
+        for (uint64_t i = 0; i < count; ++i) {
+            if (L)
+                for (int c = 0; c < 3; ++c) L[3 * i + c] = fin[4 * i + c];
+            if (bounce) std::memcpy(&bounce[i], &fin[4 * i + 3], 4);
+        }
+    }
+    if (recs && r->pv.depth) {
+        size_t per = size_t(r->pv.depth) * kProfFloats;
+        KR_HIP(hipMemcpyAsync(recs, r->pv.rec + per * first, sizeof(float) * per * count, hipMemcpyDeviceToHost, st));
+    }
     KR_HIP(hipStreamSynchronize(st));
     resolve_events(r);
     return KR_OK;
@@ -2732,6 +2917,7 @@ int nrc_after_run(kr_render *r) {
 
 KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     if (!r) return fail(KR_E_INVALID, "NULL render");
+    if (n && r->pv.enabled) return fail(KR_E_UNSUPPORTED, "a render with a profiler takes no radiance cache");
     (void)hipSetDevice(r->scene->dev->ordinal);
     (void)hipStreamSynchronize(r->scene->dev->stream);
     for (void *p : r->nrc_allocs) (void)hipFree(p);

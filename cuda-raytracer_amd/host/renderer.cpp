@@ -1,6 +1,7 @@
 // renderer.cpp -- Renderer::render and the tracer registry over the C ABI.
 #include "renderer.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <fstream>
@@ -42,6 +43,51 @@ struct DeviceScene {
     }
 };
 
+// Profiler::~Profiler (src/core/kernel/kernel.cu:53-73) writing every slice
+// with RadianceKernel::Slice::writeSlice (src/core/kernel/radiance.hpp:31-46):
+// "at [ x, y, k ]: " for every path of the area, then, for paths whose
+// radiance reaches 100 in all channels, the loop counter and the recorded
+// vertices. Vectors print as the reference's "[ a, b, c ]", floats with the
+// stream's default format.
+void write_profile(kr_render *r, const std::string &path, const uint32_t area[4], uint32_t spp) {
+    std::ofstream of(path);
+    if (!of) throw std::logic_error("cannot write '" + path + "'");
+    uint64_t n = 0;
+    uint32_t depth = 0;
+    kr_check(kr_render_profiler_size(r, &n, &depth), "kr_render_profiler_size");
+    const uint32_t aw = area[2] - area[0];
+    auto v3 = [&](const float *v) { of << "[ " << v[0] << ", " << v[1] << ", " << v[2] << " ]"; };
+    const uint64_t kRead = 1u << 16;
+    std::vector<float> L;
+    std::vector<uint32_t> bounce;
+    std::vector<kr_slice_record> recs;
+    for (uint64_t first = 0; first < n; first += kRead) {
+        uint64_t cnt = std::min(kRead, n - first);
+        L.resize(3 * cnt);
+        bounce.resize(cnt);
+        recs.resize(std::max<uint64_t>(1, cnt * depth));
+        kr_check(kr_render_read_slices(r, first, cnt, L.data(), bounce.data(), recs.data()), "kr_render_read_slices");
+        for (uint64_t i = 0; i < cnt; ++i) {
+            uint64_t sl = first + i, px = sl / spp;
+            of << "at [ " << area[0] + px % aw << ", " << area[1] + px / aw << ", " << sl % spp << " ]: " << std::endl;
+            const float *l = &L[3 * i];
+            if (!(l[0] >= 1e2f && l[1] >= 1e2f && l[2] >= 1e2f)) continue;
+            of << "bounce: " << bounce[i] << " / " << depth << std::endl;
+            for (uint32_t j = 0; j < std::min(bounce[i], depth); ++j) {
+                const kr_slice_record &b = recs[i * depth + j];
+                of << "ray: ";
+                v3(b.o);
+                v3(b.d);
+                of << std::endl << "f: ";
+                v3(b.f);
+                of << std::endl << "theta: " << b.theta << std::endl << "L: ";
+                v3(b.L);
+                of << std::endl << "shader: " << b.shader << std::endl;
+            }
+        }
+    }
+}
+
 // The wavefront integrator behind Tracer::execute: uploads the flat scene,
 // renders every pixel's samples in FP64 accumulators on the device and reads
 // back sums and valid counts (src/core/tracer/cpuMulticore.cu:47-64 semantics).
@@ -71,6 +117,21 @@ public:
         p.seed = opt.seed;
         p.kernel = ktype;
         kr_check(kr_render_create(ds.scene, &camera, &p, &ds.render), "kr_render_create");
+        // Profiler::Profiler (src/core/kernel/kernel.cu:26-51): the area
+        // clamped to the image, one slice per path of the area
+        uint32_t area[4] = {std::min(scene.profiler_area[0], image.w), std::min(scene.profiler_area[1], image.h),
+                            std::min(scene.profiler_area[2], image.w), std::min(scene.profiler_area[3], image.h)};
+        if (scene.profiler_enable) {
+            KINFO("profiler", "Enabled in", "[ " + std::to_string(area[0]) + ", " + std::to_string(area[1]) + ", " +
+                                                std::to_string(area[2]) + ", " + std::to_string(area[3]) + " ]");
+            kr_profiler_config pc{{area[0], area[1], area[2], area[3]}, scene.profiler_depth};
+            kr_check(kr_render_set_profiler(ds.render, &pc), "kr_render_set_profiler");
+            uint64_t slices = 0;
+            kr_check(kr_render_profiler_size(ds.render, &slices, nullptr), "kr_render_profiler_size");
+            KINFO("profiler", "Allocoated ", slices, "profile slices");
+        } else {
+            KINFO("profiler", "Disabled");
+        }
         KINFO("tracer", "Tracing start");
         auto t0 = std::chrono::steady_clock::now();
         kr_check(kr_render_run(ds.render, 0, spp), "kr_render_run");
@@ -84,6 +145,11 @@ public:
         kr_render_get_stats(ds.render, &st);
         KLOG(double(st.paths) / sec, "paths/s,", double(st.closest_rays + st.shadow_rays) / sec * 1e-6, "Mrays/s,",
              st.invalid_samples, "samples rejected (NaN/Inf)");
+        if (scene.profiler_enable) {
+            KINFO("profiler", "Writting profile");
+            write_profile(ds.render, scene.profiler_output, area, spp);
+            KINFO("profiler", "Written to '" + scene.profiler_output + "'");
+        }
     }
 
 private:
@@ -100,6 +166,7 @@ public:
         const std::string kernel = opt.kernel.empty() ? scene.kernel : opt.kernel;
         if (kernel_type(kernel) != KR_KERNEL_RADIANCE)
             throw std::logic_error("tracer HIPBatched renders the Radiance kernel only");
+        if (scene.profiler_enable) throw std::logic_error("tracer HIPBatched has no profiler");
         struct Handles {
             kr_device *dev = nullptr;
             kr_batch *batch = nullptr;
@@ -118,7 +185,7 @@ public:
         job.params.spp = spp;
         job.params.max_bounce = scene.max_bounce;
         job.params.seed = opt.seed;
-        kr_batch_tuning tu{opt.bvh_width ? opt.bvh_width : 4u, 32u << 20, 8, 0, 0.0};
+        kr_batch_tuning tu{opt.bvh_width ? opt.bvh_width : 4u, 32u << 20, 64, 0, 0.0};
         kr_check(kr_batch_create(h.dev, &d, 1, &job, 1, &tu, &h.batch), "kr_batch_create");
         KINFO("tracer", "Tracing start");
         auto t0 = std::chrono::steady_clock::now();

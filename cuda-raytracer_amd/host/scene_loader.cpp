@@ -330,6 +330,19 @@ LoadedScene load_scene_json(const Json &doc, const std::string &base_dir) {
         out.kernel = (*k)[0].str();
         if (k->size() > 1) out.max_bounce = uint32_t(getf((*k)[1], "maxBounce", 16.f));
     }
+    // profiler (renderer.hpp:19-20, kernel.hpp:37-43): present means enabled
+    // unless "enable" says otherwise; area is a uint4, so -1 reads as 2^32 - 1
+    if (const Json *pf = doc.find("profiler")) {
+        out.profiler_enable = true;
+        if (const Json *e = pf->find("enable")) out.profiler_enable = e->boolean();
+        if (const Json *o = pf->find("output")) out.profiler_output = o->str();
+        if (const Json *a = pf->find("area")) {
+            if (!a->is_array() || a->size() != 4) throw std::logic_error("profiler area must be [x0, y0, x1, y1]");
+            for (int c = 0; c < 4; ++c) out.profiler_area[c] = uint32_t(int64_t((*a)[c].number()));
+        }
+        if (const Json *pp = pf->find("props"))
+            if (const Json *d = pp->find("depth")) out.profiler_depth = uint32_t(d->number());
+    }
     std::vector<std::string> names;
     bool imported = false;
     if (const Json *assets = scene.find("assets")) {

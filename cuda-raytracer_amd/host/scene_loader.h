@@ -41,6 +41,13 @@ struct LoadedScene {
     std::string kernel = "Radiance";
     uint32_t max_bounce = 16;
     uint64_t light_seed = 0x4B4F49534849ull;
+    // "profiler": Renderer::Configuration's key, absent = disabled
+    // (renderer.hpp:19-20), with Profiler::Configuration's fields and
+    // defaults (kernel.hpp:37-43) and the slice's "depth" (radiance.hpp:27-30)
+    bool profiler_enable = false;
+    std::string profiler_output = "a.prof";
+    uint32_t profiler_area[4] = {0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    uint32_t profiler_depth = 8;
 
     kr_scene_desc desc(uint32_t bvh_width = 0, uint32_t leaf_size = 0) const;
 };

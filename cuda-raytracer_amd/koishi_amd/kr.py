@@ -86,6 +86,15 @@ class Hits(C.Structure):
                 ("v", C.c_void_p)]
 
 
+class ProfilerConfig(C.Structure):
+    _fields_ = [("area", u32 * 4), ("depth", u32)]
+
+
+# kr_slice_record
+SLICE_RECORD = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("f", "<f4", 3), ("L", "<f4", 3), ("theta", "<f4"),
+                         ("shader", "<u4")])
+
+
 class RenderStats(C.Structure):
     _fields_ = [("paths", u64), ("closest_rays", u64), ("shadow_rays", u64), ("invalid_samples", u64),
                 ("seconds", C.c_double), ("node_visits", u64), ("tri_tests", u64), ("trace_ms", C.c_double),
@@ -168,6 +177,9 @@ def load(path: str = LIB_PATH):
     lib.kr_render_free.argtypes = [vp]
     lib.kr_render_radiance.argtypes = [vp, C.POINTER(Camera), C.POINTER(Params), vp, vp]
     lib.kr_render_set_profiling.argtypes = [vp, u32]
+    lib.kr_render_set_profiler.argtypes = [vp, C.POINTER(ProfilerConfig)]
+    lib.kr_render_profiler_size.argtypes = [vp, C.POINTER(u64), C.POINTER(u32)]
+    lib.kr_render_read_slices.argtypes = [vp, u64, u64, vp, vp, vp]
     lib.kr_device_bandwidth.argtypes = [vp, C.c_size_t, u32, C.POINTER(C.c_double)]
     lib.kr_nrc_create.argtypes = [vp, C.POINTER(NrcConfig), C.POINTER(vp)]
     lib.kr_nrc_free.argtypes = [vp]
@@ -323,6 +335,32 @@ class Render:
 
     def set_profiling(self, flags: int):
         check(self.lib.kr_render_set_profiling(self.h, flags), "kr_render_set_profiling")
+
+    def set_profiler(self, area=None, depth=8):
+        """kr_render_set_profiler: area (x0, y0, x1, y1) or None to remove."""
+        if area is None:
+            check(self.lib.kr_render_set_profiler(self.h, None), "kr_render_set_profiler")
+            return
+        cfg = ProfilerConfig((u32 * 4)(*[min(int(a), 0xFFFFFFFF) for a in area]), depth)
+        check(self.lib.kr_render_set_profiler(self.h, C.byref(cfg)), "kr_render_set_profiler")
+
+    def profiler_size(self):
+        n, d = u64(0), u32(0)
+        check(self.lib.kr_render_profiler_size(self.h, C.byref(n), C.byref(d)), "kr_render_profiler_size")
+        return n.value, d.value
+
+    def read_slices(self, first=0, count=None):
+        """Slices [first, first + count): (L (n, 3) float32, bounce (n,) uint32,
+        records (n, depth) of SLICE_RECORD)."""
+        n, depth = self.profiler_size()
+        if count is None:
+            count = n - first
+        L = np.zeros((count, 3), np.float32)
+        bounce = np.zeros(count, np.uint32)
+        recs = np.zeros((count, depth), SLICE_RECORD)
+        check(self.lib.kr_render_read_slices(self.h, first, count, _ptr(L), _ptr(bounce), _ptr(recs)),
+              "kr_render_read_slices")
+        return L, bounce, recs
 
     def stats(self) -> RenderStats:
         s = RenderStats()

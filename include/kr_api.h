@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KR_API_VERSION 2
+#define KR_API_VERSION 3
 
 enum kr_status {
     KR_OK = 0,
@@ -307,6 +307,44 @@ int kr_render_free(kr_render *render);
 /* One-shot render of all params->spp samples into caller buffers. */
 int kr_render_radiance(kr_scene *scene, const kr_camera *camera, const kr_params *params,
                        double *rgb, uint32_t *valid);
+
+/* ---- profiler (bounce dump) -----------------------------------------------
+ * Replaces core::Profiler with RadianceKernel::Slice
+ * (src/core/kernel/kernel.hpp:35-81, src/core/kernel/kernel.cu:26-73,
+ * src/core/kernel/radiance.hpp:17-55, radiance.cu:38-50, 74-78). While a
+ * profiler is set, a render traces only the pixels x0 <= x < x1,
+ * y0 <= y < y1 of its area (CPUMulticoreTracer traces only the profiled area,
+ * src/core/tracer/cpuMulticore.cu:17); the other pixels keep zero radiance.
+ * Every path (x, y, k) of the area is a slice holding its first `depth`
+ * vertices. Slices are numbered as the reference writes them:
+ * ((y - y0) * (x1 - x0) + (x - x0)) * spp + k. */
+typedef struct kr_profiler_config {
+    uint32_t area[4]; /* x0, y0, x1, y1; clamped to [0, w] x [0, h] as Profiler's ctor does */
+    uint32_t depth;   /* vertices recorded per path (the slice's "depth" property, 8 in the reference) */
+} kr_profiler_config;
+
+/* RadianceKernel::Slice::BounceRecord, one per vertex below depth. */
+typedef struct kr_slice_record {
+    float o[3], d[3]; /* the ray that reached the vertex */
+    float f[3];       /* BSDF value of the light sample (zero when the scene has no lights) */
+    float L[3];       /* path radiance after the vertex's light sample and emission */
+    float theta;      /* acos(wo . reflect(wi, n)) in the vertex's local frame */
+    uint32_t shader;  /* shader id of the hit primitive */
+} kr_slice_record;
+
+/* Sets (config non-NULL) or removes the profiler of a Radiance / Normal /
+ * Visibility / HalfVector render; clears the slices. Not combined with a
+ * radiance cache (KR_E_UNSUPPORTED). The debug kernels record no vertices. */
+int kr_render_set_profiler(kr_render *render, const kr_profiler_config *config);
+/* Slice count (area pixels x spp) and depth of the profiler in force. */
+int kr_render_profiler_size(kr_render *render, uint64_t *num_slices, uint32_t *depth);
+/* Copies slices [first, first + count): per slice the path radiance (3
+ * floats, L), the loop counter of RadianceKernel::execute (bounce: the
+ * vertices processed, not counting one ended by Russian roulette) and `depth`
+ * records (recs, zero beyond the recorded ones). Any pointer may be NULL.
+ * Synchronises. */
+int kr_render_read_slices(kr_render *render, uint64_t first, uint64_t count, float *L, uint32_t *bounce,
+                          kr_slice_record *recs);
 
 /* ---- neural radiance cache ----------------------------------------------
  * No counterpart in the reference: an online-trained radiance cache

@@ -862,13 +862,33 @@ inline V3<R> debug_kernel(uint32_t kernel, const Scene<R> &s, Ray<R> ray, PathRn
     return V3<R>{v, v, v};
 }
 
-// RadianceKernel::execute (src/core/kernel/radiance.cu:14-81).
+// RadianceKernel::Slice (src/core/kernel/radiance.hpp:17-55): per vertex
+// below `depth`, the ray that reached it, the BSDF value of the light sample,
+// theta = acos(wo . reflect(wi, n)) in the local frame, the path radiance
+// after the vertex's NEE and emission, and the shader id; at the end, the
+// loop counter and the path radiance.
+template <class R> struct ProfRecord {
+    Ray<R> ray;
+    V3<R> f, L;
+    R theta;
+    uint32_t shader;
+};
+template <class R> struct ProfSlice {
+    std::vector<ProfRecord<R>> bounces;  // size = depth
+    uint32_t bounce = 0;
+    V3<R> L{0, 0, 0};
+};
+
+// RadianceKernel::execute (src/core/kernel/radiance.cu:14-81), recording
+// into `prof` when given (radiance.cu:38-50, 74-78).
 template <class R>
-inline V3<R> radiance(const Scene<R> &s, Ray<R> ray, PathRng &rng, uint32_t max_bounce, PathStats *st) {
+inline V3<R> radiance(const Scene<R> &s, Ray<R> ray, PathRng &rng, uint32_t max_bounce, PathStats *st,
+                      ProfSlice<R> *prof = nullptr) {
     V3<R> L{0, 0, 0}, beta{1, 1, 1};
     Vary<R> vy;
     uint32_t nL = uint32_t(s.lights.size());
-    for (uint32_t bounce = 0;; ++bounce) {
+    uint32_t bounce = 0;
+    for (;; ++bounce) {
         if (st) st->closest++;
         if (!(scene_intersect(s, ray, vy, st ? &st->trav : nullptr) && bounce <= max_bounce)) break;
         int b = int(bounce);
@@ -886,6 +906,15 @@ inline V3<R> radiance(const Scene<R> &s, Ray<R> ray, PathRng &rng, uint32_t max_
             L += beta * vy.f * li * std::fabs(dot(vy.wi, V3<R>{0, 0, 1})) / lpdf;
         }
         L += beta * vy.emission;
+        if (prof && bounce < prof->bounces.size()) {
+            // reflect(wi, n) = 2 (n . wi) n - wi with n = (0, 0, 1) (vmath.hpp:470)
+            ProfRecord<R> &b = prof->bounces[bounce];
+            b.ray = ray;
+            b.theta = std::acos(dot(vy.wo, V3<R>{-vy.wi.x, -vy.wi.y, vy.wi.z}));
+            b.f = vy.f;
+            b.L = L;
+            b.shader = vy.shader;
+        }
         int dim = 12;
         const kr_shader_desc &sh = s.shaders[resolve_shader(s.shaders, vy.shader, vy, rng, b, dim)];
         R uc = sh.type == KR_SHADER_GLASS ? R(rng.get(b, dim++)) : R(0);
@@ -899,6 +928,10 @@ inline V3<R> radiance(const Scene<R> &s, Ray<R> ray, PathRng &rng, uint32_t max_
             beta /= R(1) - q;
         }
         vy = Vary<R>();
+    }
+    if (prof) {
+        prof->bounce = bounce;
+        prof->L = L;
     }
     return L;
 }

@@ -114,6 +114,31 @@ void OR_FN(oracle_path_radiance)(void *sp, const double *o, const double *d, uin
     put(L, radiance(*s, Ray<R>{dv(o), dv(d)}, rng, max_bounce, nullptr));
 }
 
+// The profile slice of path (x, y, k) of a frame (Radiance kernel): rec
+// receives `depth` records of 14 values (ray o, d, f, L, theta, shader),
+// zero beyond the recorded ones; L the path radiance; *bounce the loop count.
+void OR_FN(oracle_profile_path)(void *sp, const kr_camera *cam, const kr_params *prm, uint32_t x, uint32_t y,
+                                uint32_t k, uint32_t depth, double *rec, double *L, uint32_t *bounce) {
+    auto *s = static_cast<Scene<R> *>(sp);
+    Lens<R> lens(*cam, prm->width, prm->height, prm->spp);
+    PathRng rng(y * prm->width + x, k, prm->seed);
+    ProfSlice<R> sl;
+    sl.bounces.assign(depth, ProfRecord<R>{Ray<R>{{0, 0, 0}, {0, 0, 0}}, {0, 0, 0}, {0, 0, 0}, R(0), 0u});
+    radiance(*s, lens.sample(x, y, k), rng, prm->max_bounce, nullptr, &sl);
+    for (uint32_t i = 0; i < depth; ++i) {
+        const ProfRecord<R> &b = sl.bounces[i];
+        double *o = rec + 14 * size_t(i);
+        put(o, b.ray.o);
+        put(o + 3, b.ray.d);
+        put(o + 6, b.f);
+        put(o + 9, b.L);
+        o[12] = double(b.theta);
+        o[13] = double(b.shader);
+    }
+    put(L, sl.L);
+    *bounce = sl.bounce;
+}
+
 int OR_FN(oracle_light_samples)(void *sp, uint32_t light, double *p_n, double *pdf) {
     auto *s = static_cast<Scene<R> *>(sp);
     if (light >= s->lights.size()) return KR_E_INVALID;

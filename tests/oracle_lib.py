@@ -46,6 +46,8 @@ def oracle():
                                            C.c_int, VP, C.c_uint32, VP, VP, VP]
             g("oracle_render").restype = D
             g("oracle_path_radiance").argtypes = [VP, PD, PD, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint32, PD]
+            g("oracle_profile_path").argtypes = [VP, C.POINTER(kr.Camera), C.POINTER(kr.Params), C.c_uint32,
+                                                 C.c_uint32, C.c_uint32, C.c_uint32, VP, VP, C.POINTER(C.c_uint32)]
             g("oracle_light_samples").argtypes = [VP, C.c_uint32, VP, PD]
             g("oracle_lens_ray").argtypes = [C.POINTER(kr.Camera), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                              C.c_uint32, C.c_uint32, PD, PD]
@@ -179,6 +181,17 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("not a mesh light")
         return pn, pdf.value
+
+    def profile_path(self, params, x, y, k, depth=8):
+        """The profiler slice of path (x, y, k): records (depth, 14) as
+        (o, d, f, L, theta, shader), the path radiance and the loop counter."""
+        cam = self.scene.camera_struct()
+        rec = np.zeros((depth, 14))
+        L = np.zeros(3)
+        b = C.c_uint32(0)
+        fn("oracle_profile_path", self.real)(self.h, C.byref(cam), C.byref(params), x, y, k, depth, _p(rec), _p(L),
+                                             C.byref(b))
+        return rec, L, b.value
 
     def free(self):
         if self.h:

@@ -82,3 +82,63 @@ def test_cr_tracers_agree_with_the_c_abi(tmp_path, device):
     ls.free()
     assert np.array_equal(val, raws["HIPWavefront"][3])
     assert np.array_equal(rgb / np.maximum(val, 1)[:, None], raws["HIPWavefront"][2])
+
+
+def test_cr_writes_the_profile(tmp_path, device):
+    """A scene file with a "profiler" key: `cr` traces only the area and
+    writes the reference's profile text (Profiler::~Profiler with
+    RadianceKernel::Slice::writeSlice) for every slice of the area."""
+    import json
+
+    s = sg.room_scene(width=24, height=18, spp=2, freq=8)
+    s.shaders["light"] = ["Emission", {"color": [1.0, 1.0, 1.0], "strength": 150.0}]
+    path = s.write(str(tmp_path), "room")
+    doc = json.load(open(path))
+    prof = str(tmp_path / "room.prof")
+    area, depth = [2, 1, 22, 17], 5
+    doc["profiler"] = {"output": prof, "area": area, "props": {"depth": depth}}
+    json.dump(doc, open(path, "w"))
+    w, h, spp = 24, 18, 2
+    out_raw = str(tmp_path / "room.krraw")
+    res = subprocess.run([host.CR_PATH, path, "-o", str(tmp_path / "room.png"), "-s", str(spp),
+                          f"--resolution={w}x{h}", "--seed", "5", "--dump-raw", out_raw],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "Enabled in [ 2, 1, 22, 17 ]" in res.stdout and "Written to '" + prof + "'" in res.stdout
+    _, _, mean, _ = _read_raw(out_raw)
+    inside = np.zeros((h, w), bool)
+    inside[1:17, 2:22] = True
+    assert not mean[~inside.ravel()].any()
+    # the same slices through the C ABI
+    ls = host.LoadedScene(path)
+    gs = kr.GpuScene(device, ls.to_desc())
+    r = kr.Render(gs, ls.camera(0), kr.Params(w, h, spp, ls.max_bounce, 5, 0, 0))
+    r.set_profiler(area, depth)
+    r.run(0, spp)
+    L, bounce, recs = r.read_slices()
+    r.free()
+    gs.free()
+    ls.free()
+    lines = open(prof).read().split("\n")
+    assert lines[-1] == ""
+    i, s_idx, fireflies = 0, 0, 0
+    aw = area[2] - area[0]
+    while i < len(lines) - 1:
+        px, k = s_idx // spp, s_idx % spp
+        assert lines[i] == f"at [ {area[0] + px % aw}, {area[1] + px // aw}, {k} ]: ", lines[i]
+        i += 1
+        if np.all(L[s_idx] >= 100.0):
+            fireflies += 1
+            assert lines[i] == f"bounce: {bounce[s_idx]} / {depth}"
+            i += 1
+            for j in range(min(int(bounce[s_idx]), depth)):
+                rec = recs[s_idx][j]
+                assert lines[i].startswith("ray: [ ") and lines[i + 1].startswith("f: [ ")
+                assert lines[i + 2].startswith("theta: ") and lines[i + 3].startswith("L: [ ")
+                assert lines[i + 4] == f"shader: {rec['shader']}"
+                got = [float(x) for x in lines[i + 3][len("L: [ "):-2].split(", ")]
+                assert np.allclose(got, rec["L"], rtol=1e-5)
+                i += 5
+        s_idx += 1
+    assert s_idx == (area[2] - area[0]) * (area[3] - area[1]) * spp
+    assert fireflies > 0
