@@ -217,6 +217,19 @@ __device__ __forceinline__ float box_enter(const RayBox &rb, float lx, float hx,
     return tn <= tf ? tn : __builtin_inff();
 }
 
+// The same test with the entering and leaving planes of every axis already
+// known (the ray's direction signs pick them: see load_node_nf); identical
+// results without the per-axis min / max.
+__device__ __forceinline__ float box_enter_nf(const RayBox &rb, float nx, float fx, float ny, float fy, float nz,
+                                              float fz, float tmax) {
+    float tn = fmaxf(fmaxf(fmaf(nx, rb.inv.x, -rb.oinv.x), fmaf(ny, rb.inv.y, -rb.oinv.y)),
+                     fmaxf(fmaf(nz, rb.inv.z, -rb.oinv.z), 0.f));
+    float tf = fminf(fminf(fmaf(fx, rb.inv.x, -rb.oinv.x), fmaf(fy, rb.inv.y, -rb.oinv.y)),
+                     fminf(fmaf(fz, rb.inv.z, -rb.oinv.z), tmax));
+    tf *= 1.00000024f;
+    return tn <= tf ? tn : __builtin_inff();
+}
+
 struct HitRec {
     float t, u, v;
     uint32_t code;  // global triangle index, kSphereBit|sphere, or kMissCode
@@ -286,6 +299,53 @@ __device__ __forceinline__ void load_qnode(const float4 *nodes, uint32_t idx, No
     for (int c = 0; c < W; ++c) nv.ref[c] = u[4 + c];
 }
 
+// FP32 node with each axis' planes in the ray's order: nv.lo = the plane the
+// ray enters through, nv.hi = the one it leaves through. Along an axis with a
+// negative inverse direction (sign bit set, as -1e30 for d = -0) that is hi
+// before lo, so the lane loads the other array of the pair (sel = its byte
+// distance, W * 4 or 0) instead of loading both and ordering them with
+// min / max per child. The selectors are derived from the sign bits per node
+// (the empty asm keeps the compiler from holding them in registers across
+// the traversal loop, where they would spill).
+template <int W>
+__device__ __forceinline__ void load_node_nf(const float4 *nodes, uint32_t idx, const RayBox &rb, NodeView<W> &nv) {
+    const char *p = reinterpret_cast<const char *>(nodes) + size_t(idx) * node_floats(W) * 4;
+    constexpr int AB = W * 4;  // bytes of one plane array
+    uint32_t bits[3] = {__float_as_uint(rb.inv.x), __float_as_uint(rb.inv.y), __float_as_uint(rb.inv.z)};
+    asm volatile("" : "+v"(bits[0]), "+v"(bits[1]), "+v"(bits[2]));
+    uint32_t sel[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) sel[a] = (bits[a] >> 31) * uint32_t(AB);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float *pn = reinterpret_cast<const float *>(p + 2 * a * AB + sel[a]);
+        const float *pf = reinterpret_cast<const float *>(p + (2 * a + 1) * AB - sel[a]);
+        if (W == 2) {
+            float2 n = *reinterpret_cast<const float2 *>(pn), f = *reinterpret_cast<const float2 *>(pf);
+            nv.lo[a][0] = n.x;
+            nv.lo[a][1] = n.y;
+            nv.hi[a][0] = f.x;
+            nv.hi[a][1] = f.y;
+        } else {
+#pragma unroll
+            for (int q = 0; q < W / 4; ++q) {
+                float4 n = reinterpret_cast<const float4 *>(pn)[q], f = reinterpret_cast<const float4 *>(pf)[q];
+                nv.lo[a][4 * q] = n.x;
+                nv.lo[a][4 * q + 1] = n.y;
+                nv.lo[a][4 * q + 2] = n.z;
+                nv.lo[a][4 * q + 3] = n.w;
+                nv.hi[a][4 * q] = f.x;
+                nv.hi[a][4 * q + 1] = f.y;
+                nv.hi[a][4 * q + 2] = f.z;
+                nv.hi[a][4 * q + 3] = f.w;
+            }
+        }
+    }
+    const uint32_t *r = reinterpret_cast<const uint32_t *>(p + 6 * AB);
+#pragma unroll
+    for (int c = 0; c < W; ++c) nv.ref[c] = r[c];
+}
+
 template <int W, bool Q>
 __device__ __forceinline__ void fetch_node(const float4 *nodes, uint32_t idx, NodeView<W> &nv) {
     if (Q)
@@ -326,14 +386,19 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
     while (true) {
         if (!(cur & kLeafBit)) {
             NodeView<W> nv;
-            fetch_node<W, Q>(S.nodes, cur, nv);
+            if (Q)
+                fetch_node<W, Q>(S.nodes, cur, nv);
+            else
+                load_node_nf<W>(S.nodes, cur, rb, nv);
             ++nvis;
             float key[W];
             uint32_t ref[W];
 #pragma unroll
             for (int c = 0; c < W; ++c) {
-                float tn = box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
-                                     h.t);
+                float tn = Q ? box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                         nv.hi[2][c], h.t)
+                             : box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                            nv.hi[2][c], h.t);
                 bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
                 key[c] = ok ? tn : __builtin_inff();
                 ref[c] = ok ? nv.ref[c] : kEmpty;
@@ -429,12 +494,17 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
     while (true) {
         while (cur != kEmpty && !(cur & kLeafBit)) {
             NodeView<W> nv;
-            fetch_node<W, Q>(S.nodes, cur, nv);
+            if (Q)
+                fetch_node<W, Q>(S.nodes, cur, nv);
+            else
+                load_node_nf<W>(S.nodes, cur, rb, nv);
             uint32_t next = kEmpty;
 #pragma unroll
             for (int c = 0; c < W; ++c) {
-                float tn = box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
-                                     limit);
+                float tn = Q ? box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                         nv.hi[2][c], limit)
+                             : box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                            nv.hi[2][c], limit);
                 if (nv.ref[c] != kEmpty && tn != __builtin_inff()) {
                     if (next == kEmpty)
                         next = nv.ref[c];
