@@ -309,17 +309,20 @@ __device__ __forceinline__ void load_qnode(const float4 *nodes, uint32_t idx, No
 // the traversal loop, where they would spill).
 template <int W>
 __device__ __forceinline__ void load_node_nf(const float4 *nodes, uint32_t idx, const RayBox &rb, NodeView<W> &nv) {
-    const char *p = reinterpret_cast<const char *>(nodes) + size_t(idx) * node_floats(W) * 4;
-    constexpr int AB = W * 4;  // bytes of one plane array
+    // 32-bit byte offsets from the array base (node arrays stay far below
+    // 4 GB), so the loads address from a scalar base plus one VGPR
+    const char *base = reinterpret_cast<const char *>(nodes);
+    const uint32_t nb = idx * uint32_t(node_floats(W) * 4);
+    constexpr uint32_t AB = W * 4;  // bytes of one plane array
     uint32_t bits[3] = {__float_as_uint(rb.inv.x), __float_as_uint(rb.inv.y), __float_as_uint(rb.inv.z)};
     asm volatile("" : "+v"(bits[0]), "+v"(bits[1]), "+v"(bits[2]));
     uint32_t sel[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) sel[a] = (bits[a] >> 31) * uint32_t(AB);
+    for (int a = 0; a < 3; ++a) sel[a] = (bits[a] >> 31) * AB;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const float *pn = reinterpret_cast<const float *>(p + 2 * a * AB + sel[a]);
-        const float *pf = reinterpret_cast<const float *>(p + (2 * a + 1) * AB - sel[a]);
+        const float *pn = reinterpret_cast<const float *>(base + (nb + 2 * a * AB + sel[a]));
+        const float *pf = reinterpret_cast<const float *>(base + (nb + (2 * a + 1) * AB - sel[a]));
         if (W == 2) {
             float2 n = *reinterpret_cast<const float2 *>(pn), f = *reinterpret_cast<const float2 *>(pf);
             nv.lo[a][0] = n.x;
@@ -341,7 +344,7 @@ __device__ __forceinline__ void load_node_nf(const float4 *nodes, uint32_t idx, 
             }
         }
     }
-    const uint32_t *r = reinterpret_cast<const uint32_t *>(p + 6 * AB);
+    const uint32_t *r = reinterpret_cast<const uint32_t *>(base + (nb + 6 * AB));
 #pragma unroll
     for (int c = 0; c < W; ++c) nv.ref[c] = r[c];
 }

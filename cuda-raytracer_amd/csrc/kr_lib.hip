@@ -51,22 +51,24 @@ int fail(int code, const std::string &msg) {
 constexpr int kBlock = 256;
 // Traversal stack entries per lane kept in LDS, and the waves per SIMD the
 // traversal kernels and the shading kernel ask the register allocator for
-// (1: its own choice). At 6 waves the closest-hit kernel fits 80 VGPRs
-// without spilling (its own choice, 82, allows 5); with a 10-entry LDS stack
-// (20 KB per block) that measured 5 % faster on the 100k-triangle frame than
-// 12 entries at 5 waves, and equal on the 1M-triangle frame.
+// (1: its own choice). At 8 waves per SIMD the traversal kernels fit 64
+// VGPRs with a few spilled; with a 10-entry LDS stack (20 KB per block, 8
+// blocks = the CU's 160 KB) that measured 3 % faster on the 100k-triangle
+// frame and 3 % on the 1M-triangle frame than 6 waves (70 VGPRs, no spill);
+// 7 waves does not compile (register allocation fails).
 #ifndef KR_LDS_DEPTH
 #define KR_LDS_DEPTH 10
 #endif
 #ifndef KR_TRACE_WAVES
-#define KR_TRACE_WAVES 6
+#define KR_TRACE_WAVES 8
 #endif
 #ifndef KR_SHADE_WAVES
 #define KR_SHADE_WAVES 1
 #endif
 constexpr int kLdsDepth = KR_LDS_DEPTH;
-// The 6-wave hint applies to the plain BVH2 / BVH4 traversals; the BVH8,
-// batch and compressed-node variants need more registers and would spill.
+// The wave hint applies to the plain BVH2 / BVH4 traversals; the BVH8,
+// batch, compressed-node and persistent variants need more registers and
+// would spill.
 constexpr int trace_waves(int w, bool batch, bool q) { return (w <= 4 && !batch && !q) ? KR_TRACE_WAVES : 1; }
 // traversal stack entries per lane beyond the LDS part, in HBM (108 in all:
 // BVH8 trees to depth 14, BVH4 to depth 34)
@@ -585,7 +587,7 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace(DevS
 constexpr int kMinActive = 32;
 
 template <int W, bool COUNT, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace_dyn(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
+__global__ __launch_bounds__(kBlock) void k_trace_dyn(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
                                                       uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
                                                       unsigned long long *stats, int sort, uint32_t *spill,
                                                       size_t spill_lanes) {
@@ -1114,7 +1116,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_debug(ShadeArgs a, uint32_t ke
 // Persistent shadow traversal (lane refill as in k_trace_dyn); same
 // additions to L and to the training records as k_shadow.
 template <int W, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow_dyn(DevScene S, JobTable T, uint64_t g0, ShadowQ sq,
+__global__ __launch_bounds__(kBlock) void k_shadow_dyn(DevScene S, JobTable T, uint64_t g0, ShadowQ sq,
                                                        uint32_t *counters, float *L, unsigned long long *stats,
                                                        uint32_t *spill, size_t spill_lanes, NrcRender nr,
                                                        uint32_t bounce, ProfView pv) {
