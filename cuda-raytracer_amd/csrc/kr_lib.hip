@@ -241,6 +241,28 @@ struct kr_nrc {
     std::vector<void *> allocs;
 };
 
+// One set of chunk buffers and the stream its kernels run on. A render has
+// two pipes when its frames are large: consecutive chunks alternate between
+// them, so the short late bounces of one chunk run beside the long early
+// bounces of the next instead of leaving the GPU half idle; only the
+// accumulations into the shared image are ordered between the pipes.
+struct Pipe {
+    RayQ q[2]{};
+    HitQ hit{};
+    uint32_t *bucket[3]{};
+    ShadowQ sq{};
+    float *L = nullptr;  // 3 * P
+    uint32_t *counters = nullptr;
+    uint32_t *spill = nullptr;
+    size_t spill_lanes = 0;
+    hipStream_t stream = nullptr;  // pipe 0: the device's stream
+    bool own_stream = false;
+    hipEvent_t acc_done = nullptr;  // recorded after each of the pipe's accumulations
+    NrcRender nr{};                 // radiance-cache buffers of the pipe's chunks
+    double *acc = nullptr;          // the image the pipe accumulates into: the render's
+    uint32_t *valid = nullptr;      // (shared, ordered by acc_done) or, in a batch, its own
+};
+
 struct kr_render {
     kr_scene *scene = nullptr;
     kr_camera cam{};
@@ -248,17 +270,12 @@ struct kr_render {
     LensDev lens{};
     uint32_t P = 0;  // paths per chunk
     uint32_t npix = 0;
-    RayQ q[2]{};
-    HitQ hit{};
-    uint32_t *bucket[3]{};
-    ShadowQ sq{};
-    float *L = nullptr;  // 3 * P
-    uint32_t *counters = nullptr;
+    Pipe pipe[2];
+    int npipes = 1;
+    hipEvent_t fork = nullptr;  // pipe 1 starts after what pipe 0 has queued
     unsigned long long *stats = nullptr;
     double *acc = nullptr;
     uint32_t *valid = nullptr;
-    uint32_t *spill = nullptr;
-    size_t spill_lanes = 0;
     int grid = 0;
     std::vector<void *> allocs;
     double last_seconds = 0;
@@ -268,6 +285,7 @@ struct kr_render {
     std::vector<hipEvent_t> event_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
     double kind_ms[K_KINDS] = {0, 0, 0, 0};
+    double kind_busy_ms[K_KINDS] = {0, 0, 0, 0};
     uint64_t kind_launches[K_KINDS] = {0, 0, 0, 0};
     // attached radiance cache
     kr_nrc *nrc = nullptr;
@@ -281,7 +299,7 @@ struct kr_render {
 
 namespace {
 void resolve_events(kr_render *r);
-void nrc_after_chunk(kr_render *r);
+void nrc_after_chunk(kr_render *r, Pipe &pp);
 int nrc_after_run(kr_render *r);
 }  // namespace
 
@@ -1578,6 +1596,9 @@ int upload(kr_scene *s, const T *host, size_t n, const T **dst) {
 #endif
 size_t grid_lanes(const kr_device *dev) { return size_t(std::max(dev->num_cu, 1)) * KR_BLOCKS_PER_CU * kBlock; }
 
+// Renders of at least this many paths get a second pipe (struct Pipe).
+constexpr uint64_t kPipeMinPaths = 8u << 20;
+
 int ensure_spill(uint32_t *&spill, size_t &lanes, std::vector<void *> &allocs, const kr_device *dev,
                  size_t want_lanes = 0) {
     if (spill) return KR_OK;
@@ -2004,6 +2025,8 @@ KR_EXPORT int kr_render_free(kr_render *r) {
     if (!r) return KR_OK;
     (void)hipSetDevice(r->scene->dev->ordinal);
     (void)hipStreamSynchronize(r->scene->dev->stream);
+    for (Pipe &pp : r->pipe)
+        if (pp.own_stream) (void)hipStreamSynchronize(pp.stream);
     for (void *p : r->allocs) (void)hipFree(p);
     for (auto &pe : r->pending) {
         (void)hipEventDestroy(pe.second.first);
@@ -2012,6 +2035,11 @@ KR_EXPORT int kr_render_free(kr_render *r) {
     for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
     for (void *p : r->nrc_allocs) (void)hipFree(p);
     for (void *p : r->prof_allocs) (void)hipFree(p);
+    for (Pipe &pp : r->pipe) {
+        if (pp.acc_done) (void)hipEventDestroy(pp.acc_done);
+        if (pp.own_stream) (void)hipStreamDestroy(pp.stream);
+    }
+    if (r->fork) (void)hipEventDestroy(r->fork);
     delete r;
     return KR_OK;
 }
@@ -2054,8 +2082,10 @@ int check_view(const kr_camera *cam, const kr_params *prm) {
     return KR_OK;
 }
 
-// Queues for r->P paths, accumulators for r->npix pixels, counters, the
-// traversal spill stack for `lanes` lanes; r->grid = lanes / kBlock.
+// Queues for r->P paths per pipe (r->npipes of them), accumulators for
+// r->npix pixels, counters, the traversal spill stack for `lanes` lanes per
+// pipe; r->grid = lanes / kBlock. Pipe 0 runs on the device's stream, pipe 1
+// on a stream of its own (default priority).
 int render_buffers(kr_render *r, size_t lanes) {
     size_t P = r->P;
     auto get = [&](auto **p, size_t cnt) {
@@ -2064,33 +2094,55 @@ int render_buffers(kr_render *r, size_t lanes) {
         return rc;
     };
     int rc = KR_OK;
-    for (int k = 0; k < 2 && !rc; ++k) {
-        RayQ &q = r->q[k];
-        rc = get(&q.slot, P);
-        float **fs[9] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.bx, &q.by, &q.bz};
-        for (auto f : fs)
-            if (!rc) rc = get(f, P);
-        if (!rc) rc = get(&q.flags, P);
+    for (int pi = 0; pi < r->npipes && !rc; ++pi) {
+        Pipe &pp = r->pipe[pi];
+        for (int k = 0; k < 2 && !rc; ++k) {
+            RayQ &q = pp.q[k];
+            rc = get(&q.slot, P);
+            float **fs[9] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.bx, &q.by, &q.bz};
+            for (auto f : fs)
+                if (!rc) rc = get(f, P);
+            if (!rc) rc = get(&q.flags, P);
+        }
+        if (!rc) rc = get(&pp.hit.t, P);
+        if (!rc) rc = get(&pp.hit.u, P);
+        if (!rc) rc = get(&pp.hit.v, P);
+        if (!rc) rc = get(&pp.hit.code, P);
+        for (int m = 0; m < 3 && !rc; ++m) rc = get(&pp.bucket[m], P);
+        if (!rc) rc = get(&pp.sq.slot, P);
+        {
+            float **fs[10] = {&pp.sq.ox, &pp.sq.oy, &pp.sq.oz, &pp.sq.dx, &pp.sq.dy,
+                              &pp.sq.dz, &pp.sq.tmax, &pp.sq.cx, &pp.sq.cy, &pp.sq.cz};
+            for (auto f : fs)
+                if (!rc) rc = get(f, P);
+        }
+        if (!rc) rc = get(&pp.L, 3 * P);
+        if (!rc) rc = get(&pp.counters, size_t(C_NUM) * kCtrStride);
+        if (!rc) rc = ensure_spill(pp.spill, pp.spill_lanes, r->allocs, r->scene->dev, lanes);
+        if (!rc) {
+            if (pi == 0) {
+                pp.stream = r->scene->dev->stream;
+            } else {
+                hipError_t e = hipStreamCreateWithFlags(&pp.stream, hipStreamNonBlocking);
+                if (e != hipSuccess) return fail(KR_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+                pp.own_stream = true;
+            }
+            hipError_t e = hipEventCreateWithFlags(&pp.acc_done, hipEventDisableTiming);
+            if (e != hipSuccess) return fail(KR_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
+        }
     }
-    if (!rc) rc = get(&r->hit.t, P);
-    if (!rc) rc = get(&r->hit.u, P);
-    if (!rc) rc = get(&r->hit.v, P);
-    if (!rc) rc = get(&r->hit.code, P);
-    for (int m = 0; m < 3 && !rc; ++m) rc = get(&r->bucket[m], P);
-    if (!rc) rc = get(&r->sq.slot, P);
-    {
-        float **fs[10] = {&r->sq.ox, &r->sq.oy, &r->sq.oz, &r->sq.dx, &r->sq.dy,
-                          &r->sq.dz, &r->sq.tmax, &r->sq.cx, &r->sq.cy, &r->sq.cz};
-        for (auto f : fs)
-            if (!rc) rc = get(f, P);
+    if (!rc && !r->fork) {
+        hipError_t e = hipEventCreateWithFlags(&r->fork, hipEventDisableTiming);
+        if (e != hipSuccess) return fail(KR_E_HIP, std::string("hipEventCreate: ") + hipGetErrorString(e));
     }
-    if (!rc) rc = get(&r->L, 3 * P);
-    if (!rc) rc = get(&r->counters, size_t(C_NUM) * kCtrStride);
     if (!rc) rc = get(&r->stats, S_NUM);
     if (!rc) rc = get(&r->acc, 3 * size_t(r->npix));
     if (!rc) rc = get(&r->valid, r->npix);
-    if (!rc) rc = ensure_spill(r->spill, r->spill_lanes, r->allocs, r->scene->dev, lanes);
-    if (!rc) r->grid = int(r->spill_lanes / kBlock);
+    for (int pi = 0; pi < r->npipes && !rc; ++pi) {
+        r->pipe[pi].acc = r->acc;
+        r->pipe[pi].valid = r->valid;
+    }
+    if (!rc) r->grid = int(r->pipe[0].spill_lanes / kBlock);
     return rc;
 }
 }  // namespace
@@ -2115,6 +2167,10 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
     P = std::min<uint64_t>(P, total);
     P = std::max<uint64_t>(P, 1);
     r->P = uint32_t(P);
+    // a second pipe once a frame is large enough to split into two chunks
+    // that each fill the GPU
+    r->npipes = total >= kPipeMinPaths ? 2 : 1;
+    if (r->npipes == 2 && !prm->chunk_paths) r->P = uint32_t(std::min<uint64_t>(r->P, (total + 1) / 2));
     rc = render_buffers(r, 0);
     if (rc) {
         kr_render_free(r);
@@ -2135,7 +2191,14 @@ KR_EXPORT int kr_render_reset(kr_render *r) {
     KR_HIP(hipMemsetAsync(r->acc, 0, sizeof(double) * 3 * size_t(r->npix), st));
     KR_HIP(hipMemsetAsync(r->valid, 0, sizeof(uint32_t) * size_t(r->npix), st));
     KR_HIP(hipMemsetAsync(r->stats, 0, sizeof(unsigned long long) * S_NUM, st));
-    KR_HIP(hipMemsetAsync(r->counters, 0, sizeof(uint32_t) * C_NUM * kCtrStride, st));
+    for (int pi = 0; pi < r->npipes; ++pi) {
+        const Pipe &pp = r->pipe[pi];
+        KR_HIP(hipMemsetAsync(pp.counters, 0, sizeof(uint32_t) * C_NUM * kCtrStride, st));
+        if (pp.acc != r->acc) {
+            KR_HIP(hipMemsetAsync(pp.acc, 0, sizeof(double) * 3 * size_t(r->npix), st));
+            KR_HIP(hipMemsetAsync(pp.valid, 0, sizeof(uint32_t) * size_t(r->npix), st));
+        }
+    }
     if (r->pv.enabled) {
         KR_HIP(hipMemsetAsync(r->pv.rec, 0, sizeof(float) * kProfFloats * r->prof_slices * r->pv.depth, st));
         KR_HIP(hipMemsetAsync(r->pv.fin, 0, sizeof(float) * 4 * r->prof_slices, st));
@@ -2144,6 +2207,7 @@ KR_EXPORT int kr_render_reset(kr_render *r) {
     resolve_events(r);
     for (int k = 0; k < K_KINDS; ++k) {
         r->kind_ms[k] = 0;
+        r->kind_busy_ms[k] = 0;
         r->kind_launches[k] = 0;
     }
     r->paths = 0;
@@ -2156,12 +2220,13 @@ namespace {
 struct Timed {
     kr_render *r;
     int kind;
+    hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
-    Timed(kr_render *r_, int k) : r(r_), kind(k) {
+    Timed(kr_render *r_, int k, hipStream_t s = nullptr) : r(r_), kind(k), st(s ? s : r_->scene->dev->stream) {
         if (!(r->profile & KR_PROFILE_EVENTS)) return;
         a = take();
         b = take();
-        if (a && b) (void)hipEventRecord(a, r->scene->dev->stream);
+        if (a && b) (void)hipEventRecord(a, st);
     }
     hipEvent_t take() {
         hipEvent_t e = nullptr;
@@ -2175,18 +2240,46 @@ struct Timed {
     }
     ~Timed() {
         if (!a || !b) return;
-        (void)hipEventRecord(b, r->scene->dev->stream);
+        (void)hipEventRecord(b, st);
         r->pending.push_back({kind, {a, b}});
     }
 };
 
+// Adds the resolved launches to the per-kind sums. With two pipes, launches
+// of one kind overlap in time, so besides the summed durations the union of
+// their intervals (measured from the first pending event) is kept: the time
+// during which at least one launch of the kind ran. Every pending event has
+// completed when this runs (after a synchronisation of both pipes), and later
+// events are later in time, so unions of successive calls add up.
 void resolve_events(kr_render *r) {
+    std::vector<std::pair<float, float>> iv[K_KINDS];
+    hipEvent_t ref = r->pending.empty() ? nullptr : r->pending.front().second.first;
     for (auto &pe : r->pending) {
-        float ms = 0.f;
+        float ms = 0.f, t0 = 0.f, t1 = 0.f;
         if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
             r->kind_ms[pe.first] += ms;
             r->kind_launches[pe.first] += 1;
+            if (hipEventElapsedTime(&t0, ref, pe.second.first) == hipSuccess &&
+                hipEventElapsedTime(&t1, ref, pe.second.second) == hipSuccess)
+                iv[pe.first].push_back({t0, t1});
         }
+    }
+    for (int k = 0; k < K_KINDS; ++k) {
+        std::sort(iv[k].begin(), iv[k].end());
+        double busy = 0.0, lo = 0.0, hi = -1.0;
+        for (auto &x : iv[k]) {
+            if (x.first > hi) {
+                if (hi > lo) busy += hi - lo;
+                lo = x.first;
+                hi = x.second;
+            } else {
+                hi = std::max<double>(hi, x.second);
+            }
+        }
+        if (hi > lo) busy += hi - lo;
+        r->kind_busy_ms[k] += busy;
+    }
+    for (auto &pe : r->pending) {
         r->event_pool.push_back(pe.second.first);
         r->event_pool.push_back(pe.second.second);
     }
@@ -2203,49 +2296,69 @@ struct BatchView {
     const std::vector<HostJob> *jobs;
 };
 
-// Enqueues one chunk of P paths starting at global path g0: camera, then
-// maxBounce + 1 rounds of trace / shade / shadow, then the FP64 accumulation.
-// Only kernel launches on the device stream, so the sequence can be captured
-// into a HIP graph. bv != nullptr: a batch (BATCH = true).
+// A batch chunk that lies inside one job, seen as a chunk of that job's own
+// render: its scene, lens, pixel count, depth, seed and accumulator slice.
+struct JobChunk {
+    DevScene S;
+    LensDev lens;
+    uint32_t npix, max_bounce;
+    uint64_t seed;
+    double *acc;
+    uint32_t *valid;
+};
+
+// Enqueues one chunk of P paths starting at global path g0 on pipe pp:
+// camera, then maxBounce + 1 rounds of trace / shade / shadow, then the FP64
+// accumulation, which first waits for `acc_after` (the other pipe's last
+// accumulation, or nullptr) and then records pp.acc_done. Only kernel
+// launches and events on the pipe's stream, so the sequence can be captured
+// into a HIP graph. bv != nullptr: a batch chunk that mixes jobs (BATCH =
+// true); jc != nullptr: a batch chunk inside one job, run by the single-render
+// kernels with g0 the job's own path index.
 template <int W, bool BATCH, bool Q>
-int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
+int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *bv, hipEvent_t acc_after,
+              const JobChunk *jc = nullptr) {
     kr_scene *s = r->scene;
-    hipStream_t st = s->dev->stream;
-    const DevScene &S = s->view;
+    hipStream_t st = pp.stream;
+    const DevScene &S = jc ? jc->S : s->view;
+    const LensDev &lens = jc ? jc->lens : r->lens;
+    const uint32_t npix = jc ? jc->npix : r->npix, max_bounce = jc ? jc->max_bounce : r->prm.max_bounce;
+    const uint64_t seed = jc ? jc->seed : r->prm.seed;
+    double *acc = jc ? jc->acc : pp.acc;
+    uint32_t *valid = jc ? jc->valid : pp.valid;
     JobTable T = bv ? bv->T : JobTable{nullptr, nullptr, 0};
     // the profiler (single renders only); the one-hit kernels record no vertices
-    ProfView pv = BATCH ? ProfView{} : r->pv;
+    ProfView pv = (BATCH || jc) ? ProfView{} : r->pv;
     if (r->prm.kernel != KR_KERNEL_RADIANCE) pv.depth = 0;
     int grid = r->grid;
     int cam_grid = int(std::min<size_t>((P + kBlock - 1) / kBlock, size_t(grid)));
     int cur = 0;
     bool count = (r->profile & KR_PROFILE_COUNTERS) != 0;
     {
-        Timed tm(r, K_OTHER);
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, 0xFFFFFFFFu);
-        hipLaunchKernelGGL((k_camera<BATCH>), dim3(cam_grid), dim3(kBlock), 0, st, r->lens, T, g0, P, r->npix,
-                           r->q[0], r->L, r->counters, C_RAY0, r->nr, uint32_t(r->prm.seed),
-                           uint32_t(r->prm.seed >> 32), pv);
+        Timed tm(r, K_OTHER, st);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, pp.counters, 0xFFFFFFFFu);
+        hipLaunchKernelGGL((k_camera<BATCH>), dim3(cam_grid), dim3(kBlock), 0, st, lens, T, g0, P, npix,
+                           pp.q[0], pp.L, pp.counters, C_RAY0, pp.nr, uint32_t(seed), uint32_t(seed >> 32), pv);
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
     bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
     const uint32_t kern = r->prm.kernel;
-    const uint32_t rounds = kern == KR_KERNEL_RADIANCE ? r->prm.max_bounce + 1 : 1;
+    const uint32_t rounds = kern == KR_KERNEL_RADIANCE ? max_bounce + 1 : 1;
     for (uint32_t b = 0; b < rounds; ++b) {
         int qi = cur == 0 ? C_RAY0 : C_RAY1;
         int qni = cur == 0 ? C_RAY1 : C_RAY0;
         {
             // clear buckets, shadow count and the next ray queue's count
-            Timed tm(r, K_OTHER);
+            Timed tm(r, K_OTHER, st);
             uint32_t mask = (0x7u << C_BUCKET) | (1u << C_SHADOW) | (1u << qni) | (1u << C_FETCH) | (1u << C_FETCH_SH);
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, r->counters, mask);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, pp.counters, mask);
         }
         {
-            Timed tm(r, K_TRACE);
+            Timed tm(r, K_TRACE, st);
             auto trace = [&](auto kern) {
-                hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->q[cur], qi, r->hit,
-                                   r->bucket[0], r->bucket[1], r->bucket[2], r->counters, r->stats, sort, r->spill,
-                                   r->spill_lanes);
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, S, T, g0, pp.q[cur], qi, pp.hit,
+                                   pp.bucket[0], pp.bucket[1], pp.bucket[2], pp.counters, r->stats, sort, pp.spill,
+                                   pp.spill_lanes);
             };
             if (dyn)
                 count ? trace(k_trace_dyn<W, true, BATCH, Q>) : trace(k_trace_dyn<W, false, BATCH, Q>);
@@ -2254,69 +2367,94 @@ int run_chunk(kr_render *r, uint64_t g0, uint32_t P, const BatchView *bv) {
         }
         ShadeArgs a;
         a.S = S;
-        a.q = r->q[cur];
-        a.qn = r->q[1 - cur];
+        a.q = pp.q[cur];
+        a.qn = pp.q[1 - cur];
         a.qi = qi;
         a.qni = qni;
-        a.hit = r->hit;
-        a.b0 = r->bucket[0];
-        a.b1 = r->bucket[1];
-        a.b2 = r->bucket[2];
-        a.sq = r->sq;
-        a.L = r->L;
-        a.counters = r->counters;
+        a.hit = pp.hit;
+        a.b0 = pp.bucket[0];
+        a.b1 = pp.bucket[1];
+        a.b2 = pp.bucket[2];
+        a.sq = pp.sq;
+        a.L = pp.L;
+        a.counters = pp.counters;
         a.stats = r->stats;
         a.g0 = g0;
-        a.npix = r->npix;
+        a.npix = npix;
         a.bounce = b;
-        a.max_bounce = r->prm.max_bounce;
-        a.seed_lo = uint32_t(r->prm.seed);
-        a.seed_hi = uint32_t(r->prm.seed >> 32);
-        a.nr = r->nr;
+        a.max_bounce = max_bounce;
+        a.seed_lo = uint32_t(seed);
+        a.seed_hi = uint32_t(seed >> 32);
+        a.nr = pp.nr;
         a.T = T;
         a.pv = pv;
         {
-            Timed tm(r, K_SHADE);
+            Timed tm(r, K_SHADE, st);
             if (kern == KR_KERNEL_RADIANCE)
                 hipLaunchKernelGGL((k_shade<BATCH>), dim3(grid), dim3(kBlock), 0, st, a);
             else
                 hipLaunchKernelGGL((k_shade_debug<BATCH>), dim3(grid), dim3(kBlock), 0, st, a, kern);
         }
         {
-            Timed tm(r, K_SHADOW);
+            Timed tm(r, K_SHADOW, st);
             if (dyn)
-                hipLaunchKernelGGL((k_shadow_dyn<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq,
-                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b, pv);
+                hipLaunchKernelGGL((k_shadow_dyn<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, pp.sq,
+                                   pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, pp.nr, b, pv);
             else
-                hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, r->sq,
-                                   r->counters, r->L, r->stats, r->spill, r->spill_lanes, r->nr, b, pv);
+                hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, pp.sq,
+                                   pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, pp.nr, b, pv);
         }
         cur = 1 - cur;
     }
-    if (!BATCH && r->nr.enabled && kern == KR_KERNEL_RADIANCE) nrc_after_chunk(r);
+    if (!BATCH && r->nr.enabled && kern == KR_KERNEL_RADIANCE) nrc_after_chunk(r, pp);
     if (pv.enabled) {
-        Timed tm(r, K_OTHER);
-        hipLaunchKernelGGL(k_prof_final, dim3(cam_grid), dim3(kBlock), 0, st, pv, r->L, g0, P);
+        Timed tm(r, K_OTHER, st);
+        hipLaunchKernelGGL(k_prof_final, dim3(cam_grid), dim3(kBlock), 0, st, pv, pp.L, g0, P);
     }
+    if (acc_after) KR_HIP(hipStreamWaitEvent(st, acc_after, 0));
     {
-        Timed tm(r, K_OTHER);
+        Timed tm(r, K_OTHER, st);
         if (!BATCH) {
-            int acc_grid = int(std::min<size_t>((r->npix + kBlock - 1) / kBlock, size_t(grid)));
-            hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, r->L, g0, P, r->npix, r->acc,
-                               r->valid, r->stats);
+            int acc_grid = int(std::min<size_t>((npix + kBlock - 1) / kBlock, size_t(grid)));
+            hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, pp.L, g0, P, npix, acc, valid,
+                               r->stats);
         } else {
             // one accumulation per job overlapping the chunk, on its slice
             for (const HostJob &J : *bv->jobs) {
                 uint64_t lo = std::max(g0, J.begin), hi = std::min(g0 + P, J.begin + J.paths);
                 if (lo >= hi) continue;
                 int acc_grid = int(std::min<size_t>((J.npix + kBlock - 1) / kBlock, size_t(grid)));
-                hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, r->L + 3 * (lo - g0),
-                                   lo - J.begin, uint32_t(hi - lo), J.npix, r->acc + 3 * size_t(J.acc_off),
-                                   r->valid + J.acc_off, r->stats);
+                hipLaunchKernelGGL(k_accumulate, dim3(acc_grid), dim3(kBlock), 0, st, pp.L + 3 * (lo - g0),
+                                   lo - J.begin, uint32_t(hi - lo), J.npix, pp.acc + 3 * size_t(J.acc_off),
+                                   pp.valid + J.acc_off, r->stats);
             }
         }
     }
+    KR_HIP(hipEventRecord(pp.acc_done, st));
     KR_HIP(hipGetLastError());
+    return KR_OK;
+}
+
+// Runs f(pipe, acc_after) for consecutive chunks, alternating between the
+// render's pipes: pipe 1 first waits for everything queued on pipe 0 (resets,
+// earlier runs), each accumulation waits for the previous one, and pipe 0
+// finally waits for pipe 1, so that work queued afterwards on the device's
+// stream sees the whole run.
+template <class F>
+int pipelined(kr_render *r, int npipes, uint64_t nchunks, F &&f) {
+    if (npipes > 1 && nchunks > 1) {
+        KR_HIP(hipEventRecord(r->fork, r->pipe[0].stream));
+        KR_HIP(hipStreamWaitEvent(r->pipe[1].stream, r->fork, 0));
+    } else {
+        npipes = 1;
+    }
+    for (uint64_t k = 0; k < nchunks; ++k) {
+        Pipe &pp = r->pipe[k % npipes];
+        hipEvent_t after = k > 0 && npipes > 1 ? r->pipe[(k - 1) % npipes].acc_done : nullptr;
+        int rc = f(pp, after, k);
+        if (rc) return rc;
+    }
+    if (npipes > 1) KR_HIP(hipStreamWaitEvent(r->pipe[0].stream, r->pipe[1].acc_done, 0));
     return KR_OK;
 }
 }  // namespace
@@ -2329,22 +2467,25 @@ KR_EXPORT int kr_render_run(kr_render *r, uint32_t spp_begin, uint32_t spp_count
         return fail(KR_E_INVALID, "sample range exceeds the render's spp (the lens strata depend on it)");
     (void)hipSetDevice(r->scene->dev->ordinal);
     auto t0 = std::chrono::steady_clock::now();
-    uint64_t g = uint64_t(spp_begin) * r->npix, gend = uint64_t(spp_begin + spp_count) * r->npix;
-    while (g < gend) {
+    const uint64_t g0 = uint64_t(spp_begin) * r->npix, gend = uint64_t(spp_begin + spp_count) * r->npix;
+    const uint64_t nchunks = (gend - g0 + r->P - 1) / r->P;
+    const int np = (r->prm.flags & KR_RENDER_ONE_STREAM) ? 1 : r->npipes;
+    int rc = pipelined(r, np, nchunks, [&](Pipe &pp, hipEvent_t after, uint64_t k) {
+        uint64_t g = g0 + k * r->P;
         uint32_t P = uint32_t(std::min<uint64_t>(r->P, gend - g));
-        int rc = dispatch_fmt(r->scene->width, r->scene->qnodes, [&](auto fm) {
+        int rc2 = dispatch_fmt(r->scene->width, r->scene->qnodes, [&](auto fm) {
             using FM = decltype(fm);
-            return run_chunk<FM::W, false, FM::Q>(r, g, P, nullptr);
+            return run_chunk<FM::W, false, FM::Q>(r, pp, g, P, nullptr, after);
         });
-        if (rc) return rc;
-        g += P;
         // keep the number of outstanding event pairs bounded
-        if (r->pending.size() > 4096) {
-            KR_HIP(hipStreamSynchronize(r->scene->dev->stream));
+        if (!rc2 && r->pending.size() > 4096) {
+            for (int pi = 0; pi < r->npipes; ++pi) KR_HIP(hipStreamSynchronize(r->pipe[pi].stream));
             resolve_events(r);
         }
-    }
-    r->paths += gend - uint64_t(spp_begin) * r->npix;
+        return rc2;
+    });
+    if (rc) return rc;
+    r->paths += gend - g0;
     if (r->nr.enabled) {
         int rc = nrc_after_run(r);
         if (rc) return rc;
@@ -2476,6 +2617,9 @@ KR_EXPORT int kr_render_get_stats(kr_render *r, kr_render_stats *out) {
     out->trace_launches = r->kind_launches[K_TRACE];
     out->shade_launches = r->kind_launches[K_SHADE];
     out->shadow_launches = r->kind_launches[K_SHADOW];
+    out->trace_busy_ms = r->kind_busy_ms[K_TRACE];
+    out->shade_busy_ms = r->kind_busy_ms[K_SHADE];
+    out->shadow_busy_ms = r->kind_busy_ms[K_SHADOW];
     return KR_OK;
 }
 
@@ -2883,15 +3027,15 @@ __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_
 }
 
 namespace {
-void nrc_after_chunk(kr_render *r) {
+void nrc_after_chunk(kr_render *r, Pipe &pp) {
     kr_nrc *n = r->nrc;
-    hipStream_t st = r->scene->dev->stream;
+    hipStream_t st = pp.stream;
     {
-        Timed tm(r, K_OTHER);
+        Timed tm(r, K_OTHER, st);
         int grid = int(std::min<size_t>((size_t(r->P) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
-        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_view(n), r->nr, r->counters, r->L);
-        int egrid = int(std::min<size_t>((size_t(r->nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
-        hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, r->nr, r->counters, n->pool_raw, n->pool_tgt,
+        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_view(n), pp.nr, pp.counters, pp.L);
+        int egrid = int(std::min<size_t>((size_t(pp.nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
+        hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, pp.nr, pp.counters, n->pool_raw, n->pool_tgt,
                            n->pool_total, n->pool_cap, n->cfg.target_clamp);
     }
 }
@@ -2922,9 +3066,11 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     if (n && r->pv.enabled) return fail(KR_E_UNSUPPORTED, "a render with a profiler takes no radiance cache");
     (void)hipSetDevice(r->scene->dev->ordinal);
     (void)hipStreamSynchronize(r->scene->dev->stream);
+    for (int pi = 0; pi < r->npipes; ++pi) (void)hipStreamSynchronize(r->pipe[pi].stream);
     for (void *p : r->nrc_allocs) (void)hipFree(p);
     r->nrc_allocs.clear();
     r->nr = NrcRender{};
+    for (Pipe &pp : r->pipe) pp.nr = NrcRender{};
     r->nrc = nullptr;
     if (!n) return KR_OK;
     if (n->dev != r->scene->dev) return fail(KR_E_INVALID, "cache and render live on different devices");
@@ -2942,19 +3088,25 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
         return rc;
     };
     int rc = KR_OK;
-    if (!rc) rc = get(&nr.tslot_of, r->P);
-    if (!rc) rc = get(&nr.rec, size_t(nr.train_cap) * nr.nv * kRecFloats);
-    if (!rc) rc = get(&nr.tcount, nr.train_cap);
-    if (!rc) rc = get(&nr.qslot, r->P);
-    if (!rc) rc = get(&nr.qbeta, 3 * size_t(r->P));
-    if (!rc) rc = get(&nr.qraw, size_t(r->P) * nrc::kRaw);
+    nr.enabled = 1;
+    // each pipe records and queries through buffers of its own
+    for (int pi = 0; pi < r->npipes && !rc; ++pi) {
+        NrcRender &pn = r->pipe[pi].nr;
+        pn = nr;
+        if (!rc) rc = get(&pn.tslot_of, r->P);
+        if (!rc) rc = get(&pn.rec, size_t(nr.train_cap) * nr.nv * kRecFloats);
+        if (!rc) rc = get(&pn.tcount, nr.train_cap);
+        if (!rc) rc = get(&pn.qslot, r->P);
+        if (!rc) rc = get(&pn.qbeta, 3 * size_t(r->P));
+        if (!rc) rc = get(&pn.qraw, size_t(r->P) * nrc::kRaw);
+    }
     if (rc) {
         for (void *p : r->nrc_allocs) (void)hipFree(p);
         r->nrc_allocs.clear();
         r->nr = NrcRender{};
+        for (Pipe &pp : r->pipe) pp.nr = NrcRender{};
         return rc;
     }
-    nr.enabled = 1;
     r->nrc = n;
     return KR_OK;
 }
@@ -2989,8 +3141,8 @@ struct kr_batch {
     JobRec *d_trec = nullptr;     // trial job table (autotuner)
     uint64_t *d_tbegin = nullptr;
     std::vector<void *> allocs;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph[2] = {nullptr, nullptr};  // one per pipe: its chunks, replayed on its stream
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
     uint32_t graph_nodes = 0, chunks = 0;
     double capture_seconds = 0, seconds = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -3002,15 +3154,20 @@ constexpr uint32_t kMaxBlocksPerCu = 64;
 constexpr uint32_t kBatchMaxChunk = 32u << 20;
 
 void batch_drop_graph(kr_batch *b) {
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    for (int pi = 0; pi < 2; ++pi) {
+        if (b->exec[pi]) (void)hipGraphExecDestroy(b->exec[pi]);
+        if (b->graph[pi]) (void)hipGraphDestroy(b->graph[pi]);
+        b->exec[pi] = nullptr;
+        b->graph[pi] = nullptr;
+    }
 }
 
 // Job tables for the first spp_limit samples of every job (0: all).
+// spp_limit: every job's first min(spp_limit, spp) samples (0: all); frac
+// (with spp_limit 0): the first max(1, spp * frac) samples of every job, a
+// scaled copy of the batch's mix of job sizes.
 void batch_tables(const kr_batch *b, uint32_t spp_limit, std::vector<JobRec> &rec, std::vector<uint64_t> &begin,
-                  std::vector<HostJob> &hj) {
+                  std::vector<HostJob> &hj, double frac = 1.0) {
     rec.resize(b->jobs.size());
     begin.resize(b->jobs.size());
     hj.resize(b->jobs.size());
@@ -3019,6 +3176,8 @@ void batch_tables(const kr_batch *b, uint32_t spp_limit, std::vector<JobRec> &re
         const kr_batch_job &J = b->jobs[k];
         uint32_t npix = J.params.width * J.params.height;
         uint32_t spp = spp_limit ? std::min(spp_limit, J.params.spp) : J.params.spp;
+        if (!spp_limit && frac < 1.0)
+            spp = std::max<uint32_t>(1, std::min<uint32_t>(J.params.spp, uint32_t(double(J.params.spp) * frac + 0.5)));
         JobRec &R = rec[k];
         R.S = b->up[J.scene]->view;
         R.lens = make_lens(&J.camera, &J.params);
@@ -3034,10 +3193,10 @@ void batch_tables(const kr_batch *b, uint32_t spp_limit, std::vector<JobRec> &re
 }
 
 int batch_upload_tables(kr_batch *b, JobRec *d_rec, uint64_t *d_begin, uint32_t spp_limit,
-                        std::vector<HostJob> &hj) {
+                        std::vector<HostJob> &hj, double frac = 1.0) {
     std::vector<JobRec> rec;
     std::vector<uint64_t> begin;
-    batch_tables(b, spp_limit, rec, begin, hj);
+    batch_tables(b, spp_limit, rec, begin, hj, frac);
     hipStream_t st = b->dev->stream;
     KR_HIP(hipMemcpyAsync(d_rec, rec.data(), sizeof(JobRec) * rec.size(), hipMemcpyHostToDevice, st));
     KR_HIP(hipMemcpyAsync(d_begin, begin.data(), sizeof(uint64_t) * begin.size(), hipMemcpyHostToDevice, st));
@@ -3069,46 +3228,101 @@ int batch_set_width(kr_batch *b, uint32_t width) {
     return batch_upload_tables(b, b->d_rec, b->d_begin, 0, b->hjobs);
 }
 
-// Enqueues every chunk of a path space of `total` paths.
-int batch_enqueue(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P) {
-    int rc = KR_OK;
-    for (uint64_t g = 0; g < total && !rc; g += P) {
-        uint32_t n = uint32_t(std::min<uint64_t>(P, total - g));
-        rc = dispatch_fmt(b->tuning.bvh_width, b->up[0]->qnodes, [&](auto fm) {
-            using FM = decltype(fm);
-            return run_chunk<FM::W, true, FM::Q>(b->r, g, n, &bv);
-        });
+// The chunks of a batch's path space of `total` paths, at most P paths each:
+// by default every job's range is cut on its own, so no chunk mixes jobs and
+// every chunk runs the single-render kernels; with KR_BATCH_MIX_JOBS the
+// space is cut into equal chunks regardless of job boundaries, and the
+// chunks that straddle one run the batch kernels (each path looks up its job).
+std::vector<std::pair<uint64_t, uint32_t>> batch_chunks(const kr_batch *b, const BatchView &bv, uint64_t total,
+                                                        uint32_t P) {
+    std::vector<std::pair<uint64_t, uint32_t>> c;
+    if (b->tuning.flags & KR_BATCH_MIX_JOBS) {
+        for (uint64_t g = 0; g < total; g += P) c.push_back({g, uint32_t(std::min<uint64_t>(P, total - g))});
+    } else {
+        for (const HostJob &J : *bv.jobs)
+            for (uint64_t g = J.begin; g < J.begin + J.paths; g += P)
+                c.push_back({g, uint32_t(std::min<uint64_t>(P, J.begin + J.paths - g))});
     }
-    return rc;
+    return c;
+}
+
+// Enqueues on pipe pi the chunks k = pi, pi + npipes, ... of the batch's
+// chunk list. A batch's pipes accumulate into images of their own (read back
+// summed), so the pipes never wait for each other and each pipe's chunks form
+// a graph of their own.
+int batch_enqueue_pipe(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P, int pi) {
+    kr_render *r = b->r;
+    const std::vector<HostJob> &hj = *bv.jobs;
+    Pipe &pp = r->pipe[pi];
+    auto chunks = batch_chunks(b, bv, total, P);
+    for (size_t k = size_t(pi); k < chunks.size(); k += size_t(r->npipes)) {
+        uint64_t g = chunks[k].first;
+        uint32_t n = chunks[k].second;
+        // a chunk inside one job runs the single-render kernels (scene views
+        // as kernel arguments, fewer registers); one that mixes jobs the
+        // batch kernels
+        uint32_t j = 0;
+        while (j + 1 < hj.size() && hj[j + 1].begin <= g) ++j;
+        bool single = g >= hj[j].begin && g + n <= hj[j].begin + hj[j].paths;
+        int rc = dispatch_fmt(b->tuning.bvh_width, b->up[0]->qnodes, [&](auto fm) {
+            using FM = decltype(fm);
+            if (!single) return run_chunk<FM::W, true, FM::Q>(r, pp, g, n, &bv, nullptr);
+            const kr_batch_job &J = b->jobs[j];
+            JobChunk jc{b->up[J.scene]->view, make_lens(&J.camera, &J.params), hj[j].npix, J.params.max_bounce,
+                        J.params.seed, pp.acc + 3 * size_t(hj[j].acc_off), pp.valid + hj[j].acc_off};
+            return run_chunk<FM::W, false, FM::Q>(r, pp, g - hj[j].begin, n, nullptr, nullptr, &jc);
+        });
+        if (rc) return rc;
+    }
+    return KR_OK;
+}
+
+// Both pipes' chunks, pipe 1 forked from and joined back into the device's
+// stream.
+int batch_enqueue(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P) {
+    kr_render *r = b->r;
+    if (r->npipes > 1) {
+        KR_HIP(hipEventRecord(r->fork, r->pipe[0].stream));
+        KR_HIP(hipStreamWaitEvent(r->pipe[1].stream, r->fork, 0));
+    }
+    for (int pi = 0; pi < r->npipes; ++pi) {
+        int rc = batch_enqueue_pipe(b, bv, total, P, pi);
+        if (rc) return rc;
+    }
+    if (r->npipes > 1) KR_HIP(hipStreamWaitEvent(r->pipe[0].stream, r->pipe[1].acc_done, 0));
+    return KR_OK;
 }
 
 void batch_apply_launch(kr_batch *b) {
     b->r->grid = int(std::min<size_t>(size_t(std::max(b->dev->num_cu, 1)) * b->tuning.blocks_per_cu,
-                                      b->r->spill_lanes / kBlock));
-    b->r->prm.flags = b->tuning.flags;
+                                      b->r->pipe[0].spill_lanes / kBlock));
+    b->r->prm.flags = b->tuning.flags & ~KR_BATCH_MIX_JOBS;
 }
 
 int batch_capture(kr_batch *b) {
     batch_drop_graph(b);
     batch_apply_launch(b);
     auto t0 = std::chrono::steady_clock::now();
-    hipStream_t st = b->dev->stream;
     BatchView bv{JobTable{b->d_rec, b->d_begin, uint32_t(b->jobs.size())}, &b->hjobs};
-    KR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int rc = batch_enqueue(b, bv, b->total_paths, b->tuning.chunk_paths);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(st, &g);
-    if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
+    b->graph_nodes = 0;
+    for (int pi = 0; pi < b->r->npipes; ++pi) {
+        hipStream_t st = b->r->pipe[pi].stream;
+        KR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        int rc = batch_enqueue_pipe(b, bv, b->total_paths, b->tuning.chunk_paths, pi);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return fail(KR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        b->graph[pi] = g;
+        size_t nodes = 0;
+        KR_HIP(hipGraphGetNodes(g, nullptr, &nodes));
+        b->graph_nodes += uint32_t(nodes);
+        KR_HIP(hipGraphInstantiate(&b->exec[pi], g, nullptr, nullptr, 0));
     }
-    if (e != hipSuccess) return fail(KR_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-    b->graph = g;
-    size_t nodes = 0;
-    KR_HIP(hipGraphGetNodes(g, nullptr, &nodes));
-    b->graph_nodes = uint32_t(nodes);
-    KR_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
-    b->chunks = uint32_t((b->total_paths + b->tuning.chunk_paths - 1) / b->tuning.chunk_paths);
+    b->chunks = uint32_t(batch_chunks(b, bv, b->total_paths, b->tuning.chunk_paths).size());
     b->capture_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return KR_OK;
 }
@@ -3218,7 +3432,16 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
         r->prm.flags = tu.flags;
         r->npix = uint32_t(pix);
         r->P = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(std::max(tu.chunk_paths, kBatchMaxChunk), total)));
+        r->npipes = total >= kPipeMinPaths ? 2 : 1;
         rc = render_buffers(r, size_t(std::max(dev->num_cu, 1)) * kMaxBlocksPerCu * kBlock);
+        // pipe 1 accumulates into an image of its own (kr_batch_read sums them)
+        if (!rc && r->npipes > 1) {
+            Pipe &p1 = r->pipe[1];
+            rc = dalloc(&p1.acc, 3 * size_t(r->npix));
+            if (!rc) r->allocs.push_back(p1.acc);
+            if (!rc) rc = dalloc(&p1.valid, size_t(r->npix));
+            if (!rc) r->allocs.push_back(p1.valid);
+        }
         if (!rc) rc = kr_render_reset(r);
     }
     if (!rc && (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess))
@@ -3235,13 +3458,19 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
 KR_EXPORT int kr_batch_run(kr_batch *b) {
     if (!b) return fail(KR_E_INVALID, "NULL batch");
     (void)hipSetDevice(b->dev->ordinal);
-    if (!b->exec) {
+    if (!b->exec[0]) {
         int rc = batch_capture(b);
         if (rc) return rc;
     }
+    kr_render *r = b->r;
     hipStream_t st = b->dev->stream;
     KR_HIP(hipEventRecord(b->e0, st));
-    KR_HIP(hipGraphLaunch(b->exec, st));
+    if (r->npipes > 1) KR_HIP(hipStreamWaitEvent(r->pipe[1].stream, b->e0, 0));
+    for (int pi = 0; pi < r->npipes; ++pi) KR_HIP(hipGraphLaunch(b->exec[pi], r->pipe[pi].stream));
+    if (r->npipes > 1) {
+        KR_HIP(hipEventRecord(r->fork, r->pipe[1].stream));
+        KR_HIP(hipStreamWaitEvent(st, r->fork, 0));
+    }
     KR_HIP(hipEventRecord(b->e1, st));
     b->timed = true;
     b->r->paths += b->total_paths;
@@ -3272,7 +3501,24 @@ KR_EXPORT int kr_batch_read(kr_batch *b, uint32_t job, double *rgb, uint32_t *va
     size_t off = b->acc_off[job], n = size_t(b->jobs[job].params.width) * b->jobs[job].params.height;
     if (rgb) KR_HIP(hipMemcpyAsync(rgb, b->r->acc + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
     if (valid) KR_HIP(hipMemcpyAsync(valid, b->r->valid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+    // the second pipe's image: the job's sums are the two images' sums
+    std::vector<double> rgb1;
+    std::vector<uint32_t> valid1;
+    const Pipe &p1 = b->r->pipe[1];
+    bool two = b->r->npipes > 1 && p1.acc != b->r->acc;
+    if (two) {
+        rgb1.resize(rgb ? 3 * n : 0);
+        valid1.resize(valid ? n : 0);
+        if (rgb) KR_HIP(hipMemcpyAsync(rgb1.data(), p1.acc + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+        if (valid) KR_HIP(hipMemcpyAsync(valid1.data(), p1.valid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+    }
     KR_HIP(hipStreamSynchronize(st));
+    if (two) {
+        if (rgb)
+            for (size_t i = 0; i < 3 * n; ++i) rgb[i] += rgb1[i];
+        if (valid)
+            for (size_t i = 0; i < n; ++i) valid[i] += valid1[i];
+    }
     return KR_OK;
 }
 
@@ -3294,14 +3540,14 @@ KR_EXPORT int kr_batch_autotune(kr_batch *b, uint32_t trial_spp, kr_batch_tuning
     if (!b) return fail(KR_E_INVALID, "NULL batch");
     (void)hipSetDevice(b->dev->ordinal);
     hipStream_t st = b->dev->stream;
-    if (trial_spp == 0) {
-        // about 64 M trial paths, at least one sample of every job
-        uint64_t pix = std::max<uint64_t>(b->total_pix, 1);
-        trial_spp = uint32_t(std::max<uint64_t>(1, (64ull << 20) / pix));
-    }
+    // trial_spp 0: the same fraction of every job's samples, about 256 M
+    // paths in all, so that the trial has the batch's mix of job sizes (a
+    // uniform sample count favoured small chunks that the full batch does not)
+    double frac = 1.0;
+    if (trial_spp == 0) frac = std::min(1.0, double(256ull << 20) / double(std::max<uint64_t>(b->total_paths, 1)));
     const uint32_t widths[3] = {2, 4, 8};
-    const uint32_t chunks[4] = {4u << 20, 8u << 20, 16u << 20, 32u << 20};
-    const uint32_t bpcs[3] = {16, 32, 64};
+    const uint32_t chunks[2] = {16u << 20, 32u << 20};
+    const uint32_t bpcs[2] = {32, 64};
     kr_batch_tuning best = b->tuning;
     best.paths_per_second = -1.0;
     uint32_t flags = b->tuning.flags;
@@ -3310,7 +3556,7 @@ KR_EXPORT int kr_batch_autotune(kr_batch *b, uint32_t trial_spp, kr_batch_tuning
         if (rc == KR_E_OVERFLOW) continue;  // a BVH of this width is too deep for the stack
         if (rc) return rc;
         std::vector<HostJob> thj;
-        rc = batch_upload_tables(b, b->d_trec, b->d_tbegin, trial_spp, thj);
+        rc = batch_upload_tables(b, b->d_trec, b->d_tbegin, trial_spp, thj, frac);
         if (rc) return rc;
         uint64_t ttotal = 0;
         for (const HostJob &h : thj) ttotal += h.paths;

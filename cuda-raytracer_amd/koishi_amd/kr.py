@@ -22,6 +22,7 @@ KR_MEM_HOST = 0
 KR_MEM_DEVICE = 1
 KR_RENDER_NO_MATERIAL_SORT = 0x1
 KR_RENDER_PERSISTENT_TRAVERSAL = 0x2
+KR_RENDER_ONE_STREAM = 0x4
 
 SHADER_LAMBERT, SHADER_GLOSSY, SHADER_EMISSION = 0, 1, 2
 SHADER_GLASS, SHADER_REFRACTION, SHADER_TRANSPARENT, SHADER_BLACKHOLE, SHADER_MIXED = 3, 4, 5, 6, 7
@@ -99,10 +100,12 @@ class RenderStats(C.Structure):
     _fields_ = [("paths", u64), ("closest_rays", u64), ("shadow_rays", u64), ("invalid_samples", u64),
                 ("seconds", C.c_double), ("node_visits", u64), ("tri_tests", u64), ("trace_ms", C.c_double),
                 ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("other_ms", C.c_double),
-                ("trace_launches", u64), ("shade_launches", u64), ("shadow_launches", u64)]
+                ("trace_launches", u64), ("shade_launches", u64), ("shadow_launches", u64),
+                ("trace_busy_ms", C.c_double), ("shade_busy_ms", C.c_double), ("shadow_busy_ms", C.c_double)]
 
 
 KR_PROFILE_EVENTS = 0x1
+KR_BATCH_MIX_JOBS = 0x100  # BatchTuning(flags=...): chunks may straddle jobs
 KR_PROFILE_COUNTERS = 0x2
 
 NRC_RAW, NRC_FEATURES, NRC_PARAMS = 16, 96, 23888

@@ -194,6 +194,9 @@ enum kr_kernel_type {
 #define KR_RENDER_PERSISTENT_TRAVERSAL 0x2u /* traversal kernels refill lanes whose ray
                                                is done from the queue (persistent
                                                threads) instead of a fixed share */
+#define KR_RENDER_ONE_STREAM 0x4u /* run every chunk on the device's stream; by
+                                     default a render of >= 8 M paths alternates
+                                     its chunks between two streams (same image) */
 
 /* ---- ray query buffers -------------------------------------------------- */
 
@@ -224,10 +227,14 @@ typedef struct kr_render_stats {
     /* filled when KR_PROFILE_COUNTERS is on: closest-hit traversal work */
     uint64_t node_visits;    /* wide-BVH nodes loaded                      */
     uint64_t tri_tests;      /* ray/triangle tests                         */
-    /* filled when KR_PROFILE_EVENTS is on: HIP-event time on the render's
-       stream, summed over launches since the last reset                   */
+    /* filled when KR_PROFILE_EVENTS is on: HIP-event time of the render's
+       launches, summed over launches since the last reset                 */
     double trace_ms, shade_ms, shadow_ms, other_ms;
     uint64_t trace_launches, shade_launches, shadow_launches;
+    /* the same launches' busy time: the union of their intervals. A render
+       of at least 8 M paths alternates its chunks between two streams, so
+       launches of one kind overlap and the sums above exceed this        */
+    double trace_busy_ms, shade_busy_ms, shadow_busy_ms;
 } kr_render_stats;
 
 #define KR_PROFILE_EVENTS 0x1u   /* time every kernel with HIP events        */
@@ -435,9 +442,14 @@ typedef struct kr_batch_tuning {
     uint32_t bvh_width;     /* 2, 4 or 8                                     */
     uint32_t chunk_paths;   /* paths in flight per wavefront chunk           */
     uint32_t blocks_per_cu; /* 256-thread blocks per CU of every launch (1..64) */
-    uint32_t flags;         /* KR_RENDER_* bits                              */
+    uint32_t flags;         /* KR_RENDER_* bits, KR_BATCH_MIX_JOBS           */
     double paths_per_second;/* kr_batch_autotune: the winner's trial rate    */
 } kr_batch_tuning;
+
+/* kr_batch_tuning.flags: cut the batch's path space into equal chunks that
+ * may straddle jobs (their paths look up their job on the device) instead of
+ * cutting every job's range on its own. */
+#define KR_BATCH_MIX_JOBS 0x100u
 
 typedef struct kr_batch_info {
     uint64_t total_paths;   /* sum of width * height * spp over the jobs     */
@@ -456,9 +468,10 @@ typedef struct kr_batch_info {
 int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint32_t num_scenes, const kr_batch_job *jobs,
                     uint32_t num_jobs, const kr_batch_tuning *tuning, kr_batch **out);
 /* Autotuner: times trial renders of the first trial_spp samples of every
- * job (0: enough samples for about 64 M paths) for every BVH width in
- * {2, 4, 8}, chunk size in {4, 8, 16, 32} M paths and blocks per CU in
- * {16, 32, 64}, applies the fastest and returns it. */
+ * job (0: the same fraction of every job's samples, about 256 M paths in
+ * all) for every BVH width in
+ * {2, 4, 8}, chunk size in {16, 32} M paths and blocks per CU in {32, 64},
+ * applies the fastest and returns it. */
 int kr_batch_autotune(kr_batch *batch, uint32_t trial_spp, kr_batch_tuning *chosen);
 /* Renders every sample of every job (adds to the accumulators) with one
  * launch of the captured graph; asynchronous. */

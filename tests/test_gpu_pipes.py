@@ -1,0 +1,87 @@
+"""Renders of >= 8 M paths alternate their chunks between two streams
+(struct Pipe): the image must be bit-identical to the same chunks run one
+after another on one stream (KR_RENDER_ONE_STREAM), with and without a
+radiance cache and a profiler; batches whose work spans both streams must
+match their jobs rendered alone, with chunks cut per job and with chunks that
+straddle jobs (KR_BATCH_MIX_JOBS)."""
+import numpy as np
+import pytest
+
+from koishi_amd import kr, scenegen as sg
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def spheres():
+    return sg.sphere_scene(width=1024, height=1024, spp=16)  # 16.8 M paths: two chunks, two streams
+
+
+def _render(device, scene, flags=0, nrc=None, seed=5):
+    gs = kr.GpuScene(device, scene.to_desc())
+    r = kr.Render(gs, scene.camera_struct(), scene.params(seed=seed, flags=flags))
+    if nrc is not None:
+        r.attach_nrc(nrc)
+    r.run(0, scene.spp)
+    out = r.read()
+    r.free()
+    gs.free()
+    return out
+
+
+def test_two_streams_match_one(device, spheres):
+    rgb2, val2 = _render(device, spheres)
+    rgb1, val1 = _render(device, spheres, flags=kr.KR_RENDER_ONE_STREAM)
+    assert np.array_equal(val2, val1) and np.array_equal(rgb2, rgb1)
+    assert val1.sum() > 0.99 * spheres.width * spheres.height * spheres.spp
+
+
+def test_two_streams_with_a_cache(device, spheres):
+    # the first run's image depends on the cache's initial weights only (the
+    # cache trains after the run), so both schedules give the same image
+    pos = np.asarray(spheres.positions)
+    out = []
+    for flags in (0, kr.KR_RENDER_ONE_STREAM):
+        nrc = kr.Nrc(device, pos.min(axis=0) - 10.0, pos.max(axis=0) + 10.0, seed=3)
+        out.append(_render(device, spheres, flags=flags, nrc=nrc))
+        nrc.free()
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
+
+
+def _batch_jobs():
+    scenes = [sg.sphere_scene(), sg.mesh_scene(freq=24)]
+    spec = [(0, 512, 512, 40, 5, 21), (1, 96, 40, 16, 8, 22), (0, 64, 48, 9, 5, 23), (1, 640, 360, 24, 8, 24)]
+    return scenes, [(k, scenes[k].camera_struct(), kr.Params(w, h, spp, mb, seed, 0, 0))
+                    for k, w, h, spp, mb, seed in spec]
+
+
+@pytest.mark.parametrize("flags", [0, kr.KR_BATCH_MIX_JOBS])
+def test_batch_on_two_streams_matches_jobs_alone(device, flags):
+    scenes, jobs = _batch_jobs()
+    ups = [kr.GpuScene(device, sc.to_desc(bvh_width=4)) for sc in scenes]
+    ref = []
+    for k, cam, p in jobs:
+        r = kr.Render(ups[k], cam, p)
+        r.run(0, p.spp)
+        ref.append(r.read())
+        r.free()
+    for u in ups:
+        u.free()
+    total = sum(p.width * p.height * p.spp for _, _, p in jobs)
+    assert total > 8 << 20
+    P = 3_000_017
+    b = kr.Batch(device, scenes, jobs, kr.BatchTuning(4, P, 32, flags, 0.0))
+    for rep in range(2):
+        b.reset()
+        b.run()
+        b.sync()
+        info = b.info()
+        if flags:
+            assert info.chunks == -(-total // P)
+        else:
+            assert info.chunks == sum(-(-(p.width * p.height * p.spp) // P) for _, _, p in jobs)
+        for j in range(len(jobs)):
+            rgb, val = b.read(j)
+            assert np.array_equal(val, ref[j][1]), (rep, j)
+            assert np.allclose(rgb, ref[j][0], rtol=1e-12, atol=1e-12), (rep, j)
+    b.free()
