@@ -228,6 +228,11 @@ struct kr_nrc {
     kr_nrc_config cfg{};
     float *master = nullptr, *adam_m = nullptr, *adam_v = nullptr;
     uint16_t *wt = nullptr, *w = nullptr;
+    // what renders query: the bias-corrected moving average of the trained
+    // parameters (FP32) and its BF16 W^T; null when renders query the
+    // trained parameters themselves
+    float *ema = nullptr;
+    uint16_t *wt_ema = nullptr;
     float *partial = nullptr;  // [batch / 64][kNumParams]
     float *loss = nullptr;     // device scalar
     uint32_t rows = 0;         // batch / 64
@@ -1410,10 +1415,14 @@ __global__ __launch_bounds__(64) void k_nrc_fwd_bwd(nrc::Weights W, TrainSource 
 }
 
 // Sums the partial rows, takes one Adam step on the FP32 masters and
-// refreshes the BF16 copies (W^T for the forward, W for the backward).
+// refreshes the BF16 copies (W^T for the forward, W for the backward). With
+// ema non-null it also moves the average towards the new masters by
+// ema_a = (1 - d) / (1 - d^t) at step t, the bias-corrected exponential
+// moving average of decay d (ema_a = 1 at the first step), and refreshes
+// its BF16 W^T.
 __global__ void k_nrc_adam(float *master, uint16_t *wt, uint16_t *w, float *m, float *v, const float *partial,
                            uint32_t rows, float lr, float b1, float b2, float eps, float c1, float c2,
-                           const uint32_t *pool_total) {
+                           const uint32_t *pool_total, float *ema, uint16_t *wt_ema, float ema_a) {
     using namespace nrc;
     if (pool_total && *pool_total == 0) return;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kNumParams; p += gridDim.x * blockDim.x) {
@@ -1425,6 +1434,11 @@ __global__ void k_nrc_adam(float *master, uint16_t *wt, uint16_t *w, float *m, f
         v[p] = vv;
         float val = master[p] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
         master[p] = val;
+        float av = 0.f;
+        if (ema) {
+            av = ema[p] + (val - ema[p]) * ema_a;
+            ema[p] = av;
+        }
         for (int l = 0; l < kLayers; ++l) {
             int in = kDims[l], out = kDims[l + 1];
             if (p >= w_off(l) && p < b_off(l)) {
@@ -1432,6 +1446,7 @@ __global__ void k_nrc_adam(float *master, uint16_t *wt, uint16_t *w, float *m, f
                 uint16_t hb = to_bf16(val);
                 w[bw_off(l) + i * out + o] = hb;
                 wt[bw_off(l) + o * in + i] = hb;
+                if (ema) wt_ema[bw_off(l) + o * in + i] = to_bf16(av);
             }
         }
     }
@@ -2657,6 +2672,16 @@ nrc::Weights nrc_view(const kr_nrc *n) {
     return W;
 }
 
+// The weights renders query (k_nrc_infer).
+nrc::Weights nrc_render_view(const kr_nrc *n) {
+    nrc::Weights W = nrc_view(n);
+    if (n->ema) {
+        W.wt = n->wt_ema;
+        W.master = n->ema;
+    }
+    return W;
+}
+
 // BF16 copies of the master weights, made on the host (set_params / create).
 void nrc_host_bf16(const float *master, std::vector<uint16_t> &w, std::vector<uint16_t> &wt) {
     using namespace nrc;
@@ -2684,6 +2709,10 @@ int nrc_upload_params(kr_nrc *n, const float *host) {
     KR_HIP(hipMemcpyAsync(n->master, host, sizeof(float) * nrc::kNumParams, hipMemcpyHostToDevice, st));
     KR_HIP(hipMemcpyAsync(n->w, w.data(), 2 * w.size(), hipMemcpyHostToDevice, st));
     KR_HIP(hipMemcpyAsync(n->wt, wt.data(), 2 * wt.size(), hipMemcpyHostToDevice, st));
+    if (n->ema) {
+        KR_HIP(hipMemcpyAsync(n->ema, host, sizeof(float) * nrc::kNumParams, hipMemcpyHostToDevice, st));
+        KR_HIP(hipMemcpyAsync(n->wt_ema, wt.data(), 2 * wt.size(), hipMemcpyHostToDevice, st));
+    }
     KR_HIP(hipMemsetAsync(n->adam_m, 0, sizeof(float) * nrc::kNumParams, st));
     KR_HIP(hipMemsetAsync(n->adam_v, 0, sizeof(float) * nrc::kNumParams, st));
     KR_HIP(hipStreamSynchronize(st));
@@ -2701,8 +2730,10 @@ int nrc_step(kr_nrc *n, const TrainSource &src, uint32_t batch, uint32_t rows) {
     float c1 = float(1.0 - std::pow(double(b1), double(n->step)));
     float c2 = float(1.0 - std::pow(double(b2), double(n->step)));
     float lr = n->cfg.learning_rate > 0.f ? n->cfg.learning_rate : 1e-2f;
+    const double d = n->cfg.ema_decay;
+    float ema_a = n->ema ? float((1.0 - d) / (1.0 - std::pow(d, double(n->step)))) : 0.f;
     hipLaunchKernelGGL(k_nrc_adam, dim3(96), dim3(256), 0, st, n->master, n->wt, n->w, n->adam_m, n->adam_v,
-                       n->partial, rows, lr, b1, b2, eps, c1, c2, src.pool_total);
+                       n->partial, rows, lr, b1, b2, eps, c1, c2, src.pool_total, n->ema, n->wt_ema, ema_a);
     KR_HIP(hipGetLastError());
     return KR_OK;
 }
@@ -2743,6 +2774,11 @@ KR_EXPORT int kr_nrc_create(kr_device *dev, const kr_nrc_config *cfg, kr_nrc **o
         return fail(KR_E_INVALID, "train_log2 must be at most 31 (or KR_NRC_TRAIN_ALL)");
     }
     if (n->cfg.rough_threshold <= 0.f) n->cfg.rough_threshold = 0.3f;
+    if (n->cfg.ema_decay == 0.f) n->cfg.ema_decay = 0.99f;
+    if (!(n->cfg.ema_decay < 1.f)) {
+        delete n;
+        return fail(KR_E_INVALID, "ema_decay must be below 1 (0 for the default, negative for none)");
+    }
     n->cfg.batch = (n->cfg.batch + 63) / 64 * 64;
     n->rows = n->cfg.batch / 64;
     n->pool_cap = 1u << 18;
@@ -2752,6 +2788,8 @@ KR_EXPORT int kr_nrc_create(kr_device *dev, const kr_nrc_config *cfg, kr_nrc **o
     if (!rc) rc = nrc_alloc(n, &n->adam_v, kNumParams);
     if (!rc) rc = nrc_alloc(n, &n->wt, kNumWeights);
     if (!rc) rc = nrc_alloc(n, &n->w, kNumWeights);
+    if (!rc && n->cfg.ema_decay > 0.f) rc = nrc_alloc(n, &n->ema, kNumParams);
+    if (!rc && n->cfg.ema_decay > 0.f) rc = nrc_alloc(n, &n->wt_ema, kNumWeights);
     if (!rc) rc = nrc_alloc(n, &n->partial, size_t(n->rows) * kNumParams);
     if (!rc) rc = nrc_alloc(n, &n->loss, 1);
     if (!rc) rc = nrc_alloc(n, &n->pool_raw, size_t(n->pool_cap) * kRaw);
@@ -2792,6 +2830,14 @@ KR_EXPORT int kr_nrc_create(kr_device *dev, const kr_nrc_config *cfg, kr_nrc **o
 KR_EXPORT int kr_nrc_get_params(kr_nrc *n, float *params) {
     if (!n || !params) return fail(KR_E_INVALID, "NULL argument");
     KR_HIP(hipMemcpyAsync(params, n->master, sizeof(float) * nrc::kNumParams, hipMemcpyDeviceToHost, n->dev->stream));
+    KR_HIP(hipStreamSynchronize(n->dev->stream));
+    return KR_OK;
+}
+
+KR_EXPORT int kr_nrc_get_render_params(kr_nrc *n, float *params) {
+    if (!n || !params) return fail(KR_E_INVALID, "NULL argument");
+    KR_HIP(hipMemcpyAsync(params, n->ema ? n->ema : n->master, sizeof(float) * nrc::kNumParams,
+                          hipMemcpyDeviceToHost, n->dev->stream));
     KR_HIP(hipStreamSynchronize(n->dev->stream));
     return KR_OK;
 }
@@ -3033,7 +3079,7 @@ void nrc_after_chunk(kr_render *r, Pipe &pp) {
     {
         Timed tm(r, K_OTHER, st);
         int grid = int(std::min<size_t>((size_t(r->P) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
-        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_view(n), pp.nr, pp.counters, pp.L);
+        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_render_view(n), pp.nr, pp.counters, pp.L);
         int egrid = int(std::min<size_t>((size_t(pp.nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
         hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, pp.nr, pp.counters, n->pool_raw, n->pool_tgt,
                            n->pool_total, n->pool_cap, n->cfg.target_clamp);

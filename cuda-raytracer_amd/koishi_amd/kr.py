@@ -135,7 +135,7 @@ class BatchInfo(C.Structure):
 class NrcConfig(C.Structure):
     _fields_ = [("box_min", f32 * 3), ("box_max", f32 * 3), ("learning_rate", f32), ("seed", u32),
                 ("batch", u32), ("steps_per_frame", u32), ("train_log2", u32), ("rough_threshold", f32),
-                ("flags", u32), ("target_clamp", f32)]
+                ("flags", u32), ("target_clamp", f32), ("ema_decay", f32)]
 
 
 EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "kr_device_close",
@@ -143,7 +143,7 @@ EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "
            "kr_trace_closest", "kr_trace_shadow", "kr_render_create", "kr_render_run", "kr_render_sync",
            "kr_render_reset", "kr_render_read", "kr_render_get_stats", "kr_render_free",
            "kr_render_radiance", "kr_render_set_profiling", "kr_device_bandwidth", "kr_nrc_create", "kr_nrc_free",
-           "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
+           "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_get_render_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
            "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc", "kr_nrc_read_pool",
            "kr_batch_create", "kr_batch_autotune", "kr_batch_run", "kr_batch_sync", "kr_batch_reset", "kr_batch_read",
            "kr_batch_get_info", "kr_batch_free"]
@@ -187,6 +187,7 @@ def load(path: str = LIB_PATH):
     lib.kr_nrc_create.argtypes = [vp, C.POINTER(NrcConfig), C.POINTER(vp)]
     lib.kr_nrc_free.argtypes = [vp]
     lib.kr_nrc_get_params.argtypes = [vp, vp]
+    lib.kr_nrc_get_render_params.argtypes = [vp, vp]
     lib.kr_nrc_set_params.argtypes = [vp, vp]
     lib.kr_nrc_encode.argtypes = [vp, vp, u32, vp]
     lib.kr_nrc_predict.argtypes = [vp, vp, u32, vp]
@@ -387,10 +388,10 @@ class Nrc:
     """kr_nrc handle: the neural radiance cache on one device."""
 
     def __init__(self, dev: Device, box_min, box_max, seed=5, learning_rate=0.0, batch=0, steps_per_frame=0,
-                 train_log2=0, rough_threshold=0.0, flags=0, target_clamp=0.0):
+                 train_log2=0, rough_threshold=0.0, flags=0, target_clamp=0.0, ema_decay=0.0):
         self.lib = dev.lib
         cfg = NrcConfig((f32 * 3)(*box_min), (f32 * 3)(*box_max), learning_rate, seed, batch, steps_per_frame,
-                        train_log2, rough_threshold, flags, target_clamp)
+                        train_log2, rough_threshold, flags, target_clamp, ema_decay)
         h = C.c_void_p()
         check(self.lib.kr_nrc_create(dev.h, C.byref(cfg), C.byref(h)), "kr_nrc_create")
         self.h = h
@@ -398,6 +399,12 @@ class Nrc:
     def get_params(self):
         p = np.zeros(NRC_PARAMS, np.float32)
         check(self.lib.kr_nrc_get_params(self.h, _ptr(p)), "kr_nrc_get_params")
+        return p
+
+    def get_render_params(self):
+        """The parameters renders query (the moving average of the trained ones)."""
+        p = np.zeros(NRC_PARAMS, np.float32)
+        check(self.lib.kr_nrc_get_render_params(self.h, _ptr(p)), "kr_nrc_get_render_params")
         return p
 
     def set_params(self, p):

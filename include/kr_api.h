@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define KR_API_VERSION 3
+#define KR_API_VERSION 4
 
 enum kr_status {
     KR_OK = 0,
@@ -386,6 +386,10 @@ typedef struct kr_nrc_config {
     float target_clamp;           /* > 0: training targets are clamped to this
                                      radiance per channel (e.g. the brightest
                                      emitter's), 0: unclamped                   */
+    float ema_decay;              /* renders query the bias-corrected moving
+                                     average of the trained parameters with this
+                                     decay per Adam step (0 < d < 1); 0 = 0.99;
+                                     < 0: they query the trained parameters     */
 } kr_nrc_config;
 
 #define KR_NRC_FROZEN 0x1u /* renders query the cache but never train it */
@@ -393,7 +397,12 @@ typedef struct kr_nrc_config {
 int kr_nrc_create(kr_device *dev, const kr_nrc_config *config, kr_nrc **out);
 int kr_nrc_free(kr_nrc *nrc);
 int kr_nrc_get_params(kr_nrc *nrc, float *params);
-int kr_nrc_set_params(kr_nrc *nrc, const float *params); /* also resets Adam */
+int kr_nrc_set_params(kr_nrc *nrc, const float *params); /* also resets Adam
+                                                          and the average */
+/* The parameters renders query: the moving average of the trained ones (the
+ * trained ones themselves when ema_decay < 0). kr_nrc_predict, _gradient and
+ * _train work on the trained ones. */
+int kr_nrc_get_render_params(kr_nrc *nrc, float *params);
 /* Feature vectors (n x KR_NRC_FEATURES floats) of n vertex records. */
 int kr_nrc_encode(kr_nrc *nrc, const float *raw, uint32_t n, float *features);
 /* Cache prediction (n x 3 floats) for n vertex records. */

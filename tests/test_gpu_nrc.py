@@ -225,3 +225,52 @@ def test_cached_render_learns_and_gains(device):
     zero.free()
     trained.free()
     gs.free()
+
+
+@pytest.mark.parametrize("decay", [0.9, -1.0])
+def test_renders_query_the_moving_average(device, decay):
+    """Renders query the bias-corrected moving average of the trained
+    parameters, m_t = m_(t-1) + (p_t - m_(t-1)) (1 - d) / (1 - d^t) over the
+    Adam steps t (m_1 = p_1), recomputed here on the host from the trained
+    parameters after every step; with a negative decay they query the trained
+    parameters. The proof that a render queries exactly these: its image
+    equals the image of a frozen cache set to them."""
+    from koishi_amd import scenegen as sg
+    s = sg.room_scene(width=96, height=64, spp=4, freq=6)
+    gs = kr.GpuScene(device, s.to_desc())
+    a = kr.Nrc(device, *_box(s), train_log2=3, steps_per_frame=1, ema_decay=decay)
+    p0 = a.get_params()
+    assert np.array_equal(a.get_render_params(), p0)
+    r = kr.Render(gs, s.camera_struct(), s.params(seed=4))
+    r.attach_nrc(a)
+    m = p0.copy()
+    for t in range(1, 7):
+        r.reset()
+        r.run(0, s.spp)
+        r.sync()
+        p = a.get_params()
+        assert not np.array_equal(p, p0)  # one Adam step per run
+        if decay > 0:
+            k = np.float32((1.0 - decay) / (1.0 - decay ** t))
+            m = (m + (p - m) * k).astype(np.float32)
+        else:
+            m = p
+        np.testing.assert_allclose(a.get_render_params(), m, rtol=1e-5, atol=1e-7)
+        p0 = p
+    if decay > 0:
+        assert np.abs(m - p).max() > 1e-4  # the average lags the trained parameters
+    # the next run queries the current average (training follows the run)
+    b = kr.Nrc(device, *_box(s), train_log2=3, flags=kr.NRC_FROZEN)
+    b.set_params(a.get_render_params())
+    out = []
+    for n in (a, b):
+        rn = kr.Render(gs, s.camera_struct(), s.params(seed=9))
+        rn.attach_nrc(n)
+        rn.run(0, s.spp)
+        out.append(rn.read())
+        rn.free()
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][0], out[1][0])
+    r.free()
+    a.free()
+    b.free()
+    gs.free()
