@@ -132,7 +132,7 @@ __device__ __forceinline__ bool tri_hit(const ShearRay &r, f3 v0, f3 v1, f3 v2, 
     float det = U + V + W;
     if (det == 0.f) return false;
     float T = U * dot(r.mz, A) + V * dot(r.mz, B) + W * dot(r.mz, C);
-    float inv = 1.0f / det;
+    float inv = __builtin_amdgcn_rcpf(det);  // v_rcp_f32, 1 ulp
     float tt = T * inv;
     if (!(tt > 0.f && tt < tmax)) return false;
     t = tt;
@@ -357,6 +357,29 @@ __device__ __forceinline__ void fetch_node(const float4 *nodes, uint32_t idx, No
         load_node<W>(nodes, idx, nv);
 }
 
+// Sorts the children's (entry distance, ref) pairs ascending by distance with
+// an optimal comparator network: 1, 5 and 19 comparators for W = 2, 4, 8.
+template <int W>
+__device__ __forceinline__ void sort_children(float (&key)[W], uint32_t (&ref)[W]) {
+    auto ce = [&](int i, int j) {
+        bool sw = key[j] < key[i];
+        float ki = sw ? key[j] : key[i], kj = sw ? key[i] : key[j];
+        uint32_t ri = sw ? ref[j] : ref[i], rj = sw ? ref[i] : ref[j];
+        key[i] = ki;
+        key[j] = kj;
+        ref[i] = ri;
+        ref[j] = rj;
+    };
+    if (W == 2) {
+        ce(0, 1);
+    } else if (W == 4) {
+        ce(0, 1), ce(2, 3), ce(0, 2), ce(1, 3), ce(1, 2);
+    } else {
+        ce(0, 2), ce(1, 3), ce(4, 6), ce(5, 7), ce(0, 4), ce(1, 5), ce(2, 6), ce(3, 7), ce(0, 1), ce(2, 3);
+        ce(4, 5), ce(6, 7), ce(2, 4), ce(3, 5), ce(1, 4), ce(3, 6), ce(1, 2), ce(3, 4), ce(5, 6);
+    }
+}
+
 // Closest hit over the wide BVH and the sphere list: ordered traversal,
 // nearest child first, far children pushed with their entry distance and
 // culled on pop. Equivalent to the reference's primitive loop + per-mesh DFS
@@ -398,34 +421,26 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
             uint32_t ref[W];
 #pragma unroll
             for (int c = 0; c < W; ++c) {
-                float tn = Q ? box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
-                                         nv.hi[2][c], h.t)
-                             : box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
-                                            nv.hi[2][c], h.t);
-                bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
-                key[c] = ok ? tn : __builtin_inff();
-                ref[c] = ok ? nv.ref[c] : kEmpty;
-            }
-            // sort (key, ref) ascending: insertion network over W entries
-#pragma unroll
-            for (int i = 1; i < W; ++i) {
-#pragma unroll
-                for (int j = i; j > 0; --j) {
-                    bool sw = key[j] < key[j - 1];
-                    float tk = sw ? key[j] : key[j - 1];
-                    float tk2 = sw ? key[j - 1] : key[j];
-                    uint32_t tr = sw ? ref[j] : ref[j - 1];
-                    uint32_t tr2 = sw ? ref[j - 1] : ref[j];
-                    key[j - 1] = tk;
-                    key[j] = tk2;
-                    ref[j - 1] = tr;
-                    ref[j] = tr2;
+                if (Q) {
+                    float tn = box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                         nv.hi[2][c], h.t);
+                    bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
+                    key[c] = ok ? tn : __builtin_inff();
+                } else {
+                    // an empty slot holds the box (+inf, -inf), whose entering
+                    // plane lies at +inf along every axis: box_enter_nf
+                    // returns +inf for it like for any missed child
+                    key[c] = box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                          nv.hi[2][c], h.t);
                 }
+                ref[c] = nv.ref[c];
             }
+            sort_children<W>(key, ref);
+            // missed children sort last, with key +inf
 #pragma unroll
             for (int c = W - 1; c >= 1; --c)
-                if (ref[c] != kEmpty) st.push(ref[c], key[c]);
-            if (ref[0] != kEmpty) {
+                if (key[c] != __builtin_inff()) st.push(ref[c], key[c]);
+            if (key[0] != __builtin_inff()) {
                 cur = ref[0];
                 continue;
             }
@@ -508,7 +523,8 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
                                          nv.hi[2][c], limit)
                              : box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
                                             nv.hi[2][c], limit);
-                if (nv.ref[c] != kEmpty && tn != __builtin_inff()) {
+                // (box_enter_nf never enters an empty slot's (+inf, -inf) box)
+                if ((Q ? nv.ref[c] != kEmpty : true) && tn != __builtin_inff()) {
                     if (next == kEmpty)
                         next = nv.ref[c];
                     else
@@ -604,21 +620,7 @@ __device__ __forceinline__ void closest_step(const DevScene &S, Stack<LDS_DEPTH>
             key[c] = ok ? tn : __builtin_inff();
             ref[c] = ok ? nv.ref[c] : kEmpty;
         }
-#pragma unroll
-        for (int i = 1; i < W; ++i) {
-#pragma unroll
-            for (int j = i; j > 0; --j) {
-                bool sw = key[j] < key[j - 1];
-                float tk = sw ? key[j] : key[j - 1];
-                float tk2 = sw ? key[j - 1] : key[j];
-                uint32_t tr = sw ? ref[j] : ref[j - 1];
-                uint32_t tr2 = sw ? ref[j - 1] : ref[j];
-                key[j - 1] = tk;
-                key[j] = tk2;
-                ref[j - 1] = tr;
-                ref[j] = tr2;
-            }
-        }
+        sort_children<W>(key, ref);
 #pragma unroll
         for (int c = W - 1; c >= 1; --c)
             if (ref[c] != kEmpty) st.push(ref[c], key[c]);
