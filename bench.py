@@ -89,17 +89,35 @@ def rmse_vs_oracle(workload, rgb, valid, width):
             "pixels": int(pix.size), "oracle_samples_per_pixel": int(f["seeds"]) * int(f["spp"])}
 
 
+# The cache's two ways of ending paths (kr_nrc_config.flags) and the settings
+# measured best for each. "second": paths end at their second diffuse or rough
+# vertex and the cache stands for everything that vertex scatters (Mueller et
+# al. 2021), the library's default. "first" (KR_NRC_INDIRECT): paths end at
+# their first diffuse or rough vertex after its light sample and the cache
+# stands for what the rest of the path brings; the first bounce carries most
+# of the noise in both scenes, and on the large scene the second vertex is
+# often never reached. "first" with 4 steps per frame and rough threshold 0.3
+# reached the large scene's target at 48 spp and never the room's; with 8
+# steps and threshold 0.6 (glossy surfaces below it keep their paths going)
+# at 24 spp and 8 spp.
+CACHE_MODES = {
+    "second": {"flags": 0, "steps_per_frame": 4, "rough_threshold": 0.3},
+    "first": {"flags": 0x2, "steps_per_frame": 8, "rough_threshold": 0.6},  # flags: KR_NRC_INDIRECT
+}
+
+
 def bench_cache(args, kr, scene, gs, p, dev):
     """Radiance cache on the workload against the plain path. The cache
-    (1/16 of paths train, 4 Adam steps of 16384 samples per frame = 2^16
-    samples, lr 3e-2, renders query the moving average of decay 0.99 per step,
-    targets clamped at the brightest emitter's radiance) is
-    trained for a number of 4-spp warm-up frames, then frames of s = 4, 8, 16,
-    then steps of spp/8 up to spp samples are timed for both paths, with RMSE
-    against the FP64 oracle averaged over 3 seeds. The target is the plain path's clamped RMSE at the
-    workload's spp; the cached wall-clock to it is the time of the cached
-    frame at the smallest spp from which on every cached frame reaches it
-    (null when the cache's bias lies above)."""
+    (1/16 of paths train, Adam steps of 16384 samples after each frame, lr
+    3e-2, renders query the moving average of decay 0.99 per step, targets
+    clamped at the brightest emitter's radiance; ending mode per CACHE_MODES)
+    is trained for a number of 4-spp warm-up frames, then frames of s = 4, 8,
+    16, then steps of spp/8 up to spp samples are timed for both paths, with
+    RMSE against the FP64 oracle averaged over 3 seeds. The target is the
+    plain path's clamped RMSE at the workload's spp; the cached wall-clock to
+    it is the time of the cached frame at the smallest spp from which on
+    every cached frame reaches it (null when the cache's bias lies above).
+    The workload's mode is reported in full, the other mode in summary."""
     import numpy as np
     pos = np.asarray(scene.positions, np.float64)
     lo, hi = pos.min(axis=0) - 0.01, pos.max(axis=0) + 0.01
@@ -108,11 +126,8 @@ def bench_cache(args, kr, scene, gs, p, dev):
     emax = max([max(c.get("color", [1, 1, 1])) * c.get("strength", 2.0)
                 for t, c in ((v[0], v[1] if len(v) > 1 else {}) for v in scene.shaders.values()) if t == "Emission"]
                + [max(col) * k for _, col, k in scene.point_lights] + [1.0])
-    # lr 3e-2 with the averaged weights: on the closed room 1e-2 reached the
-    # target at 40 spp, 3e-2 at 32 spp, 5e-2 gave no further gain
-    nrc = kr.Nrc(dev, lo, hi, train_log2=4, steps_per_frame=4, batch=16384, target_clamp=emax,
-                 learning_rate=3e-2, ema_decay=0.99)
     cam = scene.camera_struct()
+    mode = args.cache_mode if args.cache_mode != "auto" else "first"
 
     fx = np.load(os.path.join(ROOT, "tests", "golden", f"oracle_{args.workload}.npz"))
     fpix, fref = fx["pixels"].astype(np.int64), fx["mean"]
@@ -126,7 +141,7 @@ def bench_cache(args, kr, scene, gs, p, dev):
         rgb, val = r.read()
         return dt, rgb[fpix] / np.maximum(val[fpix], 1)[:, None]
 
-    def over_seeds(mode, s):
+    def over_seeds(nrc, s):
         """Best time, RMSE against the oracle averaged over 3 independent seeds
         (as MSE), and the bias: the error of the 3-seed mean minus its own
         noise, b^2 = |mean - ref|^2 - var / 3 (var from the spread of the
@@ -134,7 +149,7 @@ def bench_cache(args, kr, scene, gs, p, dev):
         dts, imgs = [], []
         for seed in (11, 12, 13):
             r = kr.Render(gs, cam, scene.params(seed=seed, spp=s))
-            if mode == "cached":
+            if nrc is not None:
                 r.attach_nrc(nrc)
             frame(r, s)
             dt, img = frame(r, s)
@@ -151,55 +166,71 @@ def bench_cache(args, kr, scene, gs, p, dev):
             out["bias" + key] = float(np.sign(b2) * np.sqrt(abs(b2)))
         return out
 
-    warm = 0
-    t_warm = time.perf_counter()
-    rw = kr.Render(gs, cam, scene.params(seed=7, spp=4))
-    rw.attach_nrc(nrc)
-    for warm in range(1, args.cache_warmup + 1):
-        rw.reset()
-        rw.run(0, 4)
-    rw.sync()
-    t_warm = time.perf_counter() - t_warm
-    rw.free()
-    log(f"[bench] cache warm-up: {warm} frames of 4 spp in {t_warm:.2f}s, loss {nrc.stats()[1]:.3f}")
-    rows = []
     # 4, 8, 16, then steps of spp/8 up to the workload's spp
     ladder = sorted({s for s in [4, 8, 16] + [p.spp * k // 8 for k in range(2, 9)] if 1 <= s <= p.spp})
-    for s in ladder:
-        res = {"spp": s}
-        for mode in ("plain", "cached"):
-            res[mode] = over_seeds(mode, s)
-        log(f"[bench] spp {s}: plain {res['plain']['seconds'] * 1e3:.1f} ms rmse {res['plain']['rmse']:.4f} / "
-            f"{res['plain']['rmse_clamped']:.4f} clamped, cached {res['cached']['seconds'] * 1e3:.1f} ms rmse "
-            f"{res['cached']['rmse']:.4f} / {res['cached']['rmse_clamped']:.4f} clamped")
-        rows.append(res)
-    nrc.free()
+    plain = {s: over_seeds(None, s) for s in ladder}
     # the target: the plain frame's clamped RMSE at the workload's spp (what the
     # reference's 8-bit output shows; the unclamped RMSE of these scenes is
     # dominated by rare very bright samples and is reported beside it)
-    target = rows[-1]["plain"]["rmse_clamped"] if rows else None
+    target = plain[ladder[-1]]["rmse_clamped"]
+    plain_time = plain[ladder[-1]]["seconds"]
 
-    # the cached frame reaches the target at the first spp from which every
-    # larger spp also stays at or below it (single frames are noisy, and the
-    # cache keeps training between them)
-    cached_time = None
-    for k, r in enumerate(rows):
-        if target is not None and all(x["cached"]["rmse_clamped"] <= target for x in rows[k:]):
-            cached_time = r["cached"]["seconds"]
-            break
-    plain_time = rows[-1]["plain"]["seconds"] if rows else None
-    return {
-        "target_rmse_clamped": target,
-        "target_clamp": emax,
-        "plain_seconds_to_target": plain_time,
-        "cached_seconds_to_target": cached_time,
-        "speedup": (plain_time / cached_time) if cached_time else None,
-        "warmup_frames": warm, "warmup_seconds": t_warm,
-        "train_fraction": 1 / 16, "samples_per_frame": 4 * 16384,
-        "bias_at_full_spp": {m: {"unclamped": rows[-1][m]["bias"], "clamped": rows[-1][m]["bias_clamped"]}
-                             for m in ("plain", "cached")} if rows else None,
-        "by_spp": rows,
+    def cached_ladder(m):
+        cfg = CACHE_MODES[m]
+        nrc = kr.Nrc(dev, lo, hi, train_log2=4, batch=16384, target_clamp=emax, learning_rate=3e-2, ema_decay=0.99,
+                     **cfg)
+        warm = 0
+        t_warm = time.perf_counter()
+        rw = kr.Render(gs, cam, scene.params(seed=7, spp=4))
+        rw.attach_nrc(nrc)
+        for warm in range(1, args.cache_warmup + 1):
+            rw.reset()
+            rw.run(0, 4)
+        rw.sync()
+        t_warm = time.perf_counter() - t_warm
+        rw.free()
+        log(f"[bench] cache ({m}) warm-up: {warm} frames of 4 spp in {t_warm:.2f}s, loss {nrc.stats()[1]:.3f}")
+        rows = []
+        for s in ladder:
+            res = {"spp": s, "plain": plain[s], "cached": over_seeds(nrc, s)}
+            log(f"[bench] {m} spp {s}: plain {res['plain']['seconds'] * 1e3:.1f} ms rmse "
+                f"{res['plain']['rmse']:.4f} / {res['plain']['rmse_clamped']:.4f} clamped, cached "
+                f"{res['cached']['seconds'] * 1e3:.1f} ms rmse {res['cached']['rmse']:.4f} / "
+                f"{res['cached']['rmse_clamped']:.4f} clamped")
+            rows.append(res)
+        nrc.free()
+        # the cached frame reaches the target at the first spp from which every
+        # larger spp also stays at or below it (single frames are noisy, and the
+        # cache keeps training between them)
+        cached_time = None
+        for k, r in enumerate(rows):
+            if all(x["cached"]["rmse_clamped"] <= target for x in rows[k:]):
+                cached_time = r["cached"]["seconds"]
+                break
+        return {
+            "mode": m, "settings": {**cfg, "train_fraction": 1 / 16, "batch": 16384, "learning_rate": 3e-2,
+                                    "ema_decay": 0.99},
+            "cached_seconds_to_target": cached_time,
+            "speedup": (plain_time / cached_time) if cached_time else None,
+            "warmup_frames": warm, "warmup_seconds": t_warm,
+            "samples_per_frame": cfg["steps_per_frame"] * 16384,
+            "bias_at_full_spp": {w: {"unclamped": rows[-1][w]["bias"], "clamped": rows[-1][w]["bias_clamped"]}
+                                 for w in ("plain", "cached")},
+            "by_spp": rows,
+        }
+
+    main_res = cached_ladder(mode)
+    other = cached_ladder("second" if mode == "first" else "first")
+    out = {"target_rmse_clamped": target, "target_clamp": emax, "plain_seconds_to_target": plain_time}
+    out.update(main_res)
+    out["other_mode"] = {
+        "mode": other["mode"], "settings": other["settings"], "speedup": other["speedup"],
+        "cached_seconds_to_target": other["cached_seconds_to_target"],
+        "bias_at_full_spp": other["bias_at_full_spp"]["cached"],
+        "by_spp": [{"spp": r["spp"], "seconds": r["cached"]["seconds"], "rmse_clamped": r["cached"]["rmse_clamped"]}
+                   for r in other["by_spp"]],
     }
+    return out
 
 
 def bench_batch(args, kr, sg, barrier):
@@ -357,6 +388,8 @@ def main():
                          "against the plain path at equal RMSE")
     ap.add_argument("--cache-warmup", type=int, default=64,
                     help="4-spp training frames before the cached frames are timed")
+    ap.add_argument("--cache-mode", default="auto", choices=["auto", "second", "first"],
+                    help="where cached paths end (CACHE_MODES); auto: first")
     ap.add_argument("--persistent-traversal", action="store_true",
                     help="traversal kernels refill finished lanes from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)

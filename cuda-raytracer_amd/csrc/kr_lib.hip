@@ -157,6 +157,10 @@ struct NrcRender {
     uint32_t train_log2;   // one path in every group of 2^train_log2 consecutive paths trains
     uint32_t train_mask;   // 2^train_log2 - 1
     float rough_threshold; // GlossyBSDF roughness counted as rough
+    uint32_t end_at;       // a path ends at its end_at-th diffuse or rough vertex
+    int indirect;          // the cache holds what a vertex reflects from its
+                           // continuation, without its own light sample and
+                           // emission (KR_NRC_INDIRECT)
     uint32_t *tslot_of;    // slot -> training-path index or kMissCode
     float *rec;            // training vertex records [train_cap][nv][kRec]
     uint32_t *tcount;      // vertices recorded per training path
@@ -793,6 +797,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
             bool touch_L = false;
             // radiance cache: training paths record this vertex; other paths
             // end at their second diffuse or rough vertex with a cache query
+            // (KR_NRC_INDIRECT: at their first, after its light sample)
             uint32_t flags = a.q.flags[i];
             bool train = (flags & 1u) != 0;
             uint32_t ndr = (flags >> 1) & 3u;
@@ -804,7 +809,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
                           ((sh.type == KR_SHADER_GLOSSY || sh.type == KR_SHADER_GLASS ||
                             sh.type == KR_SHADER_REFRACTION) &&
                            sh.dist == KR_DIST_BECKMANN && sh.roughness >= a.nr.rough_threshold);
-                if (!train && dr && ++ndr >= 2u) terminate = true;
+                if (!train && dr && ++ndr >= a.nr.end_at) terminate = true;
                 if (train) {
                     uint32_t ts = a.nr.tslot_of[slot];
                     rec = a.nr.rec + (size_t(ts) * a.nr.nv + a.bounce) * kRecFloats;
@@ -823,7 +828,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
                     }
                 }
             }
-            if (terminate) {
+            if (terminate && !a.nr.indirect) {
                 // emission as usual; the cache stands for everything scattered here
                 if (sh.type == KR_SHADER_EMISSION && S.nlights) {
                     f3 e = beta * f3{sh.emission[0], sh.emission[1], sh.emission[2]};
@@ -925,6 +930,12 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
                 pr[12] = acosf(dot(wo, f3{-wi_v.x, -wi_v.y, wi_v.z}));
                 pr[13] = __uint_as_float(sid);
             }
+            if (terminate) {
+                // KR_NRC_INDIRECT: light sample and emission done above; the
+                // cache stands for what the BSDF-sampled continuation brings
+                push_query = true;
+                nb = beta;
+            } else {
             // BSDF sample (radiance.cu:56-60)
             float ucs;
             uint32_t sample_leaf = resolve_leaf(S, sid, wo, pixel, sample, 2 * a.bounce + 1, seed_lo, seed_hi, ucs);
@@ -960,7 +971,8 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
                 nb = beta;
                 nflags = (train ? 1u : 0u) | (min(ndr, 3u) << 1);
             }
-            }  // not terminated
+            }  // continued
+            }  // not terminated before the light sample
         }
         if (a.nr.enabled) {
             uint32_t pq = wave_append(&a.counters[CTR(C_NRCQ)], push_query);
@@ -3018,7 +3030,8 @@ __global__ __launch_bounds__(256, KR_NRC_INFER_WAVES) void k_nrc_infer(nrc::Weig
 
 // Training targets of the chunk's training paths: for vertex j the radiance
 // the rest of the path carried back to it, (sum_{k>=j} c_k - beta_j E_j) / beta_j,
-// where c_k is everything vertex k added to the path's L.
+// where c_k is everything vertex k added to the path's L; with nr.indirect
+// what the vertices after j carried back, (sum_{k>j} c_k) / beta_j.
 __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_t *counters, float *pool_raw,
                                                      float *pool_tgt, uint32_t *pool_total, uint32_t cap,
                                                      float clamp) {
@@ -3040,7 +3053,8 @@ __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_
                 sz += rec[21];
                 float bx = rec[16], by = rec[17], bz = rec[18];
                 if (!(bx > 0.f && by > 0.f && bz > 0.f)) continue;
-                float tx = (sx - rec[22]) / bx, ty = (sy - rec[23]) / by, tz = (sz - rec[24]) / bz;
+                const int own = nr.indirect ? 19 : 22;
+                float tx = (sx - rec[own]) / bx, ty = (sy - rec[own + 1]) / by, tz = (sz - rec[own + 2]) / bz;
                 if (isfinite(tx) && isfinite(ty) && isfinite(tz)) ++keep;
             }
         }
@@ -3054,7 +3068,8 @@ __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_
             sz += rec[21];
             float bx = rec[16], by = rec[17], bz = rec[18];
             if (!(bx > 0.f && by > 0.f && bz > 0.f)) continue;
-            float tx = (sx - rec[22]) / bx, ty = (sy - rec[23]) / by, tz = (sz - rec[24]) / bz;
+            const int own = nr.indirect ? 19 : 22;
+            float tx = (sx - rec[own]) / bx, ty = (sy - rec[own + 1]) / by, tz = (sz - rec[own + 2]) / bz;
             if (!(isfinite(tx) && isfinite(ty) && isfinite(tz))) continue;
             if (clamp > 0.f) {
                 tx = fminf(tx, clamp);
@@ -3125,6 +3140,8 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     nr.train_log2 = log2;               // <= 31 (kr_nrc_create)
     nr.train_mask = (1u << log2) - 1u;
     nr.rough_threshold = n->cfg.rough_threshold;
+    nr.indirect = (n->cfg.flags & KR_NRC_INDIRECT) ? 1 : 0;
+    nr.end_at = nr.indirect ? 1u : 2u;
     nr.nv = r->prm.max_bounce + 1;
     // groups a chunk of P paths can touch: P / 2^log2 + 2 partial ones
     nr.train_cap = uint32_t(std::min<uint64_t>(r->P, (uint64_t(r->P) >> log2) + 2));

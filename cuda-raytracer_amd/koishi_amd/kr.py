@@ -111,6 +111,7 @@ KR_PROFILE_COUNTERS = 0x2
 NRC_RAW, NRC_FEATURES, NRC_PARAMS = 16, 96, 23888
 NRC_TRAIN_ALL = 0xFFFFFFFF  # Nrc(train_log2=...): every path trains
 NRC_FROZEN = 0x1  # Nrc(flags=...): renders query the cache but never train it
+NRC_INDIRECT = 0x2  # Nrc(flags=...): the cache holds a vertex's reflected continuation; paths end at the first
 
 
 class BatchJob(C.Structure):

@@ -393,6 +393,13 @@ typedef struct kr_nrc_config {
 } kr_nrc_config;
 
 #define KR_NRC_FROZEN 0x1u /* renders query the cache but never train it */
+/* The cache holds the radiance a vertex reflects from its BSDF-sampled
+ * continuation (everything the later vertices of a path carry back), not its
+ * own light sample and emission; paths end at their first diffuse or rough
+ * vertex, after its light sample, with a query. Without the flag the cache
+ * holds everything a vertex scatters except its emission, and paths end at
+ * their second diffuse or rough vertex without a light sample there. */
+#define KR_NRC_INDIRECT 0x2u
 
 int kr_nrc_create(kr_device *dev, const kr_nrc_config *config, kr_nrc **out);
 int kr_nrc_free(kr_nrc *nrc);

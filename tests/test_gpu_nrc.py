@@ -176,11 +176,88 @@ def test_training_targets_are_path_radiance(device):
     gs.free()
 
 
+def _floor_light_scene(width=48, height=32, max_bounce=0):
+    """A Lambert floor lit by an Emission quad: every vertex is diffuse."""
+    from koishi_amd import scenegen as sg
+    b = sg.Builder()
+    b.shaders["floor"] = ["Lambert", {"color": [0.6, 0.5, 0.4]}]
+    b.shaders["light"] = ["Emission", {"color": [1.0, 1.0, 1.0], "strength": 20.0}]
+    v, nn, t = sg.quad((0.0, 0.0, 0.0), 8.0, 8.0, 1.0)
+    b.add_mesh("floor", "floor", v, nn, t)
+    v, nn, t = sg.quad((0.7, 2.0, -0.3), 1.5, 1.0, -1.0)
+    b.add_mesh("light", "light", v, nn, t)
+    return b.finish((0.3, 1.0, 0.2), (0.35, 0.0, 0.25), 60.0, width, height, 1, max_bounce, "nee")
+
+
+def _one_spp(device, gs, s, p, nrc=None):
+    r = kr.Render(gs, s.camera_struct(), p)
+    if nrc is not None:
+        r.attach_nrc(nrc)
+    r.run(0, 1)
+    r.sync()
+    out = r.read()
+    r.free()
+    return out
+
+
+def test_indirect_targets_are_the_continuation(device):
+    """KR_NRC_INDIRECT targets leave out the vertex's own light sample and
+    emission. maxBounce 1, 1 spp, camera throughput 1: the first vertex's
+    target is what the second vertex added, i.e. the pixel of the 2-vertex
+    render minus the pixel of the 1-vertex render (same RNG streams); the
+    second vertex's target is 0."""
+    s = _floor_light_scene(max_bounce=1)
+    gs = kr.GpuScene(device, s.to_desc())
+    p1 = s.params(seed=3)
+    p0 = s.params(seed=3)
+    p0.max_bounce = 0
+    rgb0, _ = _one_spp(device, gs, s, p0)
+    n = kr.Nrc(device, *_box(s), train_log2=kr.NRC_TRAIN_ALL, steps_per_frame=1, flags=kr.NRC_INDIRECT)
+    rgb1, val1 = _one_spp(device, gs, s, p1, n)
+    raw, tgt = n.read_pool(1 << 16)
+    cont = (rgb1 - rgb0)[val1 == 1]
+    a = np.sort(cont[cont.max(axis=1) > 0].astype(np.float32), axis=0)
+    bb = np.sort(tgt[tgt.max(axis=1) > 0], axis=0)
+    assert a.shape == bb.shape and a.shape[0] > 50
+    assert np.allclose(a, bb, rtol=1e-5, atol=1e-5)
+    # as many records as vertices: first vertices and the second ones
+    assert raw.shape[0] >= 2 * a.shape[0] - 1
+    n.free()
+    gs.free()
+
+
+def test_indirect_paths_end_at_the_first_diffuse_vertex(device):
+    """With KR_NRC_INDIRECT and an all-zero frozen cache a path ends after its
+    first diffuse vertex's light sample and emission: on a scene whose
+    vertices are all diffuse the image is the 1-vertex render's, except for
+    the training paths (one per group of 2^30 paths at most: 2 here)."""
+    s = _floor_light_scene(max_bounce=8)
+    gs = kr.GpuScene(device, s.to_desc())
+    p0 = s.params(seed=5)
+    p0.max_bounce = 0
+    rgb0, val0 = _one_spp(device, gs, s, p0)
+    n = kr.Nrc(device, *_box(s), train_log2=30, flags=kr.NRC_FROZEN | kr.NRC_INDIRECT)
+    n.set_params(np.zeros(kr.NRC_PARAMS, np.float32))
+    rgb, val = _one_spp(device, gs, s, s.params(seed=5), n)
+    assert np.array_equal(val, val0)
+    differ = np.any(rgb != rgb0, axis=1)
+    assert differ.sum() <= 2, differ.sum()
+    # without the flag the same cache ends paths at their second vertex
+    m = kr.Nrc(device, *_box(s), train_log2=30, flags=kr.NRC_FROZEN)
+    m.set_params(np.zeros(kr.NRC_PARAMS, np.float32))
+    rgb2, _ = _one_spp(device, gs, s, s.params(seed=5), m)
+    assert np.any(rgb2 != rgb0, axis=1).sum() > 10
+    n.free()
+    m.free()
+    gs.free()
+
+
 def _mean_image(rgb, val):
     return rgb / np.maximum(val, 1)[:, None]
 
 
-def test_cached_render_learns_and_gains(device):
+@pytest.mark.parametrize("flags", [0, kr.NRC_INDIRECT])
+def test_cached_render_learns_and_gains(device, flags):
     """What the trained cache gains over the method without it. Paths end at
     their second diffuse or rough vertex in both cached renders; a frozen
     all-zero network stands for "no cache" (the cut-off radiance is dropped),
@@ -200,9 +277,9 @@ def test_cached_render_learns_and_gains(device):
         plain.free()
     ref /= 8
 
-    zero = kr.Nrc(device, *_box(s), train_log2=5, flags=kr.NRC_FROZEN)
+    zero = kr.Nrc(device, *_box(s), train_log2=5, flags=kr.NRC_FROZEN | flags)
     zero.set_params(np.zeros(kr.NRC_PARAMS, np.float32))
-    trained = kr.Nrc(device, *_box(s), train_log2=5, steps_per_frame=16)
+    trained = kr.Nrc(device, *_box(s), train_log2=5, steps_per_frame=16, flags=flags)
     out = {}
     for name, n in (("zero", zero), ("trained", trained)):
         r = kr.Render(gs, s.camera_struct(), p)
