@@ -390,6 +390,8 @@ def main():
                     help="4-spp training frames before the cached frames are timed")
     ap.add_argument("--cache-mode", default="auto", choices=["auto", "second", "first"],
                     help="where cached paths end (CACHE_MODES); auto: first")
+    ap.add_argument("--no-material-sort", action="store_true",
+                    help="shade every hit from one mixed-material queue")
     ap.add_argument("--persistent-traversal", action="store_true",
                     help="traversal kernels refill finished lanes from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -426,7 +428,8 @@ def main():
     log(f"[bench] scene: {info.num_triangles} tris, {info.num_spheres} spheres, {info.num_lights} lights, "
         f"BVH{info.bvh_width} {info.bvh_nodes} nodes depth {info.bvh_depth}, build {info.build_seconds:.2f}s")
     p = scene.params(seed=1, chunk_paths=args.chunk_paths,
-                     flags=kr.KR_RENDER_PERSISTENT_TRAVERSAL if args.persistent_traversal else 0)
+                     flags=(kr.KR_RENDER_PERSISTENT_TRAVERSAL if args.persistent_traversal else 0)
+                     | (kr.KR_RENDER_NO_MATERIAL_SORT if args.no_material_sort else 0))
     rend = kr.Render(gs, scene.camera_struct(), p)
     paths_per_step = p.width * p.height * p.spp
 
@@ -471,7 +474,7 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "k_trace_mesh_traffic.json")
     # the committed counter summary is for the default mesh frame only
     if (args.workload == "mesh" and info.bvh_width == 4 and not args.chunk_paths and not args.compressed_nodes
-            and not args.persistent_traversal and os.path.exists(tpath)):
+            and not args.persistent_traversal and not args.no_material_sort and os.path.exists(tpath)):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
 

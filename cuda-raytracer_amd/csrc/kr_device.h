@@ -164,8 +164,7 @@ template <int LDS_DEPTH>
 struct Stack {
     uint32_t *lds_ref;  // [LDS_DEPTH][blockDim]
     float *lds_t;
-    uint32_t *spill_ref;  // per lane, (cap - LDS_DEPTH) entries, stride = lanes
-    float *spill_t;
+    uint2 *spill;       // this lane's (ref, t) entries beyond LDS_DEPTH, stride gstride
     uint32_t lane, lanes, gstride;
     int sp = 0;
     __device__ __forceinline__ void push(uint32_t ref, float t) {
@@ -173,9 +172,7 @@ struct Stack {
             lds_ref[sp * lanes + lane] = ref;
             lds_t[sp * lanes + lane] = t;
         } else {
-            size_t k = size_t(sp - LDS_DEPTH) * gstride;
-            spill_ref[k] = ref;
-            spill_t[k] = t;
+            spill[size_t(sp - LDS_DEPTH) * gstride] = make_uint2(ref, __float_as_uint(t));
         }
         ++sp;
     }
@@ -185,9 +182,9 @@ struct Stack {
             ref = lds_ref[sp * lanes + lane];
             t = lds_t[sp * lanes + lane];
         } else {
-            size_t k = size_t(sp - LDS_DEPTH) * gstride;
-            ref = spill_ref[k];
-            t = spill_t[k];
+            uint2 e = spill[size_t(sp - LDS_DEPTH) * gstride];
+            ref = e.x;
+            t = __uint_as_float(e.y);
         }
     }
 };

@@ -251,10 +251,18 @@ struct kr_nrc {
 };
 
 // One set of chunk buffers and the stream its kernels run on. A render has
-// two pipes when its frames are large: consecutive chunks alternate between
-// them, so the short late bounces of one chunk run beside the long early
-// bounces of the next instead of leaving the GPU half idle; only the
-// accumulations into the shared image are ordered between the pipes.
+// KR_RENDER_PIPES pipes when its frames are large: consecutive chunks cycle
+// through them, so the short late bounces of one chunk run beside the long
+// early bounces of the others instead of leaving the GPU half idle; only the
+// accumulations into the shared image are ordered between the pipes. Four
+// pipes, one per hardware queue HIP gives a process by default, measured
+// 1.16x (100k triangles) and 1.10x (1M) the frame rate of two; six and eight
+// share those queues and were slower than four.
+#ifndef KR_RENDER_PIPES
+#define KR_RENDER_PIPES 4
+#endif
+constexpr int kMaxPipes = 4;
+static_assert(KR_RENDER_PIPES >= 1 && KR_RENDER_PIPES <= kMaxPipes, "pipes per render");
 struct Pipe {
     RayQ q[2]{};
     HitQ hit{};
@@ -279,9 +287,9 @@ struct kr_render {
     LensDev lens{};
     uint32_t P = 0;  // paths per chunk
     uint32_t npix = 0;
-    Pipe pipe[2];
+    Pipe pipe[kMaxPipes];
     int npipes = 1;
-    hipEvent_t fork = nullptr;  // pipe 1 starts after what pipe 0 has queued
+    hipEvent_t fork = nullptr;  // pipes 1.. start after what pipe 0 has queued
     unsigned long long *stats = nullptr;
     double *acc = nullptr;
     uint32_t *valid = nullptr;
@@ -387,8 +395,7 @@ __device__ __forceinline__ Stack<LDS> make_stack(uint32_t *lds_ref, float *lds_t
     st.lanes = blockDim.x;
     size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     st.gstride = uint32_t(spill_lanes);
-    st.spill_ref = spill + g;
-    st.spill_t = reinterpret_cast<float *>(spill + spill_lanes * kSpillDepth) + g;
+    st.spill = reinterpret_cast<uint2 *>(spill) + g;
     return st;
 }
 
@@ -1616,22 +1623,28 @@ int upload(kr_scene *s, const T *host, size_t n, const T **dst) {
 // (registers), so most blocks queue behind resident ones: a block that
 // finishes early is replaced, which balances the uneven cost of rays. 64
 // blocks per CU was measured 21 % faster than 8 on the 100k-triangle frame
-// and 13 % on the 1M-triangle frame; the price is the spill stack, 784 B per
-// lane of the grid (3.3 GB on 256 CUs).
+// and 13 % on the 1M-triangle frame; the price is the spill stack of every
+// lane of the grid, 8 B per entry the scene's tree depth can need beyond the
+// LDS part (32 entries, 1.1 GB on 256 CUs, for a BVH4 of depth 14).
 #ifndef KR_BLOCKS_PER_CU
 #define KR_BLOCKS_PER_CU 64
 #endif
 size_t grid_lanes(const kr_device *dev) { return size_t(std::max(dev->num_cu, 1)) * KR_BLOCKS_PER_CU * kBlock; }
 
-// Renders of at least this many paths get a second pipe (struct Pipe).
+// Renders of at least this many paths get KR_RENDER_PIPES pipes (struct Pipe).
 constexpr uint64_t kPipeMinPaths = 8u << 20;
 
+// Spill entries a traversal of a scene with this stack capacity can need.
+uint32_t spill_depth(uint32_t stack_cap) { return stack_cap > uint32_t(kLdsDepth) ? stack_cap - kLdsDepth : 1u; }
+
+// Spill stacks: `depth` (ref, t) entries per lane beyond the LDS part,
+// entry k of lane g at k * lanes + g; depth 0 = the largest a scene may need.
 int ensure_spill(uint32_t *&spill, size_t &lanes, std::vector<void *> &allocs, const kr_device *dev,
-                 size_t want_lanes = 0) {
+                 size_t want_lanes = 0, uint32_t depth = 0) {
     if (spill) return KR_OK;
     lanes = want_lanes ? want_lanes : grid_lanes(dev);
-    // kSpillDepth stack entries (ref + t) per lane beyond the LDS part
-    int rc = dalloc(&spill, lanes * kSpillDepth * 2);
+    if (depth == 0 || depth > uint32_t(kSpillDepth)) depth = kSpillDepth;
+    int rc = dalloc(&spill, lanes * depth * 2);
     if (rc) return rc;
     allocs.push_back(spill);
     return KR_OK;
@@ -1964,7 +1977,7 @@ KR_EXPORT int kr_trace_closest(kr_scene *s, const kr_rays *rays, uint32_t n, kr_
     if (n == 0) return KR_OK;
     (void)hipSetDevice(s->dev->ordinal);
     hipStream_t stream = s->dev->stream;
-    int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev);
+    int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev, 0, spill_depth(s->view.stack_cap));
     if (rc) return rc;
     DevBufs b;
     const float *o = rays->origin, *d = rays->direction;
@@ -2011,7 +2024,7 @@ KR_EXPORT int kr_trace_shadow(kr_scene *s, const kr_rays *rays, uint32_t n, uint
     if (n == 0) return KR_OK;
     (void)hipSetDevice(s->dev->ordinal);
     hipStream_t stream = s->dev->stream;
-    int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev);
+    int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev, 0, spill_depth(s->view.stack_cap));
     if (rc) return rc;
     DevBufs b;
     const float *o = rays->origin, *d = rays->direction, *tm = rays->tmax;
@@ -2111,9 +2124,10 @@ int check_view(const kr_camera *cam, const kr_params *prm) {
 
 // Queues for r->P paths per pipe (r->npipes of them), accumulators for
 // r->npix pixels, counters, the traversal spill stack for `lanes` lanes per
-// pipe; r->grid = lanes / kBlock. Pipe 0 runs on the device's stream, pipe 1
-// on a stream of its own (default priority).
-int render_buffers(kr_render *r, size_t lanes) {
+// pipe (spill_entries per lane, 0 = the most any scene may need); r->grid =
+// lanes / kBlock. Pipe 0 runs on the device's stream, the others on streams
+// of their own (default priority).
+int render_buffers(kr_render *r, size_t lanes, uint32_t spill_entries) {
     size_t P = r->P;
     auto get = [&](auto **p, size_t cnt) {
         int rc = dalloc(p, cnt);
@@ -2145,7 +2159,7 @@ int render_buffers(kr_render *r, size_t lanes) {
         }
         if (!rc) rc = get(&pp.L, 3 * P);
         if (!rc) rc = get(&pp.counters, size_t(C_NUM) * kCtrStride);
-        if (!rc) rc = ensure_spill(pp.spill, pp.spill_lanes, r->allocs, r->scene->dev, lanes);
+        if (!rc) rc = ensure_spill(pp.spill, pp.spill_lanes, r->allocs, r->scene->dev, lanes, spill_entries);
         if (!rc) {
             if (pi == 0) {
                 pp.stream = r->scene->dev->stream;
@@ -2196,9 +2210,14 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
     r->P = uint32_t(P);
     // a second pipe once a frame is large enough to split into two chunks
     // that each fill the GPU
-    r->npipes = total >= kPipeMinPaths ? 2 : 1;
-    if (r->npipes == 2 && !prm->chunk_paths) r->P = uint32_t(std::min<uint64_t>(r->P, (total + 1) / 2));
-    rc = render_buffers(r, 0);
+    r->npipes = total >= kPipeMinPaths ? KR_RENDER_PIPES : 1;
+    if (r->npipes > 1 && !prm->chunk_paths) {
+        // a multiple of npipes equal chunks of at most P paths
+        uint64_t per = (total + r->npipes - 1) / r->npipes;
+        uint64_t m = (per + P - 1) / P;
+        r->P = uint32_t((per + m - 1) / m);
+    }
+    rc = render_buffers(r, 0, spill_depth(s->view.stack_cap));
     if (rc) {
         kr_render_free(r);
         return rc;
@@ -2469,9 +2488,10 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
 // stream sees the whole run.
 template <class F>
 int pipelined(kr_render *r, int npipes, uint64_t nchunks, F &&f) {
-    if (npipes > 1 && nchunks > 1) {
+    npipes = int(std::min<uint64_t>(uint64_t(npipes), nchunks));
+    if (npipes > 1) {
         KR_HIP(hipEventRecord(r->fork, r->pipe[0].stream));
-        KR_HIP(hipStreamWaitEvent(r->pipe[1].stream, r->fork, 0));
+        for (int pi = 1; pi < npipes; ++pi) KR_HIP(hipStreamWaitEvent(r->pipe[pi].stream, r->fork, 0));
     } else {
         npipes = 1;
     }
@@ -2481,7 +2501,7 @@ int pipelined(kr_render *r, int npipes, uint64_t nchunks, F &&f) {
         int rc = f(pp, after, k);
         if (rc) return rc;
     }
-    if (npipes > 1) KR_HIP(hipStreamWaitEvent(r->pipe[0].stream, r->pipe[1].acc_done, 0));
+    for (int pi = 1; pi < npipes; ++pi) KR_HIP(hipStreamWaitEvent(r->pipe[0].stream, r->pipe[pi].acc_done, 0));
     return KR_OK;
 }
 }  // namespace
@@ -3496,7 +3516,8 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
         r->npix = uint32_t(pix);
         r->P = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(std::max(tu.chunk_paths, kBatchMaxChunk), total)));
         r->npipes = total >= kPipeMinPaths ? 2 : 1;
-        rc = render_buffers(r, size_t(std::max(dev->num_cu, 1)) * kMaxBlocksPerCu * kBlock);
+        // the autotuner may re-upload the scenes with deeper trees: room for any
+        rc = render_buffers(r, size_t(std::max(dev->num_cu, 1)) * kMaxBlocksPerCu * kBlock, 0);
         // pipe 1 accumulates into an image of its own (kr_batch_read sums them)
         if (!rc && r->npipes > 1) {
             Pipe &p1 = r->pipe[1];
