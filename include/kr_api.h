@@ -195,8 +195,8 @@ enum kr_kernel_type {
                                                is done from the queue (persistent
                                                threads) instead of a fixed share */
 #define KR_RENDER_ONE_STREAM 0x4u /* run every chunk on the device's stream; by
-                                     default a render of >= 8 M paths alternates
-                                     its chunks between two streams (same image) */
+                                     default a render of >= 8 M paths cycles its
+                                     chunks through four streams (same image)   */
 
 /* ---- ray query buffers -------------------------------------------------- */
 
@@ -232,7 +232,7 @@ typedef struct kr_render_stats {
     double trace_ms, shade_ms, shadow_ms, other_ms;
     uint64_t trace_launches, shade_launches, shadow_launches;
     /* the same launches' busy time: the union of their intervals. A render
-       of at least 8 M paths alternates its chunks between two streams, so
+       of at least 8 M paths cycles its chunks through four streams, so
        launches of one kind overlap and the sums above exceed this        */
     double trace_busy_ms, shade_busy_ms, shadow_busy_ms;
 } kr_render_stats;

@@ -1,7 +1,7 @@
-"""Renders of >= 8 M paths alternate their chunks between two streams
-(struct Pipe): the image must be bit-identical to the same chunks run one
+"""Renders of >= 8 M paths cycle their chunks through four streams (struct
+Pipe): the image must be bit-identical to the same chunks run one
 after another on one stream (KR_RENDER_ONE_STREAM), with and without a
-radiance cache and a profiler; batches whose work spans both streams must
+radiance cache; batches, whose work spans their two streams, must
 match their jobs rendered alone, with chunks cut per job and with chunks that
 straddle jobs (KR_BATCH_MIX_JOBS)."""
 import numpy as np
@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def spheres():
-    return sg.sphere_scene(width=1024, height=1024, spp=16)  # 16.8 M paths: two chunks, two streams
+    return sg.sphere_scene(width=1024, height=1024, spp=16)  # 16.8 M paths: four chunks, four streams
 
 
 def _render(device, scene, flags=0, nrc=None, seed=5):
