@@ -2208,8 +2208,8 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
     P = std::min<uint64_t>(P, total);
     P = std::max<uint64_t>(P, 1);
     r->P = uint32_t(P);
-    // a second pipe once a frame is large enough to split into two chunks
-    // that each fill the GPU
+    // more pipes once a frame is large enough to split into chunks that each
+    // still fill the GPU
     r->npipes = total >= kPipeMinPaths ? KR_RENDER_PIPES : 1;
     if (r->npipes > 1 && !prm->chunk_paths) {
         // a multiple of npipes equal chunks of at most P paths
@@ -2295,7 +2295,7 @@ struct Timed {
 // of one kind overlap in time, so besides the summed durations the union of
 // their intervals (measured from the first pending event) is kept: the time
 // during which at least one launch of the kind ran. Every pending event has
-// completed when this runs (after a synchronisation of both pipes), and later
+// completed when this runs (after a synchronisation of every pipe), and later
 // events are later in time, so unions of successive calls add up.
 void resolve_events(kr_render *r) {
     std::vector<std::pair<float, float>> iv[K_KINDS];
@@ -3224,8 +3224,8 @@ struct kr_batch {
     JobRec *d_trec = nullptr;     // trial job table (autotuner)
     uint64_t *d_tbegin = nullptr;
     std::vector<void *> allocs;
-    hipGraph_t graph[2] = {nullptr, nullptr};  // one per pipe: its chunks, replayed on its stream
-    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[kMaxPipes] = {};  // one per pipe: its chunks, replayed on its stream
+    hipGraphExec_t exec[kMaxPipes] = {};
     uint32_t graph_nodes = 0, chunks = 0;
     double capture_seconds = 0, seconds = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -3237,7 +3237,7 @@ constexpr uint32_t kMaxBlocksPerCu = 64;
 constexpr uint32_t kBatchMaxChunk = 32u << 20;
 
 void batch_drop_graph(kr_batch *b) {
-    for (int pi = 0; pi < 2; ++pi) {
+    for (int pi = 0; pi < kMaxPipes; ++pi) {
         if (b->exec[pi]) (void)hipGraphExecDestroy(b->exec[pi]);
         if (b->graph[pi]) (void)hipGraphDestroy(b->graph[pi]);
         b->exec[pi] = nullptr;
@@ -3360,19 +3360,19 @@ int batch_enqueue_pipe(kr_batch *b, const BatchView &bv, uint64_t total, uint32_
     return KR_OK;
 }
 
-// Both pipes' chunks, pipe 1 forked from and joined back into the device's
-// stream.
+// Every pipe's chunks, pipes 1.. forked from and joined back into the
+// device's stream.
 int batch_enqueue(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P) {
     kr_render *r = b->r;
     if (r->npipes > 1) {
         KR_HIP(hipEventRecord(r->fork, r->pipe[0].stream));
-        KR_HIP(hipStreamWaitEvent(r->pipe[1].stream, r->fork, 0));
+        for (int pi = 1; pi < r->npipes; ++pi) KR_HIP(hipStreamWaitEvent(r->pipe[pi].stream, r->fork, 0));
     }
     for (int pi = 0; pi < r->npipes; ++pi) {
         int rc = batch_enqueue_pipe(b, bv, total, P, pi);
         if (rc) return rc;
     }
-    if (r->npipes > 1) KR_HIP(hipStreamWaitEvent(r->pipe[0].stream, r->pipe[1].acc_done, 0));
+    for (int pi = 1; pi < r->npipes; ++pi) KR_HIP(hipStreamWaitEvent(r->pipe[0].stream, r->pipe[pi].acc_done, 0));
     return KR_OK;
 }
 
@@ -3515,16 +3515,16 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
         r->prm.flags = tu.flags;
         r->npix = uint32_t(pix);
         r->P = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(std::max(tu.chunk_paths, kBatchMaxChunk), total)));
-        r->npipes = total >= kPipeMinPaths ? 2 : 1;
+        r->npipes = total >= kPipeMinPaths ? KR_RENDER_PIPES : 1;
         // the autotuner may re-upload the scenes with deeper trees: room for any
         rc = render_buffers(r, size_t(std::max(dev->num_cu, 1)) * kMaxBlocksPerCu * kBlock, 0);
-        // pipe 1 accumulates into an image of its own (kr_batch_read sums them)
-        if (!rc && r->npipes > 1) {
-            Pipe &p1 = r->pipe[1];
-            rc = dalloc(&p1.acc, 3 * size_t(r->npix));
-            if (!rc) r->allocs.push_back(p1.acc);
-            if (!rc) rc = dalloc(&p1.valid, size_t(r->npix));
-            if (!rc) r->allocs.push_back(p1.valid);
+        // pipes 1.. accumulate into images of their own (kr_batch_read sums them)
+        for (int pi = 1; pi < r->npipes && !rc; ++pi) {
+            Pipe &pp = r->pipe[pi];
+            rc = dalloc(&pp.acc, 3 * size_t(r->npix));
+            if (!rc) r->allocs.push_back(pp.acc);
+            if (!rc) rc = dalloc(&pp.valid, size_t(r->npix));
+            if (!rc) r->allocs.push_back(pp.valid);
         }
         if (!rc) rc = kr_render_reset(r);
     }
@@ -3549,10 +3549,11 @@ KR_EXPORT int kr_batch_run(kr_batch *b) {
     kr_render *r = b->r;
     hipStream_t st = b->dev->stream;
     KR_HIP(hipEventRecord(b->e0, st));
-    if (r->npipes > 1) KR_HIP(hipStreamWaitEvent(r->pipe[1].stream, b->e0, 0));
+    for (int pi = 1; pi < r->npipes; ++pi) KR_HIP(hipStreamWaitEvent(r->pipe[pi].stream, b->e0, 0));
     for (int pi = 0; pi < r->npipes; ++pi) KR_HIP(hipGraphLaunch(b->exec[pi], r->pipe[pi].stream));
-    if (r->npipes > 1) {
-        KR_HIP(hipEventRecord(r->fork, r->pipe[1].stream));
+    // join: a wait takes the event's latest record, so one event serves every pipe
+    for (int pi = 1; pi < r->npipes; ++pi) {
+        KR_HIP(hipEventRecord(r->fork, r->pipe[pi].stream));
         KR_HIP(hipStreamWaitEvent(st, r->fork, 0));
     }
     KR_HIP(hipEventRecord(b->e1, st));
@@ -3585,23 +3586,27 @@ KR_EXPORT int kr_batch_read(kr_batch *b, uint32_t job, double *rgb, uint32_t *va
     size_t off = b->acc_off[job], n = size_t(b->jobs[job].params.width) * b->jobs[job].params.height;
     if (rgb) KR_HIP(hipMemcpyAsync(rgb, b->r->acc + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
     if (valid) KR_HIP(hipMemcpyAsync(valid, b->r->valid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
-    // the second pipe's image: the job's sums are the two images' sums
-    std::vector<double> rgb1;
-    std::vector<uint32_t> valid1;
-    const Pipe &p1 = b->r->pipe[1];
-    bool two = b->r->npipes > 1 && p1.acc != b->r->acc;
-    if (two) {
-        rgb1.resize(rgb ? 3 * n : 0);
-        valid1.resize(valid ? n : 0);
-        if (rgb) KR_HIP(hipMemcpyAsync(rgb1.data(), p1.acc + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
-        if (valid) KR_HIP(hipMemcpyAsync(valid1.data(), p1.valid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+    // the other pipes' images: the job's sums are the images' sums, in pipe order
+    const int np = b->r->npipes;
+    std::vector<double> rgbp(rgb ? size_t(np) * 3 * n : 0);
+    std::vector<uint32_t> validp(valid ? size_t(np) * n : 0);
+    for (int pi = 1; pi < np; ++pi) {
+        const Pipe &pp = b->r->pipe[pi];
+        if (pp.acc == b->r->acc) continue;
+        if (rgb)
+            KR_HIP(hipMemcpyAsync(rgbp.data() + size_t(pi) * 3 * n, pp.acc + 3 * off, sizeof(double) * 3 * n,
+                                  hipMemcpyDeviceToHost, st));
+        if (valid)
+            KR_HIP(hipMemcpyAsync(validp.data() + size_t(pi) * n, pp.valid + off, sizeof(uint32_t) * n,
+                                  hipMemcpyDeviceToHost, st));
     }
     KR_HIP(hipStreamSynchronize(st));
-    if (two) {
+    for (int pi = 1; pi < np; ++pi) {
+        if (b->r->pipe[pi].acc == b->r->acc) continue;
         if (rgb)
-            for (size_t i = 0; i < 3 * n; ++i) rgb[i] += rgb1[i];
+            for (size_t i = 0; i < 3 * n; ++i) rgb[i] += rgbp[size_t(pi) * 3 * n + i];
         if (valid)
-            for (size_t i = 0; i < n; ++i) valid[i] += valid1[i];
+            for (size_t i = 0; i < n; ++i) valid[i] += validp[size_t(pi) * n + i];
     }
     return KR_OK;
 }
