@@ -168,6 +168,8 @@ def bench_cache(args, kr, scene, gs, p, dev):
 
     # 4, 8, 16, then steps of spp/8 up to the workload's spp
     ladder = sorted({s for s in [4, 8, 16] + [p.spp * k // 8 for k in range(2, 9)] if 1 <= s <= p.spp})
+    if args.cache_ladder:
+        ladder = sorted({int(x) for x in args.cache_ladder.split(",") if 1 <= int(x) <= p.spp})
     plain = {s: over_seeds(None, s) for s in ladder}
     # the target: the plain frame's clamped RMSE at the workload's spp (what the
     # reference's 8-bit output shows; the unclamped RMSE of these scenes is
@@ -220,9 +222,11 @@ def bench_cache(args, kr, scene, gs, p, dev):
         }
 
     main_res = cached_ladder(mode)
-    other = cached_ladder("second" if mode == "first" else "first")
     out = {"target_rmse_clamped": target, "target_clamp": emax, "plain_seconds_to_target": plain_time}
     out.update(main_res)
+    if args.cache_one_mode:
+        return out
+    other = cached_ladder("second" if mode == "first" else "first")
     out["other_mode"] = {
         "mode": other["mode"], "settings": other["settings"], "speedup": other["speedup"],
         "cached_seconds_to_target": other["cached_seconds_to_target"],
@@ -363,7 +367,7 @@ def bench_batch(args, kr, sg, barrier):
             "bytes_per_query": bytes_per_query,
             "queries_per_launch": closest / max(1, trace_launches),
             "avg_launch_ms": trace_busy / max(1, trace_launches),
-            "launch_time": "union of the k_trace launch intervals per launch (two streams per render)",
+            "launch_time": "union of the k_trace launch intervals per launch (four streams per render)",
             "avg_overlapped_launch_ms": trace_ms / max(1, trace_launches),
         },
         "cpu_baseline": cpu,
@@ -388,6 +392,10 @@ def main():
                          "against the plain path at equal RMSE")
     ap.add_argument("--cache-warmup", type=int, default=64,
                     help="4-spp training frames before the cached frames are timed")
+    ap.add_argument("--cache-ladder", default="",
+                    help="comma-separated spp of the timed frames (default 4, 8, 16, then steps of spp/8); "
+                         "the target is the plain frame's RMSE at the largest")
+    ap.add_argument("--cache-one-mode", action="store_true", help="measure only the chosen cache mode")
     ap.add_argument("--cache-mode", default="auto", choices=["auto", "second", "first"],
                     help="where cached paths end (CACHE_MODES); auto: first")
     ap.add_argument("--no-material-sort", action="store_true",
@@ -464,8 +472,8 @@ def main():
     rgb, valid = rend.read()
     value = paths_per_step * args.steps / elapsed
     trace_bytes = st.closest_rays * bytes_per_query
-    # frames of two chunks run them on two streams, so launches overlap: the
-    # kernel's time is the union of its launch intervals (busy time)
+    # a frame's chunks run on four streams, so launches overlap: the kernel's
+    # time is the union of its launch intervals (busy time)
     achieved = trace_bytes / (st.trace_busy_ms * 1e-3) / 1e9 if st.trace_busy_ms > 0 else 0.0
 
     rmse = rmse_vs_oracle(args.workload, rgb, valid, p.width)
@@ -535,7 +543,7 @@ def main():
             "queries_per_launch": st.closest_rays / max(1, st.trace_launches),
             "avg_launch_ms": st.trace_busy_ms / max(1, st.trace_launches),
             "launch_time": "union of the k_trace launch intervals over the timed region per launch (the "
-                           "frame's two chunks run on two streams, so launches overlap; the summed durations "
+                           "frame's chunks run on four streams, so launches overlap; the summed durations "
                            "per launch are in avg_overlapped_launch_ms)",
             "avg_overlapped_launch_ms": st.trace_ms / max(1, st.trace_launches),
         },
