@@ -99,18 +99,23 @@ def rmse_vs_oracle(workload, rgb, valid, width):
 # often never reached. "first" with 4 steps per frame and rough threshold 0.3
 # reached the large scene's target at 48 spp and never the room's; with 8
 # steps and threshold 0.6 (glossy surfaces below it keep their paths going)
-# at 24 spp and 8 spp.
+# at 24 spp and 8 spp. At lr 3e-2 one of three "first" runs on the room
+# trained into a worse state (bias 0.148 against 0.111); at 1e-2 three of
+# three reached the targets. The "second" mode reached the room's target at
+# 32-40 spp with lr 3e-2 against 40 with 1e-2.
 CACHE_MODES = {
-    "second": {"flags": 0, "steps_per_frame": 4, "rough_threshold": 0.3},
-    "first": {"flags": 0x2, "steps_per_frame": 8, "rough_threshold": 0.6},  # flags: KR_NRC_INDIRECT
+    "second": {"flags": 0, "steps_per_frame": 4, "rough_threshold": 0.3, "learning_rate": 3e-2},
+    "first": {"flags": 0x2, "steps_per_frame": 8, "rough_threshold": 0.6,  # flags: KR_NRC_INDIRECT
+              "learning_rate": 1e-2},
 }
 
 
 def bench_cache(args, kr, scene, gs, p, dev):
     """Radiance cache on the workload against the plain path. The cache
-    (1/16 of paths train, Adam steps of 16384 samples after each frame, lr
-    3e-2, renders query the moving average of decay 0.99 per step, targets
-    clamped at the brightest emitter's radiance; ending mode per CACHE_MODES)
+    (1/16 of paths train, Adam steps of 16384 samples after each frame,
+    renders query the moving average of decay 0.99 per step, targets clamped
+    at the brightest emitter's radiance; ending mode, steps, rough threshold
+    and learning rate per CACHE_MODES)
     is trained for a number of 4-spp warm-up frames, then frames of s = 4, 8,
     16, then steps of spp/8 up to spp samples are timed for both paths, with
     RMSE against the FP64 oracle averaged over 3 seeds. The target is the
@@ -179,8 +184,7 @@ def bench_cache(args, kr, scene, gs, p, dev):
 
     def cached_ladder(m):
         cfg = CACHE_MODES[m]
-        nrc = kr.Nrc(dev, lo, hi, train_log2=4, batch=16384, target_clamp=emax, learning_rate=3e-2, ema_decay=0.99,
-                     **cfg)
+        nrc = kr.Nrc(dev, lo, hi, train_log2=4, batch=16384, target_clamp=emax, ema_decay=0.99, **cfg)
         warm = 0
         t_warm = time.perf_counter()
         rw = kr.Render(gs, cam, scene.params(seed=7, spp=4))
@@ -210,8 +214,7 @@ def bench_cache(args, kr, scene, gs, p, dev):
                 cached_time = r["cached"]["seconds"]
                 break
         return {
-            "mode": m, "settings": {**cfg, "train_fraction": 1 / 16, "batch": 16384, "learning_rate": 3e-2,
-                                    "ema_decay": 0.99},
+            "mode": m, "settings": {**cfg, "train_fraction": 1 / 16, "batch": 16384, "ema_decay": 0.99},
             "cached_seconds_to_target": cached_time,
             "speedup": (plain_time / cached_time) if cached_time else None,
             "warmup_frames": warm, "warmup_seconds": t_warm,
