@@ -158,10 +158,13 @@ def bench_cache(args, kr, scene, gs, p, dev):
                 r.attach_nrc(nrc)
             frame(r, s)
             dt, img = frame(r, s)
+            queries = r.stats().cache_queries
             r.free()
             dts.append(dt)
             imgs.append(img)
         out = {"seconds": min(dts)}
+        if nrc is not None:
+            out["cache_queries"] = queries
         for key, f in (("", lambda x: x), ("_clamped", lambda x: np.clip(x, 0.0, 1.0))):
             im = np.stack([f(x) for x in imgs])
             ref = f(fref)

@@ -224,7 +224,7 @@ enum : int {
 // serialise on one L2 channel.
 constexpr int kCtrStride = 32;
 #define CTR(i) (kCtrStride * (i))
-enum : int { S_CLOSEST = 0, S_SHADOW = 1, S_INVALID = 2, S_PATHS = 3, S_NODES = 4, S_TRIS = 5, S_NUM = 8 };
+enum : int { S_CLOSEST = 0, S_SHADOW = 1, S_INVALID = 2, S_PATHS = 3, S_NODES = 4, S_TRIS = 5, S_NRCQ = 6, S_NUM = 8 };
 enum : int { K_TRACE = 0, K_SHADE = 1, K_SHADOW = 2, K_OTHER = 3, K_KINDS = 4 };
 
 struct kr_nrc {
@@ -2655,6 +2655,7 @@ KR_EXPORT int kr_render_get_stats(kr_render *r, kr_render_stats *out) {
     out->invalid_samples = h[S_INVALID];
     out->node_visits = h[S_NODES];
     out->tri_tests = h[S_TRIS];
+    out->cache_queries = h[S_NRCQ];
     out->paths = r->paths;
     out->seconds = r->last_seconds;
     out->trace_ms = r->kind_ms[K_TRACE];
@@ -3022,10 +3023,12 @@ KR_EXPORT int kr_nrc_get_stats(kr_nrc *n, uint64_t *samples_trained, float *last
 #ifndef KR_NRC_INFER_WAVES
 #define KR_NRC_INFER_WAVES 2
 #endif
-__global__ __launch_bounds__(256, KR_NRC_INFER_WAVES) void k_nrc_infer(nrc::Weights W, NrcRender nr, const uint32_t *counters, float *L) {
+__global__ __launch_bounds__(256, KR_NRC_INFER_WAVES) void k_nrc_infer(nrc::Weights W, NrcRender nr, const uint32_t *counters, float *L,
+                                                                         unsigned long long *stats) {
     using namespace nrc;
     int lane = __lane_id(), s = lane & 15, q = lane >> 4;
     uint32_t n = counters[CTR(C_NRCQ)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_NRCQ], (unsigned long long)n);
     uint32_t waves = gridDim.x * (blockDim.x / 64);
     for (uint32_t base = (blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) * 64; base < n; base += waves * 64) {
         v4s in[6][4];
@@ -3114,7 +3117,8 @@ void nrc_after_chunk(kr_render *r, Pipe &pp) {
     {
         Timed tm(r, K_OTHER, st);
         int grid = int(std::min<size_t>((size_t(r->P) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
-        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_render_view(n), pp.nr, pp.counters, pp.L);
+        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_render_view(n), pp.nr, pp.counters, pp.L,
+                           r->stats);
         int egrid = int(std::min<size_t>((size_t(pp.nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
         hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, pp.nr, pp.counters, n->pool_raw, n->pool_tgt,
                            n->pool_total, n->pool_cap, n->cfg.target_clamp);

@@ -101,7 +101,8 @@ class RenderStats(C.Structure):
                 ("seconds", C.c_double), ("node_visits", u64), ("tri_tests", u64), ("trace_ms", C.c_double),
                 ("shade_ms", C.c_double), ("shadow_ms", C.c_double), ("other_ms", C.c_double),
                 ("trace_launches", u64), ("shade_launches", u64), ("shadow_launches", u64),
-                ("trace_busy_ms", C.c_double), ("shade_busy_ms", C.c_double), ("shadow_busy_ms", C.c_double)]
+                ("trace_busy_ms", C.c_double), ("shade_busy_ms", C.c_double), ("shadow_busy_ms", C.c_double),
+                ("cache_queries", u64)]
 
 
 KR_PROFILE_EVENTS = 0x1

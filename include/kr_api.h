@@ -235,6 +235,7 @@ typedef struct kr_render_stats {
        of at least 8 M paths cycles its chunks through four streams, so
        launches of one kind overlap and the sums above exceed this        */
     double trace_busy_ms, shade_busy_ms, shadow_busy_ms;
+    uint64_t cache_queries;  /* radiance-cache queries (paths ended by it)  */
 } kr_render_stats;
 
 #define KR_PROFILE_EVENTS 0x1u   /* time every kernel with HIP events        */

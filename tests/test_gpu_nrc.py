@@ -189,13 +189,15 @@ def _floor_light_scene(width=48, height=32, max_bounce=0):
     return b.finish((0.3, 1.0, 0.2), (0.35, 0.0, 0.25), 60.0, width, height, 1, max_bounce, "nee")
 
 
-def _one_spp(device, gs, s, p, nrc=None):
+def _one_spp(device, gs, s, p, nrc=None, stats=None):
     r = kr.Render(gs, s.camera_struct(), p)
     if nrc is not None:
         r.attach_nrc(nrc)
     r.run(0, 1)
     r.sync()
     out = r.read()
+    if stats is not None:
+        stats.append(r.stats())
     r.free()
     return out
 
@@ -238,10 +240,18 @@ def test_indirect_paths_end_at_the_first_diffuse_vertex(device):
     rgb0, val0 = _one_spp(device, gs, s, p0)
     n = kr.Nrc(device, *_box(s), train_log2=30, flags=kr.NRC_FROZEN | kr.NRC_INDIRECT)
     n.set_params(np.zeros(kr.NRC_PARAMS, np.float32))
-    rgb, val = _one_spp(device, gs, s, s.params(seed=5), n)
+    st = []
+    rgb, val = _one_spp(device, gs, s, s.params(seed=5), n, st)
     assert np.array_equal(val, val0)
     differ = np.any(rgb != rgb0, axis=1)
     assert differ.sum() <= 2, differ.sum()
+    # one query per camera ray that hits the scene, training paths aside
+    # (every hit is lit: the light sees the whole floor and itself emits)
+    st0 = []
+    _one_spp(device, gs, s, p0, None, st0)
+    hits = int((rgb0.max(axis=1) > 0).sum())
+    assert st0[0].cache_queries == 0 and hits > 100
+    assert hits - 2 <= st[0].cache_queries <= min(hits, st0[0].closest_rays)
     # without the flag the same cache ends paths at their second vertex
     m = kr.Nrc(device, *_box(s), train_log2=30, flags=kr.NRC_FROZEN)
     m.set_params(np.zeros(kr.NRC_PARAMS, np.float32))
