@@ -2269,7 +2269,7 @@ struct Timed {
     hipStream_t st;
     hipEvent_t a = nullptr, b = nullptr;
     Timed(kr_render *r_, int k, hipStream_t s = nullptr) : r(r_), kind(k), st(s ? s : r_->scene->dev->stream) {
-        if (!(r->profile & KR_PROFILE_EVENTS)) return;
+        if (!(r->profile & KR_PROFILE_EVENTS) && !((r->profile & KR_PROFILE_TRACE_EVENTS) && kind == K_TRACE)) return;
         a = take();
         b = take();
         if (a && b) (void)hipEventRecord(a, st);

@@ -108,6 +108,7 @@ class RenderStats(C.Structure):
 KR_PROFILE_EVENTS = 0x1
 KR_BATCH_MIX_JOBS = 0x100  # BatchTuning(flags=...): chunks may straddle jobs
 KR_PROFILE_COUNTERS = 0x2
+KR_PROFILE_TRACE_EVENTS = 0x4  # time only the closest-hit traversal launches
 
 NRC_RAW, NRC_FEATURES, NRC_PARAMS = 16, 96, 23888
 NRC_TRAIN_ALL = 0xFFFFFFFF  # Nrc(train_log2=...): every path trains

@@ -240,6 +240,8 @@ typedef struct kr_render_stats {
 
 #define KR_PROFILE_EVENTS 0x1u   /* time every kernel with HIP events        */
 #define KR_PROFILE_COUNTERS 0x2u /* count node visits / triangle tests      */
+#define KR_PROFILE_TRACE_EVENTS 0x4u /* time only the closest-hit traversal
+                                        launches (fewer events in the stream)  */
 
 /* ---- library and device ------------------------------------------------- */
 
