@@ -1,7 +1,7 @@
 """Renders of >= 8 M paths cycle their chunks through four streams (struct
 Pipe): the image must be bit-identical to the same chunks run one
 after another on one stream (KR_RENDER_ONE_STREAM), with and without a
-radiance cache; batches, whose work spans their two streams, must
+radiance cache; batches, whose work spans their four streams, must
 match their jobs rendered alone, with chunks cut per job and with chunks that
 straddle jobs (KR_BATCH_MIX_JOBS)."""
 import numpy as np
