@@ -3590,27 +3590,22 @@ KR_EXPORT int kr_batch_read(kr_batch *b, uint32_t job, double *rgb, uint32_t *va
     size_t off = b->acc_off[job], n = size_t(b->jobs[job].params.width) * b->jobs[job].params.height;
     if (rgb) KR_HIP(hipMemcpyAsync(rgb, b->r->acc + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
     if (valid) KR_HIP(hipMemcpyAsync(valid, b->r->valid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+    KR_HIP(hipStreamSynchronize(st));
     // the other pipes' images: the job's sums are the images' sums, in pipe order
-    const int np = b->r->npipes;
-    std::vector<double> rgbp(rgb ? size_t(np) * 3 * n : 0);
-    std::vector<uint32_t> validp(valid ? size_t(np) * n : 0);
-    for (int pi = 1; pi < np; ++pi) {
+    std::vector<double> rgbp(rgb ? 3 * n : 0);
+    std::vector<uint32_t> validp(valid ? n : 0);
+    for (int pi = 1; pi < b->r->npipes; ++pi) {
         const Pipe &pp = b->r->pipe[pi];
         if (pp.acc == b->r->acc) continue;
         if (rgb)
-            KR_HIP(hipMemcpyAsync(rgbp.data() + size_t(pi) * 3 * n, pp.acc + 3 * off, sizeof(double) * 3 * n,
-                                  hipMemcpyDeviceToHost, st));
+            KR_HIP(hipMemcpyAsync(rgbp.data(), pp.acc + 3 * off, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
         if (valid)
-            KR_HIP(hipMemcpyAsync(validp.data() + size_t(pi) * n, pp.valid + off, sizeof(uint32_t) * n,
-                                  hipMemcpyDeviceToHost, st));
-    }
-    KR_HIP(hipStreamSynchronize(st));
-    for (int pi = 1; pi < np; ++pi) {
-        if (b->r->pipe[pi].acc == b->r->acc) continue;
+            KR_HIP(hipMemcpyAsync(validp.data(), pp.valid + off, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+        KR_HIP(hipStreamSynchronize(st));
         if (rgb)
-            for (size_t i = 0; i < 3 * n; ++i) rgb[i] += rgbp[size_t(pi) * 3 * n + i];
+            for (size_t i = 0; i < 3 * n; ++i) rgb[i] += rgbp[i];
         if (valid)
-            for (size_t i = 0; i < n; ++i) valid[i] += validp[size_t(pi) * n + i];
+            for (size_t i = 0; i < n; ++i) valid[i] += validp[i];
     }
     return KR_OK;
 }
