@@ -2291,8 +2291,8 @@ struct Timed {
     }
 };
 
-// Adds the resolved launches to the per-kind sums. With two pipes, launches
-// of one kind overlap in time, so besides the summed durations the union of
+// Adds the resolved launches to the per-kind sums. With several pipes,
+// launches of one kind overlap in time, so besides the summed durations the union of
 // their intervals (measured from the first pending event) is kept: the time
 // during which at least one launch of the kind ran. Every pending event has
 // completed when this runs (after a synchronisation of every pipe), and later
@@ -2481,11 +2481,11 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
     return KR_OK;
 }
 
-// Runs f(pipe, acc_after) for consecutive chunks, alternating between the
-// render's pipes: pipe 1 first waits for everything queued on pipe 0 (resets,
-// earlier runs), each accumulation waits for the previous one, and pipe 0
-// finally waits for pipe 1, so that work queued afterwards on the device's
-// stream sees the whole run.
+// Runs f(pipe, acc_after) for consecutive chunks, cycling through the
+// render's pipes: pipes 1.. first wait for everything queued on pipe 0
+// (resets, earlier runs), each accumulation waits for the previous one, and
+// pipe 0 finally waits for every other pipe, so that work queued afterwards
+// on the device's stream sees the whole run.
 template <class F>
 int pipelined(kr_render *r, int npipes, uint64_t nchunks, F &&f) {
     npipes = int(std::min<uint64_t>(uint64_t(npipes), nchunks));
