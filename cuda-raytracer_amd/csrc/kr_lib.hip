@@ -62,6 +62,9 @@ constexpr int kBlock = 256;
 #ifndef KR_TRACE_WAVES
 #define KR_TRACE_WAVES 8
 #endif
+#ifndef KR_SHADE_OCTANT_SORT
+#define KR_SHADE_OCTANT_SORT 1
+#endif
 #ifndef KR_SHADE_WAVES
 #define KR_SHADE_WAVES 1
 #endif
@@ -1009,7 +1012,32 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
             a.sq.cy[ps] = contrib.y;
             a.sq.cz[ps] = contrib.z;
         }
+#if KR_SHADE_OCTANT_SORT
+        // the block's next rays enter the queue grouped by direction octant,
+        // so that a traversal wave holds rays of one or two octants, which
+        // order their nodes alike (one queue atomic per block)
+        __shared__ uint32_t s_oct[8];
+        __shared__ uint32_t s_base;
+        if (threadIdx.x < 8) s_oct[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t oct = (nd.x < 0.f ? 1u : 0u) | (nd.y < 0.f ? 2u : 0u) | (nd.z < 0.f ? 4u : 0u);
+        uint32_t rank = push_ray ? atomicAdd(&s_oct[oct], 1u) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int o = 0; o < 8; ++o) {
+                uint32_t c = s_oct[o];
+                s_oct[o] = tot;
+                tot += c;
+            }
+            s_base = tot ? atomicAdd(&a.counters[CTR(a.qni)], tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t pr = s_base + s_oct[oct] + rank;
+        __syncthreads();  // s_oct and s_base are rewritten by the next iteration
+#else
         uint32_t pr = wave_append(&a.counters[CTR(a.qni)], push_ray);
+#endif
         if (push_ray) {
             a.qn.slot[pr] = slot;
             a.qn.ox[pr] = no.x;
