@@ -65,6 +65,7 @@ constexpr int kBlock = 256;
 #ifndef KR_SHADE_OCTANT_SORT
 #define KR_SHADE_OCTANT_SORT 1
 #endif
+
 #ifndef KR_SHADE_WAVES
 #define KR_SHADE_WAVES 1
 #endif
@@ -998,7 +999,30 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
                 dst[3] = make_float4(qrec[12], 0.f, 0.f, 0.f);
             }
         }
+#if KR_SHADE_OCTANT_SORT
+        // shadow segments grouped by direction octant within the block, as
+        // the next rays below
+        __shared__ uint32_t s_soct[8];
+        __shared__ uint32_t s_sbase;
+        if (threadIdx.x < 8) s_soct[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t soct = (sd.x < 0.f ? 1u : 0u) | (sd.y < 0.f ? 2u : 0u) | (sd.z < 0.f ? 4u : 0u);
+        uint32_t srank = push_shadow ? atomicAdd(&s_soct[soct], 1u) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int o = 0; o < 8; ++o) {
+                uint32_t c = s_soct[o];
+                s_soct[o] = tot;
+                tot += c;
+            }
+            s_sbase = tot ? atomicAdd(&a.counters[CTR(C_SHADOW)], tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t ps = s_sbase + s_soct[soct] + srank;
+#else
         uint32_t ps = wave_append(&a.counters[CTR(C_SHADOW)], push_shadow);
+#endif
         if (push_shadow) {
             a.sq.slot[ps] = slot;
             a.sq.ox[ps] = so.x;
@@ -1166,7 +1190,30 @@ __global__ __launch_bounds__(kBlock) void k_shade_debug(ShadeArgs a, uint32_t ke
             a.L[3 * slot + 1] = L.y;
             a.L[3 * slot + 2] = L.z;
         }
+#if KR_SHADE_OCTANT_SORT
+        // shadow segments grouped by direction octant within the block, as
+        // the next rays below
+        __shared__ uint32_t s_soct[8];
+        __shared__ uint32_t s_sbase;
+        if (threadIdx.x < 8) s_soct[threadIdx.x] = 0;
+        __syncthreads();
+        uint32_t soct = (sd.x < 0.f ? 1u : 0u) | (sd.y < 0.f ? 2u : 0u) | (sd.z < 0.f ? 4u : 0u);
+        uint32_t srank = push_shadow ? atomicAdd(&s_soct[soct], 1u) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tot = 0;
+            for (int o = 0; o < 8; ++o) {
+                uint32_t c = s_soct[o];
+                s_soct[o] = tot;
+                tot += c;
+            }
+            s_sbase = tot ? atomicAdd(&a.counters[CTR(C_SHADOW)], tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t ps = s_sbase + s_soct[soct] + srank;
+#else
         uint32_t ps = wave_append(&a.counters[CTR(C_SHADOW)], push_shadow);
+#endif
         if (push_shadow) {
             a.sq.slot[ps] = slot;
             a.sq.ox[ps] = so.x;
