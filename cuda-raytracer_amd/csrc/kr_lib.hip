@@ -129,7 +129,20 @@ struct kr_device {
     hipStream_t stream = nullptr;
     int num_cu = 0;
     std::string name;
+    // device address of this device's copy of `scratch` (kr_trace_closest_noalloc)
+    uint2 *scratch = nullptr;
 };
+
+// Scratch memory of kr_trace_closest_noalloc: the traversal spill stacks of
+// the lanes of one launch, (ref, t) pairs, entry k of lane g at k * lanes + g
+// (the layout make_stack expects). A file-scope device array, so that the
+// entry point allocates nothing: every GPU the library runs on holds its own
+// copy, whose device address kr_device_open looks up once per device with
+// hipGetSymbolAddress. Host code uses that address, never the array's name
+// (on the host the name is only a stand-in symbol). All calls on one GPU
+// share the array, so calls that overlap on one GPU race.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) uint2 scratch[kScratchBytes / sizeof(uint2)];
 
 struct kr_scene {
     kr_device *dev = nullptr;
@@ -1620,7 +1633,14 @@ KR_EXPORT int kr_device_open(int ordinal, kr_device **out) {
     d->ordinal = ordinal;
     d->num_cu = prop.multiProcessorCount;
     d->name = prop.gcnArchName;
-    hipError_t e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+    void *sp = nullptr;
+    hipError_t e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+        delete d;
+        return fail(KR_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    d->scratch = static_cast<uint2 *>(sp);
+    e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete d;
         return fail(KR_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
@@ -2089,6 +2109,43 @@ KR_EXPORT int kr_trace_closest(kr_scene *s, const kr_rays *rays, uint32_t n, kr_
         KR_HIP(hipMemcpyAsync(hits->prim, prim, 4 * size_t(n), hipMemcpyDeviceToHost, stream));
         KR_HIP(hipMemcpyAsync(hits->tri, tri, 4 * size_t(n), hipMemcpyDeviceToHost, stream));
     }
+    KR_HIP(hipStreamSynchronize(stream));
+    return KR_OK;
+}
+
+KR_EXPORT int kr_trace_closest_noalloc(kr_scene *s, const kr_rays *rays, uint32_t n, kr_hits *hits) {
+    if (!s || !rays || !hits || !rays->origin || !rays->direction || !hits->t || !hits->prim || !hits->tri ||
+        !hits->u || !hits->v)
+        return fail(KR_E_INVALID, "NULL argument");
+    if (n == 0) return KR_OK;
+    kr_device *dev = s->dev;
+    if (!dev->scratch) return fail(KR_E_HIP, "device scratch address unknown");
+    // The scratch holds `depth` spill entries for every lane in flight: the
+    // launch is cut to as many lanes as fit, and its grid-stride loop covers
+    // any n. A scene whose stack needs more than one block's worth is refused.
+    const uint32_t depth = spill_depth(s->view.stack_cap);
+    size_t lanes = kScratchBytes / (sizeof(uint2) * size_t(depth)) / kBlock * kBlock;
+    if (lanes < size_t(kBlock))
+        return fail(KR_E_OVERFLOW, "kr_trace_closest_noalloc: the scene's traversal stack needs " +
+                                       std::to_string(size_t(depth) * kBlock * sizeof(uint2)) +
+                                       " bytes of scratch per block, more than the fixed " +
+                                       std::to_string(kScratchBytes) + " bytes");
+    lanes = std::min(lanes, grid_lanes(dev));
+    size_t blocks = (size_t(n) + kBlock - 1) / kBlock;
+    int grid = int(std::max<size_t>(1, std::min(blocks, lanes / kBlock)));
+    KR_HIP(hipSetDevice(dev->ordinal));
+    hipStream_t stream = dev->stream;
+    uint32_t *spill = reinterpret_cast<uint32_t *>(dev->scratch);
+    int rc = dispatch_fmt(s->width, s->qnodes, [&](auto fm) {
+        using FM = decltype(fm);
+        auto kern = k_query_closest<FM::W, FM::Q>;
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, stream, s->view,
+                           rays->origin, rays->direction, n, hits->t, hits->prim, hits->tri, hits->u, hits->v, spill,
+                           lanes, (uint64_t *)nullptr);
+        return KR_OK;
+    });
+    if (rc) return rc;
+    KR_HIP(hipGetLastError());
     KR_HIP(hipStreamSynchronize(stream));
     return KR_OK;
 }

@@ -149,7 +149,7 @@ EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "
            "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_get_render_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
            "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc", "kr_nrc_read_pool",
            "kr_batch_create", "kr_batch_autotune", "kr_batch_run", "kr_batch_sync", "kr_batch_reset", "kr_batch_read",
-           "kr_batch_get_info", "kr_batch_free"]
+           "kr_batch_get_info", "kr_batch_free", "kr_trace_closest_noalloc"]
 
 _lib = None
 
@@ -174,6 +174,7 @@ def load(path: str = LIB_PATH):
     lib.kr_scene_light_samples.argtypes = [vp, u32, vp, C.POINTER(f32)]
     lib.kr_trace_closest.argtypes = [vp, C.POINTER(Rays), u32, C.POINTER(Hits), u32]
     lib.kr_trace_shadow.argtypes = [vp, C.POINTER(Rays), u32, vp, u32]
+    lib.kr_trace_closest_noalloc.argtypes = [vp, C.POINTER(Rays), u32, C.POINTER(Hits)]
     lib.kr_render_create.argtypes = [vp, C.POINTER(Camera), C.POINTER(Params), C.POINTER(vp)]
     lib.kr_render_run.argtypes = [vp, u32, u32]
     lib.kr_render_sync.argtypes = [vp]
@@ -293,6 +294,19 @@ class GpuScene:
         hits = Hits(_ptr(t), _ptr(prim), _ptr(tri), _ptr(u), _ptr(v))
         check(self.lib.kr_trace_closest(self.h, C.byref(rays), n, C.byref(hits), KR_MEM_HOST), "kr_trace_closest")
         return t, prim, tri, u, v
+
+    def trace_closest_device(self, origin, direction, n, t, prim, tri, u, v, noalloc=False):
+        """Closest hit of n rays whose buffers are HIP device addresses
+        (ints): kr_trace_closest with KR_MEM_DEVICE, or
+        kr_trace_closest_noalloc."""
+        rays = Rays(C.c_void_p(origin), C.c_void_p(direction), None)
+        hits = Hits(*(C.c_void_p(x) for x in (t, prim, tri, u, v)))
+        if noalloc:
+            check(self.lib.kr_trace_closest_noalloc(self.h, C.byref(rays), n, C.byref(hits)),
+                  "kr_trace_closest_noalloc")
+        else:
+            check(self.lib.kr_trace_closest(self.h, C.byref(rays), n, C.byref(hits), KR_MEM_DEVICE),
+                  "kr_trace_closest")
 
     def trace_shadow(self, origin, direction, tmax):
         o = np.ascontiguousarray(origin, np.float32)

@@ -291,6 +291,19 @@ int kr_trace_closest(kr_scene *scene, const kr_rays *rays, uint32_t n, kr_hits *
 int kr_trace_shadow(kr_scene *scene, const kr_rays *rays, uint32_t n, uint8_t *occluded,
                     uint32_t mem);
 
+/* kr_trace_closest with every pointer of rays and hits a HIP device pointer
+ * (KR_MEM_DEVICE), calling no hipMalloc, hipFree or allocating library
+ * routine. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation). The traversal spill stacks live in a fixed
+ * 32 MiB device array per GPU, held until the process ends; the launch is
+ * cut to the lanes whose stacks fit in it, so any n is accepted, and a scene
+ * whose stack depth needs more than 256 lanes' worth is refused with
+ * KR_E_OVERFLOW. Every call on one GPU shares that array: calls that overlap
+ * on one GPU (from two threads, or two kr_device handles of one ordinal)
+ * race. Synchronous on the device's stream, like kr_trace_closest. */
+int kr_trace_closest_noalloc(kr_scene *scene, const kr_rays *rays, uint32_t n, kr_hits *hits);
+
 /* ---- Radiance integrator -------------------------------------------------
  * Replaces Tracer::execute (src/core/tracer/tracer.hpp:20-21) running
  * RadianceKernel::execute (src/core/kernel/radiance.cu:14-81) per sample and
