@@ -37,6 +37,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <thread>
@@ -393,6 +395,26 @@ struct Counters {
     uint64_t nodes = 0, tris = 0;
 };
 
+// The DFS stack of a mesh traversal: a fixed array on the thread's stack,
+// as the reference's Stack is a fixed block of its per-thread pool
+// (src/core/primitive/mesh.cu:204), instead of a heap vector per query. A
+// DFS holds at most one pending right child per tree level; kr_scene_desc
+// meshes build trees far shallower than the capacity, which is checked.
+struct NodeStack {
+    static constexpr int kCap = 256;
+    uint32_t v[kCap];
+    int top = 0;
+    bool empty() const { return top == 0; }
+    void push(uint32_t x) {
+        if (top == kCap) {
+            std::fprintf(stderr, "oracle: BVH traversal stack overflow\n");
+            std::abort();
+        }
+        v[top++] = x;
+    }
+    uint32_t pop() { return v[--top]; }
+};
+
 template <class R> struct Scene {
     std::vector<Mesh<R>> meshes;
     std::vector<Sphere<R>> spheres;
@@ -506,13 +528,11 @@ template <class R> struct Scene {
     // only skips boxes that cannot hold a strictly closer hit.
     bool mesh_intersect(const Mesh<R> &m, const Ray<R> &ray, Hit<R> &hit, Counters *cnt) const {
         hit.t = std::numeric_limits<R>::infinity();
-        std::vector<uint32_t> Q;
-        Q.reserve(64);
-        Q.push_back(0);
+        NodeStack Q;
+        Q.push(0);
         const R slack = R(1) + R(64) * std::numeric_limits<R>::epsilon();
         while (!Q.empty()) {
-            uint32_t i = Q.back();
-            Q.pop_back();
+            uint32_t i = Q.pop();
             bool dead = false;
             while (uint32_t off = m.bvh[i].offset) {
                 R t0l, t0r;
@@ -527,7 +547,7 @@ template <class R> struct Scene {
                     dead = true;
                     break;
                 }
-                if (left && right) Q.push_back(i + off);
+                if (left && right) Q.push(i + off);
                 i = left ? i + 1 : i + off;
             }
             if (dead) continue;
@@ -546,12 +566,10 @@ template <class R> struct Scene {
 
     // Mesh::intersect(Seg) (src/core/primitive/mesh.cu:269-330): any hit.
     bool mesh_occluded(const Mesh<R> &m, const Seg<R> &seg) const {
-        std::vector<uint32_t> Q;
-        Q.reserve(64);
-        Q.push_back(0);
+        NodeStack Q;
+        Q.push(0);
         while (!Q.empty()) {
-            uint32_t i = Q.back();
-            Q.pop_back();
+            uint32_t i = Q.pop();
             bool dead = false;
             while (uint32_t off = m.bvh[i].offset) {
                 int left = seg_bbox(seg, m.bvh[i + 1].vmin, m.bvh[i + 1].vmax);
@@ -560,7 +578,7 @@ template <class R> struct Scene {
                     dead = true;
                     break;
                 }
-                if (left && right) Q.push_back(i + off);
+                if (left && right) Q.push(i + off);
                 i = left ? i + 1 : i + off;
             }
             if (dead) continue;

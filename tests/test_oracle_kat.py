@@ -10,6 +10,7 @@ Each oracle function is checked against something it does not restate:
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 import pytest
@@ -89,8 +90,17 @@ def test_reference_sphere_cases(real):
 
 
 # -------------------------------------------- against the reference headers --
-ref = ol.ref()
-needs_ref = pytest.mark.skipif(ref is None, reason="oracle/_ref not built (no reference sources)")
+class _LazyRef:
+    """oracle/_ref/libkoishi_ref.so, loaded by the first test that calls into
+    it (so that collecting this module, e.g. for a GPU-only run, maps
+    nothing)."""
+
+    def __getattr__(self, name):
+        return getattr(ol.ref(), name)
+
+
+ref = _LazyRef()
+needs_ref = pytest.mark.skipif(not os.path.exists(ol.REF_SO), reason="oracle/_ref not built (no reference sources)")
 
 
 def _unit(rng, n):
