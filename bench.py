@@ -1,21 +1,29 @@
 #!/usr/bin/env python3
 """Benchmark of the gfx950 Radiance path tracer on one MI355X.
 
-Workload (default): the triangle-mesh BVH scene -- a displaced geodesic
-icosphere of 100,820 triangles (+2 ground, +2 light), 1024x512, 64 spp,
-maxBounce 8, Beckmann 0.3 mesh, Lambert ground, one quad area light, FP32
-traversal/shading, FP64 per-pixel accumulation. One step = one full frame
-(33,554,432 camera paths). Scene data (geometry, BVH, light tables) is
-resident in HBM before the timed region; the accumulators stay on the device.
+Workload (default): the large scene -- 40 seeded instances of a displaced
+geodesic icosphere (25,920 triangles each, 1,036,800 in all) with random
+rotations and scales on a 20 x 20 ground plane, 4 quad area lights, a 50/50
+mix of Lambert and Beckmann materials; 1920x1080, 64 spp, maxBounce 8, FP32
+traversal/shading, FP64 per-pixel accumulation, material-sorted shading
+queues. One step = one full frame (132,710,400 camera paths). Scene data
+(geometry, BVH, light tables) is resident in HBM before the timed region;
+the accumulators stay on the device.
 
-Prints one JSON line. `roofline` is for the closest-hit traversal kernel
-(k_trace): algorithmic bytes per query = 44 B of queue I/O + node visits x
-node bytes + triangle tests x 48 B, with the visit counts measured by the
-same kernel on a 4-spp sample of the frame, divided by the kernel time from
-HIP events on the render's stream over the timed steps. `cpu_baseline` is
-the oracle's FP32 restatement of the reference CPU path (CPUMulticore thread
-scheme, reference BVH and traversal, release flags) on a bounded sample of
-the same frame on this host.
+Prints one JSON line. `value` is the plain path's paths/s. `config.cached`
+is the wall-clock to a fixed RMSE against the FP64 oracle fixture, plain
+against the radiance cache: first in the mode the workload specifies (paths
+end at their second diffuse or rough vertex, 2^16 training samples per frame
+from 1/16 of paths, Adam lr 1e-2), then in the first-vertex mode, against
+both the clamped and the unclamped RMSE of the plain 64-spp frame.
+`roofline` is for the closest-hit traversal kernel (k_trace): algorithmic
+bytes per query = 44 B of queue I/O + node visits x node bytes + triangle
+tests x 48 B, with the visit counts measured by the same kernel on a 4-spp
+sample of the frame, divided by the kernel's average launch duration from
+HIP events recorded around every k_trace launch on the stream it runs on,
+over the timed steps. `cpu_baseline` is the oracle's FP32 restatement of the
+reference CPU path (CPUMulticore thread scheme, reference BVH and traversal,
+release flags) on a bounded sample of the same frame on this host.
 """
 import argparse
 import json
@@ -43,7 +51,11 @@ def cpu_baseline(scene, target_seconds):
     orc = ol.OracleScene(scene, "f32", traversal=0)
     p = scene.params(seed=1)
     w, h = p.width, p.height
-    nthreads = max(1, (os.cpu_count() or 2) - 1)
+    # the reference's hardware_concurrency() - 1 tracer threads
+    # (src/core/tracer/cpuMulticore.cu:19-21), over the CPUs this process may
+    # use (at most 16: one GPU's share of the host)
+    ncpu = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 2))
+    nthreads = max(1, ncpu - 1)
     # probe on 8 rows at 1 spp, then size the sample (rows, then samples per
     # pixel of the frame's own strata) to ~target_seconds
     rows = np.arange(0, h, max(1, h // 8))[:8]
@@ -58,6 +70,12 @@ def cpu_baseline(scene, target_seconds):
     _, _, st = orc.render(p, 0, spp, nthreads, pix)
     orc.free()
     paths = pix.size * spp
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": paths / st["seconds"],
         "unit": "paths/s",
@@ -66,7 +84,11 @@ def cpu_baseline(scene, target_seconds):
         "sample": f"{nrows} of {h} rows x {w} px x {spp} spp ({paths} paths, {st['seconds']:.1f} s) of the "
                   f"same frame; FP32 restatement of CPUMulticoreTracer + RadianceKernel, reference BVH and "
                   f"traversal, -O3 -ffast-math -fno-finite-math-only -funroll-loops; "
-                  f"{os.cpu_count()} logical CPUs, {nthreads} tracer threads",
+                  f"{ncpu} logical CPUs usable, {nthreads} tracer threads",
+        "cpu_model": model,
+        "hardware_concurrency": ncpu,
+        "host_logical_cpus": os.cpu_count(),
+        "threads": nthreads,
     }
 
 
@@ -89,24 +111,30 @@ def rmse_vs_oracle(workload, rgb, valid, width):
             "pixels": int(pix.size), "oracle_samples_per_pixel": int(f["seeds"]) * int(f["spp"])}
 
 
-# The cache's two ways of ending paths (kr_nrc_config.flags) and the settings
-# measured best for each. "second": paths end at their second diffuse or rough
-# vertex and the cache stands for everything that vertex scatters (Mueller et
-# al. 2021), the library's default. "first" (KR_NRC_INDIRECT): paths end at
-# their first diffuse or rough vertex after its light sample and the cache
-# stands for what the rest of the path brings; the first bounce carries most
-# of the noise in both scenes, and on the large scene the second vertex is
-# often never reached. "first" with 4 steps per frame and rough threshold 0.3
-# reached the large scene's target at 48 spp and never the room's; with 8
-# steps and threshold 0.6 (glossy surfaces below it keep their paths going)
-# at 24 spp and 8 spp. At lr 3e-2 one of three "first" runs on the room
-# trained into a worse state (bias 0.148 against 0.111); at 1e-2 three of
-# three reached the targets. The "second" mode reached the room's target at
-# 32-40 spp with lr 3e-2 against 40 with 1e-2.
+# The cache's two ways of ending paths (kr_nrc_config.flags) and their
+# settings. "spec" is the workload's own statement of the cache: paths end at
+# their second diffuse or rough vertex (GlossyBSDF roughness >= 0.3 counts as
+# rough) and the cache stands for everything that vertex scatters (Mueller et
+# al. 2021); Adam lr 1e-2; 4 steps of 16,384 samples = 2^16 training samples
+# per frame from 1/16 of the paths, traced unbiased to maxBounce. "first"
+# (KR_NRC_INDIRECT) deviates from it: paths end at their first diffuse or
+# rough vertex after its light sample and the cache stands for what the rest
+# of the path brings; 8 steps (2^17 samples) per frame and rough threshold
+# 0.6 (glossy lobes below it keep their paths going). The first bounce
+# carries most of the noise of these scenes, and on the large scene the
+# second vertex is often never reached, so "first" removes more work.
 CACHE_MODES = {
-    "second": {"flags": 0, "steps_per_frame": 4, "rough_threshold": 0.3, "learning_rate": 3e-2},
+    "spec": {"flags": 0, "steps_per_frame": 4, "rough_threshold": 0.3, "learning_rate": 1e-2},
     "first": {"flags": 0x2, "steps_per_frame": 8, "rough_threshold": 0.6,  # flags: KR_NRC_INDIRECT
               "learning_rate": 1e-2},
+}
+CACHE_MODE_TEXT = {
+    "spec": "paths end at their second diffuse or rough vertex (Beckmann roughness >= 0.3), which queries the "
+            "cache for everything it scatters; 2^16 training samples per frame from 1/16 of paths traced "
+            "unbiased to maxBounce; Adam lr 1e-2",
+    "first": "deviating mode: paths end at their first diffuse or rough vertex (Beckmann roughness >= 0.6) "
+             "after its light sample, and the cache stands for the rest of the path; 2^17 training samples "
+             "per frame; Adam lr 1e-2",
 }
 
 
@@ -115,14 +143,14 @@ def bench_cache(args, kr, scene, gs, p, dev):
     (1/16 of paths train, Adam steps of 16384 samples after each frame,
     renders query the moving average of decay 0.99 per step, targets clamped
     at the brightest emitter's radiance; ending mode, steps, rough threshold
-    and learning rate per CACHE_MODES)
-    is trained for a number of 4-spp warm-up frames, then frames of s = 4, 8,
-    16, then steps of spp/8 up to spp samples are timed for both paths, with
-    RMSE against the FP64 oracle averaged over 3 seeds. The target is the
-    plain path's clamped RMSE at the workload's spp; the cached wall-clock to
-    it is the time of the cached frame at the smallest spp from which on
-    every cached frame reaches it (null when the cache's bias lies above).
-    The workload's mode is reported in full, the other mode in summary."""
+    and learning rate per CACHE_MODES) is trained for a number of 4-spp
+    warm-up frames, then frames of s = 4, 8, 16, 24, 32, 48, 64 spp (scaled to
+    the workload's spp) are timed for both paths, with RMSE against the FP64
+    oracle over 3 seeds. The targets are the plain path's RMSE at the
+    workload's spp, clamped to [0, 1] (what the reference's 8-bit output
+    shows) and unclamped; the cached wall-clock to a target is the time of
+    the cached frame at the smallest spp from which on every cached frame
+    reaches it (null when the cache's bias lies above it)."""
     import numpy as np
     pos = np.asarray(scene.positions, np.float64)
     lo, hi = pos.min(axis=0) - 0.01, pos.max(axis=0) + 0.01
@@ -132,7 +160,7 @@ def bench_cache(args, kr, scene, gs, p, dev):
                 for t, c in ((v[0], v[1] if len(v) > 1 else {}) for v in scene.shaders.values()) if t == "Emission"]
                + [max(col) * k for _, col, k in scene.point_lights] + [1.0])
     cam = scene.camera_struct()
-    mode = args.cache_mode if args.cache_mode != "auto" else "first"
+    modes = ["spec", "first"] if args.cache_mode == "both" else [args.cache_mode]
 
     fx = np.load(os.path.join(ROOT, "tests", "golden", f"oracle_{args.workload}.npz"))
     fpix, fref = fx["pixels"].astype(np.int64), fx["mean"]
@@ -147,10 +175,10 @@ def bench_cache(args, kr, scene, gs, p, dev):
         return dt, rgb[fpix] / np.maximum(val[fpix], 1)[:, None]
 
     def over_seeds(nrc, s):
-        """Best time, RMSE against the oracle averaged over 3 independent seeds
-        (as MSE), and the bias: the error of the 3-seed mean minus its own
-        noise, b^2 = |mean - ref|^2 - var / 3 (var from the spread of the
-        seeds), unclamped and clamped to [0, 1]."""
+        """Best time, RMSE against the oracle over 3 independent seeds (as
+        MSE), and the bias: the error of the 3-seed mean minus its own noise,
+        b^2 = |mean - ref|^2 - var / 3 (var from the spread of the seeds),
+        unclamped and clamped to [0, 1]."""
         dts, imgs = [], []
         for seed in (11, 12, 13):
             r = kr.Render(gs, cam, scene.params(seed=seed, spp=s))
@@ -174,16 +202,24 @@ def bench_cache(args, kr, scene, gs, p, dev):
             out["bias" + key] = float(np.sign(b2) * np.sqrt(abs(b2)))
         return out
 
-    # 4, 8, 16, then steps of spp/8 up to the workload's spp
-    ladder = sorted({s for s in [4, 8, 16] + [p.spp * k // 8 for k in range(2, 9)] if 1 <= s <= p.spp})
+    ladder = sorted({max(1, p.spp * k // 64) for k in (4, 8, 16, 24, 32, 48, 64)})
     if args.cache_ladder:
         ladder = sorted({int(x) for x in args.cache_ladder.split(",") if 1 <= int(x) <= p.spp})
     plain = {s: over_seeds(None, s) for s in ladder}
-    # the target: the plain frame's clamped RMSE at the workload's spp (what the
-    # reference's 8-bit output shows; the unclamped RMSE of these scenes is
-    # dominated by rare very bright samples and is reported beside it)
-    target = plain[ladder[-1]]["rmse_clamped"]
-    plain_time = plain[ladder[-1]]["seconds"]
+    for s in ladder:
+        log(f"[bench] plain spp {s}: {plain[s]['seconds'] * 1e3:.1f} ms rmse {plain[s]['rmse']:.4f} / "
+            f"{plain[s]['rmse_clamped']:.4f} clamped")
+    top = plain[ladder[-1]]
+    targets = {"clamped": top["rmse_clamped"], "unclamped": top["rmse"]}
+    plain_time = top["seconds"]
+
+    def first_reaching(rows, key, target):
+        # the first spp from which every larger spp also stays at or below the
+        # target (single frames are noisy, and the cache keeps training)
+        for k, r in enumerate(rows):
+            if all(x["cached"][key] <= target for x in rows[k:]):
+                return r["spp"], r["cached"]["seconds"]
+        return None, None
 
     def cached_ladder(m):
         cfg = CACHE_MODES[m]
@@ -201,45 +237,37 @@ def bench_cache(args, kr, scene, gs, p, dev):
         log(f"[bench] cache ({m}) warm-up: {warm} frames of 4 spp in {t_warm:.2f}s, loss {nrc.stats()[1]:.3f}")
         rows = []
         for s in ladder:
-            res = {"spp": s, "plain": plain[s], "cached": over_seeds(nrc, s)}
-            log(f"[bench] {m} spp {s}: plain {res['plain']['seconds'] * 1e3:.1f} ms rmse "
-                f"{res['plain']['rmse']:.4f} / {res['plain']['rmse_clamped']:.4f} clamped, cached "
-                f"{res['cached']['seconds'] * 1e3:.1f} ms rmse {res['cached']['rmse']:.4f} / "
-                f"{res['cached']['rmse_clamped']:.4f} clamped")
+            res = {"spp": s, "cached": over_seeds(nrc, s)}
+            log(f"[bench] {m} spp {s}: cached {res['cached']['seconds'] * 1e3:.1f} ms rmse "
+                f"{res['cached']['rmse']:.4f} / {res['cached']['rmse_clamped']:.4f} clamped")
             rows.append(res)
         nrc.free()
-        # the cached frame reaches the target at the first spp from which every
-        # larger spp also stays at or below it (single frames are noisy, and the
-        # cache keeps training between them)
-        cached_time = None
-        for k, r in enumerate(rows):
-            if all(x["cached"]["rmse_clamped"] <= target for x in rows[k:]):
-                cached_time = r["cached"]["seconds"]
-                break
-        return {
-            "mode": m, "settings": {**cfg, "train_fraction": 1 / 16, "batch": 16384, "ema_decay": 0.99},
-            "cached_seconds_to_target": cached_time,
-            "speedup": (plain_time / cached_time) if cached_time else None,
-            "warmup_frames": warm, "warmup_seconds": t_warm,
-            "samples_per_frame": cfg["steps_per_frame"] * 16384,
-            "bias_at_full_spp": {w: {"unclamped": rows[-1][w]["bias"], "clamped": rows[-1][w]["bias_clamped"]}
-                                 for w in ("plain", "cached")},
-            "by_spp": rows,
-        }
-
-    main_res = cached_ladder(mode)
-    out = {"target_rmse_clamped": target, "target_clamp": emax, "plain_seconds_to_target": plain_time}
-    out.update(main_res)
-    if args.cache_one_mode:
+        out = {"mode": m, "description": CACHE_MODE_TEXT[m],
+               "settings": {**cfg, "train_fraction": 1 / 16, "batch": 16384, "ema_decay": 0.99},
+               "samples_per_frame": cfg["steps_per_frame"] * 16384,
+               "warmup_frames": warm, "warmup_seconds": t_warm}
+        for name, key in (("clamped", "rmse_clamped"), ("unclamped", "rmse")):
+            spp, sec = first_reaching(rows, key, targets[name])
+            out[name] = {"target_rmse": targets[name], "cached_spp": spp, "cached_seconds": sec,
+                         "speedup": (plain_time / sec) if sec else None}
+        out["bias_at_full_spp"] = {"unclamped": rows[-1]["cached"]["bias"],
+                                   "clamped": rows[-1]["cached"]["bias_clamped"]}
+        out["by_spp"] = [{"spp": r["spp"], "seconds": r["cached"]["seconds"], "rmse": r["cached"]["rmse"],
+                          "rmse_clamped": r["cached"]["rmse_clamped"],
+                          "cache_queries": r["cached"].get("cache_queries")} for r in rows]
         return out
-    other = cached_ladder("second" if mode == "first" else "first")
-    out["other_mode"] = {
-        "mode": other["mode"], "settings": other["settings"], "speedup": other["speedup"],
-        "cached_seconds_to_target": other["cached_seconds_to_target"],
-        "bias_at_full_spp": other["bias_at_full_spp"]["cached"],
-        "by_spp": [{"spp": r["spp"], "seconds": r["cached"]["seconds"], "rmse_clamped": r["cached"]["rmse_clamped"]}
-                   for r in other["by_spp"]],
+
+    out = {
+        "plain_spp": ladder[-1], "plain_seconds": plain_time, "targets": targets, "target_clamp": emax,
+        "plain_bias_at_full_spp": {"unclamped": top["bias"], "clamped": top["bias_clamped"]},
+        "plain_by_spp": [{"spp": s, "seconds": plain[s]["seconds"], "rmse": plain[s]["rmse"],
+                          "rmse_clamped": plain[s]["rmse_clamped"]} for s in ladder],
+        "modes": {m: cached_ladder(m) for m in modes},
     }
+    if "spec" in out["modes"]:
+        sp = out["modes"]["spec"]
+        out["spec_speedup_clamped"] = sp["clamped"]["speedup"]
+        out["spec_speedup_unclamped"] = sp["unclamped"]["speedup"]
     return out
 
 
@@ -388,22 +416,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mesh", choices=["mesh", "spheres", "large", "room", "batch"])
+    ap.add_argument("--workload", default="large", choices=["mesh", "spheres", "large", "room", "batch"])
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--chunk-paths", type=int, default=0)
     ap.add_argument("--compressed-nodes", action="store_true",
                     help="8-bit quantised BVH child boxes instead of FP32")
-    ap.add_argument("--cache", action="store_true",
-                    help="also measure the radiance-cache render: RMSE vs the FP64 oracle and wall-clock "
-                         "against the plain path at equal RMSE")
+    ap.add_argument("--no-cache", action="store_true",
+                    help="skip the radiance-cache leg (plain vs cached wall-clock to the plain frame's RMSE "
+                         "against the FP64 oracle), which runs by default on workloads with an oracle fixture")
     ap.add_argument("--cache-warmup", type=int, default=64,
                     help="4-spp training frames before the cached frames are timed")
     ap.add_argument("--cache-ladder", default="",
-                    help="comma-separated spp of the timed frames (default 4, 8, 16, then steps of spp/8); "
-                         "the target is the plain frame's RMSE at the largest")
-    ap.add_argument("--cache-one-mode", action="store_true", help="measure only the chosen cache mode")
-    ap.add_argument("--cache-mode", default="auto", choices=["auto", "second", "first"],
-                    help="where cached paths end (CACHE_MODES); auto: first")
+                    help="comma-separated spp of the timed frames (default 4, 8, 16, 24, 32, 48, 64 scaled to the "
+                         "workload's spp); the targets are the plain frame's RMSE at the largest")
+    ap.add_argument("--cache-mode", default="both", choices=["both", "spec", "first"],
+                    help="where cached paths end (CACHE_MODES): spec = the workload's second diffuse or rough "
+                         "vertex, first = the first; both measures spec, then first")
     ap.add_argument("--no-material-sort", action="store_true",
                     help="shade every hit from one mixed-material queue")
     ap.add_argument("--persistent-traversal", action="store_true",
@@ -481,10 +509,13 @@ def main():
     log(f"[bench] {args.steps} steps in {elapsed:.3f}s")
     rgb, valid = rend.read()
     value = paths_per_step * args.steps / elapsed
-    trace_bytes = st.closest_rays * bytes_per_query
-    # a frame's chunks run on four streams, so launches overlap: the kernel's
-    # time is the union of its launch intervals (busy time)
-    achieved = trace_bytes / (st.trace_busy_ms * 1e-3) / 1e9 if st.trace_busy_ms > 0 else 0.0
+    # the kernel's average launch duration: HIP events around every k_trace
+    # launch, on the stream it runs on, over the timed steps (a frame's chunks
+    # run on four streams, so a launch shares the GPU with the other streams'
+    # kernels for part of its duration)
+    avg_launch_ms = st.trace_ms / max(1, st.trace_launches)
+    queries_per_launch = st.closest_rays / max(1, st.trace_launches)
+    achieved = queries_per_launch * bytes_per_query / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
 
     rmse = rmse_vs_oracle(args.workload, rgb, valid, p.width)
     # every launch timed, over as many frames as the timed region, after one
@@ -500,16 +531,8 @@ def main():
     kst = rend.stats()
     rend.set_profiling(0)
 
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "k_trace_mesh_traffic.json")
-    # the committed counter summary is for the default mesh frame only
-    if (args.workload == "mesh" and info.bvh_width == 4 and not args.chunk_paths and not args.compressed_nodes
-            and not args.persistent_traversal and not args.no_material_sort and os.path.exists(tpath)):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-
     cached = None
-    if args.cache:
+    if not args.no_cache and os.path.exists(os.path.join(ROOT, "tests", "golden", f"oracle_{args.workload}.npz")):
         rend.set_profiling(0)
         cached = bench_cache(args, kr, scene, gs, p, dev)
 
@@ -541,17 +564,26 @@ def main():
                                   "lights, 1920x1080, 64 spp, maxBounce 8",
                          "room": "closed room: 6 walls, displaced icosphere 8,000 tris, ceiling light, "
                                  "1024x768, 64 spp, maxBounce 8"}[args.workload],
+            "scene_data": "seeded synthetic scene generated in-process; the plain steps run without a cache",
             "width": p.width, "height": p.height, "spp": p.spp, "max_bounce": p.max_bounce,
             "triangles": info.num_triangles, "bvh_width": info.bvh_width, "bvh_node_bytes": info.node_bytes,
             "paths_per_step": paths_per_step,
             "mrays_per_s": (st.closest_rays + st.shadow_rays) / elapsed / 1e6,
+            # the timed steps' frames accumulated (steps x spp samples per pixel)
             "rmse_vs_fp64_oracle": rmse,
-            "wall_clock_to_rmse_s": elapsed / args.steps,
-            # frames after the timed region, every launch timed
-            "kernel_ms_per_step": {"trace": kst.trace_busy_ms / args.steps, "shade": kst.shade_busy_ms / args.steps,
-                                   "shadow": kst.shadow_busy_ms / args.steps, "other": kst.other_ms / args.steps},
+            "rmse_samples_per_pixel": p.spp * args.steps,
+            "wall_clock_per_frame_s": elapsed / args.steps,
+            # frames after the timed region, every launch timed: per kernel kind
+            # the busy time (union of its launches' intervals over the four
+            # streams) and the summed launch durations per frame
+            "kernel_busy_ms_per_step": {"trace": kst.trace_busy_ms / args.steps,
+                                        "shade": kst.shade_busy_ms / args.steps,
+                                        "shadow": kst.shadow_busy_ms / args.steps},
             "kernel_ms_per_step_summed": {"trace": kst.trace_ms / args.steps, "shade": kst.shade_ms / args.steps,
-                                          "shadow": kst.shadow_ms / args.steps},
+                                          "shadow": kst.shadow_ms / args.steps, "other": kst.other_ms / args.steps},
+            "kernel_launches_per_step": {"trace": kst.trace_launches / args.steps,
+                                         "shade": kst.shade_launches / args.steps,
+                                         "shadow": kst.shadow_launches / args.steps},
             "cached": cached,
         },
         "roofline": {
@@ -561,14 +593,17 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            # HBM bytes per launch come from rocprofv3 FETCH_SIZE / WRITE_SIZE
+            # runs of this command (committed summaries), not from this run
+            "traffic": None,
             "bytes_per_query": bytes_per_query,
-            "queries_per_launch": st.closest_rays / max(1, st.trace_launches),
-            "avg_launch_ms": st.trace_busy_ms / max(1, st.trace_launches),
-            "launch_time": "union of the k_trace launch intervals over the timed region per launch (the "
-                           "frame's chunks run on four streams, so launches overlap; the summed durations "
-                           "per launch are in avg_overlapped_launch_ms)",
-            "avg_overlapped_launch_ms": st.trace_ms / max(1, st.trace_launches),
+            "queries_per_launch": queries_per_launch,
+            "avg_launch_ms": avg_launch_ms,
+            "launches": int(st.trace_launches),
+            "launch_time": "mean of HIP-event durations of every k_trace launch over the timed steps, each "
+                           "recorded on the stream the launch runs on (four streams per frame share the GPU)",
+            "limiter": "VALU issue of divergent traversal waves and L2 misses to the MALL, not HBM bandwidth "
+                       "(SQ and TCC counters of this kernel)",
         },
         "cpu_baseline": cpu,
     }
