@@ -434,6 +434,9 @@ def main():
                          "vertex, first = the first; both measures spec, then first")
     ap.add_argument("--no-material-sort", action="store_true",
                     help="shade every hit from one mixed-material queue")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="run every chunk on one stream (KR_RENDER_ONE_STREAM): kernels do not overlap, so "
+                         "their launch durations are their own")
     ap.add_argument("--persistent-traversal", action="store_true",
                     help="traversal kernels refill finished lanes from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -471,7 +474,8 @@ def main():
         f"BVH{info.bvh_width} {info.bvh_nodes} nodes depth {info.bvh_depth}, build {info.build_seconds:.2f}s")
     p = scene.params(seed=1, chunk_paths=args.chunk_paths,
                      flags=(kr.KR_RENDER_PERSISTENT_TRAVERSAL if args.persistent_traversal else 0)
-                     | (kr.KR_RENDER_NO_MATERIAL_SORT if args.no_material_sort else 0))
+                     | (kr.KR_RENDER_NO_MATERIAL_SORT if args.no_material_sort else 0)
+                     | (kr.KR_RENDER_ONE_STREAM if args.one_stream else 0))
     rend = kr.Render(gs, scene.camera_struct(), p)
     paths_per_step = p.width * p.height * p.spp
 

@@ -85,3 +85,18 @@ def test_batch_on_two_streams_matches_jobs_alone(device, flags):
             assert np.array_equal(val, ref[j][1]), (rep, j)
             assert np.allclose(rgb, ref[j][0], rtol=1e-12, atol=1e-12), (rep, j)
     b.free()
+
+
+@pytest.mark.parametrize("which", ["mesh", "spheres"])
+def test_tile_ordered_camera_queue_is_bit_identical(device, spheres, which, monkeypatch):
+    """The camera rays of a chunk of whole samples enter the queue in 4 x 4
+    pixel tiles of 4 samples (TileOrder): each path keeps its index, RNG
+    streams and pixel, so the image equals the row-major queue's bit for bit
+    (KR_CAMERA_TILE=0), on one stream and on four."""
+    scene = sg.mesh_scene(width=256, height=128, spp=16) if which == "mesh" else spheres
+    out = []
+    for tile in ("0", "4x4x4", "8x8x1", "2x2x16"):
+        monkeypatch.setenv("KR_CAMERA_TILE", tile)
+        out.append(_render(device, scene))
+    for rgb, val in out[1:]:
+        assert np.array_equal(val, out[0][1]) and np.array_equal(rgb, out[0][0])
