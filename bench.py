@@ -226,11 +226,13 @@ def bench_cache(args, kr, scene, gs, p, dev):
         nrc = kr.Nrc(dev, lo, hi, train_log2=4, batch=16384, target_clamp=emax, ema_decay=0.99, **cfg)
         warm = 0
         t_warm = time.perf_counter()
-        rw = kr.Render(gs, cam, scene.params(seed=7, spp=4))
+        # warm-up frames of 4 spp, each a different sample range of one
+        # render: with counter-based random numbers, re-rendering one range
+        # would train on the same paths every frame
+        rw = kr.Render(gs, cam, scene.params(seed=7, spp=4 * args.cache_warmup))
         rw.attach_nrc(nrc)
         for warm in range(1, args.cache_warmup + 1):
-            rw.reset()
-            rw.run(0, 4)
+            rw.run(4 * (warm - 1), 4)
         rw.sync()
         t_warm = time.perf_counter() - t_warm
         rw.free()
@@ -425,7 +427,7 @@ def main():
                     help="skip the radiance-cache leg (plain vs cached wall-clock to the plain frame's RMSE "
                          "against the FP64 oracle), which runs by default on workloads with an oracle fixture")
     ap.add_argument("--cache-warmup", type=int, default=64,
-                    help="4-spp training frames before the cached frames are timed")
+                    help="4-spp training frames (distinct sample ranges) before the cached frames are timed")
     ap.add_argument("--cache-ladder", default="",
                     help="comma-separated spp of the timed frames (default 4, 8, 16, 24, 32, 48, 64 scaled to the "
                          "workload's spp); the targets are the plain frame's RMSE at the largest")

@@ -230,6 +230,7 @@ __device__ __forceinline__ float box_enter_nf(const RayBox &rb, float nx, float 
 struct HitRec {
     float t, u, v;
     uint32_t code;  // global triangle index, kSphereBit|sphere, or kMissCode
+    uint32_t mat;   // triangle hits: the material bucket stored with the triangle
 };
 
 template <int W>
@@ -389,6 +390,7 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
     h.t = __builtin_inff();
     h.u = h.v = 0.f;
     h.code = kMissCode;
+    h.mat = 0;
     // spheres (linear, FP64)
     for (uint32_t s = 0; s < S.nspheres; ++s) {
         double ts;
@@ -454,6 +456,7 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
                     h.u = b1;
                     h.v = b2;
                     h.code = __float_as_uint(b.w);
+                    h.mat = __float_as_uint(c.w);
                 }
             }
         }

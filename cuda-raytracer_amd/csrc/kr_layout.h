@@ -8,7 +8,8 @@
 //  first triangle) or EMPTY.
 //
 //  Triangles in leaf order, 3 float4 each (48 B):
-//    {v0.xyz, prim index bits}, {v1.xyz, global triangle index bits}, {v2.xyz, 0}
+//    {v0.xyz, prim index bits}, {v1.xyz, global triangle index bits},
+//    {v2.xyz, material bucket bits: min(shader type, 2)}
 //
 //  Per input triangle (global index, gathered only on a hit): 9 floats of
 //  vertex normals and the owning primitive.

@@ -75,9 +75,10 @@ constexpr int kLdsDepth = KR_LDS_DEPTH;
 // batch, compressed-node and persistent variants need more registers and
 // would spill.
 constexpr int trace_waves(int w, bool batch, bool q) { return (w <= 4 && !batch && !q) ? KR_TRACE_WAVES : 1; }
+constexpr int kLdsDepthMin = kLdsDepth;
 // traversal stack entries per lane beyond the LDS part, in HBM (108 in all:
 // BVH8 trees to depth 14, BVH4 to depth 34)
-constexpr int kSpillDepth = 108 - kLdsDepth;
+constexpr int kSpillDepth = 108 - kLdsDepthMin;
 
 // Host Philox4x32-10 for the light tables (same function as the device one).
 void philox_host(const uint32_t ctr_in[4], const uint32_t key_in[2], uint32_t out[4]) {
@@ -646,8 +647,9 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace(DevS
         bool hitp = active && h.code != kMissCode;
         uint32_t mat = 0;
         if (hitp && sort) {
-            uint32_t prim = (h.code & kSphereBit) ? Sx.sphere_prim[h.code & ~kSphereBit] : Sx.tri_prim[h.code];
-            mat = min(Sx.shaders[Sx.prims[prim].shader].type, 2u);  // Lambert, Glossy, everything else
+            // Lambert, Glossy, everything else: stored with the triangle, looked up for spheres
+            mat = (h.code & kSphereBit) ? min(Sx.shaders[Sx.prims[Sx.sphere_prim[h.code & ~kSphereBit]].shader].type, 2u)
+                                        : h.mat;
         }
         uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
         if (hitp && mat == 0) b0[p0] = i;
@@ -1759,7 +1761,9 @@ size_t grid_lanes(const kr_device *dev) { return size_t(std::max(dev->num_cu, 1)
 constexpr uint64_t kPipeMinPaths = 8u << 20;
 
 // Spill entries a traversal of a scene with this stack capacity can need.
-uint32_t spill_depth(uint32_t stack_cap) { return stack_cap > uint32_t(kLdsDepth) ? stack_cap - kLdsDepth : 1u; }
+uint32_t spill_depth(uint32_t stack_cap) {
+    return stack_cap > uint32_t(kLdsDepthMin) ? stack_cap - kLdsDepthMin : 1u;
+}
 
 // Spill stacks: `depth` (ref, t) entries per lane beyond the LDS part,
 // entry k of lane g at k * lanes + g; depth 0 = the largest a scene may need.
@@ -1958,13 +1962,17 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     for (uint32_t k = 0; k < d->num_triangles; ++k) {
         uint32_t t = bvh.tri_order[k];
         const float *v = &tv[9 * size_t(t)];
-        float w0, w1;
+        float w0, w1, w2;
         uint32_t pb = tri_prim[t];
+        // the material bucket of the triangle's shader (Lambert, GlossyBSDF,
+        // everything else), so that a hit is bucketed without gathers
+        uint32_t mat = std::min(d->shaders[d->prims[pb].shader].type, 2u);
         std::memcpy(&w0, &pb, 4);
         std::memcpy(&w1, &t, 4);
+        std::memcpy(&w2, &mat, 4);
         tris[3 * size_t(k) + 0] = make_float4(v[0], v[1], v[2], w0);
         tris[3 * size_t(k) + 1] = make_float4(v[3], v[4], v[5], w1);
-        tris[3 * size_t(k) + 2] = make_float4(v[6], v[7], v[8], 0.f);
+        tris[3 * size_t(k) + 2] = make_float4(v[6], v[7], v[8], w2);
     }
     for (uint32_t t = 0; t < d->num_triangles; ++t)
         for (int k = 0; k < 3; ++k)
@@ -2015,7 +2023,7 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     v.ntris = d->num_triangles;
     uint32_t cap = (width - 1) * (bvh.depth + 1) + 1;
     v.stack_cap = cap;
-    if (cap > kLdsDepth + kSpillDepth) {
+    if (cap > uint32_t(kLdsDepthMin + kSpillDepth)) {
         kr_scene_free(s);
         return fail(KR_E_OVERFLOW, "BVH too deep for the traversal stack");
     }

@@ -13,6 +13,10 @@ Output: tests/golden/oracle_<workload>.npz with pixels, mean rgb, valid
 counts, seeds, spp and a hash of the scene arrays.
 
     python tests/golden/make_oracle_images.py --workload mesh --seeds 1024
+
+`--batch-job K` renders job K of the batched uneven workload
+(scenegen.batch_workload, at the job's own resolution and spp) instead, into
+tests/golden/oracle_batch<K>.npz.
 """
 import argparse
 import hashlib
@@ -54,8 +58,14 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--pixels", type=int, default=4096, help="0 = full frame")
     ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--batch-job", type=int, default=-1, help="job of the batched workload instead")
     args = ap.parse_args()
-    s = sg.WORKLOADS[args.workload]()
+    if args.batch_job >= 0:
+        s = sg.batch_workload()[0][args.batch_job]
+        name = f"batch{args.batch_job}"
+    else:
+        s = sg.WORKLOADS[args.workload]()
+        name = args.workload
     pix = subset(s, args.pixels)
     orc = ol.OracleScene(s, "f64", traversal=1)
     acc = np.zeros((pix.size, 3))
@@ -69,7 +79,7 @@ def main():
         if (k + 1) % 64 == 0:
             print(f"{k + 1}/{args.seeds} seeds, {time.time() - t0:.0f} s", flush=True)
     mean = acc / np.maximum(valid, 1)[:, None]
-    out = os.path.join(HERE, f"oracle_{args.workload}.npz")
+    out = os.path.join(HERE, f"oracle_{name}.npz")
     np.savez_compressed(out, pixels=pix, mean=mean, valid=valid, seeds=args.seeds, spp=s.spp,
                         width=s.width, height=s.height, scene_hash=scene_hash(s))
     print(f"wrote {out}: {pix.size} px x {args.seeds * s.spp} samples in {time.time() - t0:.0f} s")
