@@ -62,8 +62,14 @@ struct Builder {
     std::vector<BinNode> nodes;
 
     static constexpr int kBins = 32;
+    // Below this depth splits are binned SAH; deeper ones split at the object
+    // median of the widest centroid axis, so that adversarial geometry (SAH
+    // peeling one triangle per level) cannot make the tree, and this
+    // recursion, O(n) deep: the binary depth stays below
+    // kMaxSahDepth + log2(n) + 1.
+    static constexpr uint32_t kMaxSahDepth = 40;
 
-    uint32_t build(uint32_t begin, uint32_t end) {
+    uint32_t build(uint32_t begin, uint32_t end, uint32_t depth = 0) {
         uint32_t me = uint32_t(nodes.size());
         nodes.emplace_back();
         Box box, cb;
@@ -88,7 +94,9 @@ struct Builder {
         if (ext[2] > ext[axis]) axis = 2;
         uint32_t mid = begin + n / 2;
         bool split_found = false;
-        if (ext[axis] > 0.f) {
+        if (depth >= kMaxSahDepth) {
+            if (n <= leaf_size) return make_leaf_node();
+        } else if (ext[axis] > 0.f) {
             Box bins[kBins];
             uint32_t cnt[kBins] = {0};
             for (auto &b : bins) b.reset();
@@ -148,8 +156,8 @@ struct Builder {
                     return cent[3 * size_t(a) + axis] < cent[3 * size_t(b) + axis];
                 });
         }
-        uint32_t l = build(begin, mid);
-        uint32_t r = build(mid, end);
+        uint32_t l = build(begin, mid, depth + 1);
+        uint32_t r = build(mid, end, depth + 1);
         nodes[me].left = l;
         nodes[me].right = r;
         return me;

@@ -248,6 +248,34 @@ void Json::set(const std::string &k, Json v) {
     obj_.emplace_back(k, std::move(v));
 }
 
+namespace {
+// A JSON string literal (RFC 8259 section 7): quote, backslash and control
+// characters escaped.
+std::string quote(const std::string &s) {
+    std::string o = "\"";
+    for (unsigned char c : s) {
+        switch (c) {
+        case '"': o += "\\\""; break;
+        case '\\': o += "\\\\"; break;
+        case '\b': o += "\\b"; break;
+        case '\f': o += "\\f"; break;
+        case '\n': o += "\\n"; break;
+        case '\r': o += "\\r"; break;
+        case '\t': o += "\\t"; break;
+        default:
+            if (c < 0x20) {
+                char buf[8];
+                std::snprintf(buf, sizeof buf, "\\u%04x", c);
+                o += buf;
+            } else {
+                o += char(c);
+            }
+        }
+    }
+    return o + "\"";
+}
+}  // namespace
+
 std::string Json::dump() const {
     std::ostringstream os;
     switch (type_) {
@@ -259,7 +287,7 @@ std::string Json::dump() const {
         os << buf;
         break;
     }
-    case String: os << '"' << s_ << '"'; break;
+    case String: os << quote(s_); break;
     case Array:
         os << '[';
         for (size_t k = 0; k < arr_.size(); ++k) os << (k ? "," : "") << arr_[k].dump();
@@ -267,7 +295,7 @@ std::string Json::dump() const {
         break;
     case Object:
         os << '{';
-        for (size_t k = 0; k < obj_.size(); ++k) os << (k ? "," : "") << '"' << obj_[k].first << "\":" << obj_[k].second.dump();
+        for (size_t k = 0; k < obj_.size(); ++k) os << (k ? "," : "") << quote(obj_[k].first) << ':' << obj_[k].second.dump();
         os << '}';
         break;
     }

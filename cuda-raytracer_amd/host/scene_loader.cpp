@@ -1,7 +1,12 @@
-// scene_loader.cpp -- reference scene JSON + OBJ import -> kr_scene_desc arrays.
+// scene_loader.cpp -- reference scene JSON + OBJ / PLY import -> kr_scene_desc arrays.
 #include "scene_loader.h"
 
 #include <algorithm>
+#include <cctype>
+#include <tuple>
+#include <array>
+#include <cstdint>
+#include <cstring>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -62,6 +67,59 @@ struct Key {
 struct KeyHash {
     size_t operator()(const Key &k) const { return std::hash<long>()(k.v * 1000003L ^ k.n); }
 };
+
+// Normals for the vertices a file gives none, as assimp's GenSmoothNormals
+// step does (the reference imports with aiProcess_GenSmoothNormals,
+// src/core/meta/scene.cu:39-44): every face has its unit geometric normal
+// (e1 x e2, normalised); a vertex gets the normalised sum of the unit normals
+// of all faces with a corner at its position (positions compared exactly),
+// each face counted once per corner -- not weighted by area.
+void smooth_missing_normals(ObjMesh &m, const std::vector<char> &needs) {
+    bool any = false;
+    for (char c : needs) any |= c != 0;
+    if (!any) return;
+    struct PosKey {
+        uint32_t b[3];
+        bool operator==(const PosKey &o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
+    };
+    struct PosHash {
+        size_t operator()(const PosKey &k) const {
+            return (size_t(k.b[0]) * 0x9E3779B1u) ^ (size_t(k.b[1]) * 0x85EBCA77u) ^ (size_t(k.b[2]) * 0xC2B2AE3Du);
+        }
+    };
+    auto key = [&](uint32_t v) {
+        PosKey k;
+        for (int c = 0; c < 3; ++c) {
+            float x = m.positions[3 * size_t(v) + c];
+            if (x == 0.f) x = 0.f;  // +0 and -0 are one position
+            std::memcpy(&k.b[c], &x, 4);
+        }
+        return k;
+    };
+    std::unordered_map<PosKey, std::array<double, 3>, PosHash> acc;
+    for (size_t t = 0; t < m.indices.size(); t += 3) {
+        const float *a = &m.positions[3 * size_t(m.indices[t])], *b = &m.positions[3 * size_t(m.indices[t + 1])],
+                    *c = &m.positions[3 * size_t(m.indices[t + 2])];
+        double e1[3] = {double(b[0]) - a[0], double(b[1]) - a[1], double(b[2]) - a[2]};
+        double e2[3] = {double(c[0]) - a[0], double(c[1]) - a[1], double(c[2]) - a[2]};
+        double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        double l = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (!(l > 0)) continue;  // a degenerate face has no normal
+        for (int k = 0; k < 3; ++k) {
+            auto &s3 = acc.emplace(key(m.indices[t + k]), std::array<double, 3>{0, 0, 0}).first->second;
+            for (int c3 = 0; c3 < 3; ++c3) s3[c3] += n[c3] / l;
+        }
+    }
+    for (size_t v = 0; v < needs.size(); ++v) {
+        if (!needs[v]) continue;
+        auto it = acc.find(key(uint32_t(v)));
+        double s3[3] = {0, 0, 0};
+        if (it != acc.end())
+            for (int c3 = 0; c3 < 3; ++c3) s3[c3] = it->second[c3];
+        double l = std::sqrt(s3[0] * s3[0] + s3[1] * s3[1] + s3[2] * s3[2]);
+        for (int c3 = 0; c3 < 3; ++c3) m.normals[3 * v + c3] = l > 0 ? float(s3[c3] / l) : 0.f;
+    }
+}
 
 }  // namespace
 
@@ -156,28 +214,7 @@ std::vector<ObjMesh> load_obj(const std::string &path, std::vector<std::string> 
         }
         // vt, s, mtllib, l, p: not used by the Radiance path
     }
-    // smooth normals where the file has none: area-weighted face normals
-    for (size_t mi = 0; mi < meshes.size(); ++mi) {
-        ObjMesh &m = meshes[mi];
-        bool any = false;
-        for (char c : needs_normal[mi]) any |= c != 0;
-        if (!any) continue;
-        std::vector<double> acc(m.positions.size(), 0.0);
-        for (size_t t = 0; t < m.indices.size(); t += 3) {
-            const float *a = &m.positions[3 * m.indices[t]], *b = &m.positions[3 * m.indices[t + 1]],
-                        *c = &m.positions[3 * m.indices[t + 2]];
-            double e1[3] = {double(b[0]) - a[0], double(b[1]) - a[1], double(b[2]) - a[2]};
-            double e2[3] = {double(c[0]) - a[0], double(c[1]) - a[1], double(c[2]) - a[2]};
-            double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
-            for (int k = 0; k < 3; ++k)
-                for (int c3 = 0; c3 < 3; ++c3) acc[3 * m.indices[t + k] + c3] += n[c3];
-        }
-        for (size_t v = 0; v < needs_normal[mi].size(); ++v) {
-            if (!needs_normal[mi][v]) continue;
-            double l = std::sqrt(acc[3 * v] * acc[3 * v] + acc[3 * v + 1] * acc[3 * v + 1] + acc[3 * v + 2] * acc[3 * v + 2]);
-            for (int c3 = 0; c3 < 3; ++c3) m.normals[3 * v + c3] = l > 0 ? float(acc[3 * v + c3] / l) : 0.f;
-        }
-    }
+    for (size_t mi = 0; mi < meshes.size(); ++mi) smooth_missing_normals(meshes[mi], needs_normal[mi]);
     material_names.clear();
     if (used_default) material_names.push_back("DefaultMaterial");
     for (auto &u : used) material_names.push_back(u);
@@ -185,6 +222,217 @@ std::vector<ObjMesh> load_obj(const std::string &path, std::vector<std::string> 
     for (auto &m : meshes)
         if (!m.indices.empty()) out.push_back(std::move(m));
     return out;
+}
+
+// PLY (Turk 1994; the format the reference's assimp import also reads):
+// ascii, binary_little_endian or binary_big_endian 1.0. The "vertex" element
+// gives x, y, z and, when present, nx, ny, nz; the "face" element a list of
+// vertex indices ("vertex_indices" or "vertex_index") per polygon, split into
+// a fan of triangles like OBJ faces (aiProcess_Triangulate on convex
+// polygons). Other elements and properties are read and skipped. A PLY file
+// has no materials: assimp gives its one mesh the material
+// "DefaultMaterial". Missing normals are generated as for OBJ.
+namespace {
+struct PlyProp {
+    std::string name;
+    int type = 0;       // scalar or list item type (bytes: 1, 2, 4, 8; float flag below)
+    bool is_float = false, is_signed = false;
+    bool list = false;
+    int count_type = 0; // list count: bytes
+    bool count_signed = false;
+};
+struct PlyElem {
+    std::string name;
+    size_t count = 0;
+    std::vector<PlyProp> props;
+};
+
+bool ply_type(const std::string &t, int &bytes, bool &is_float, bool &is_signed) {
+    static const std::map<std::string, std::tuple<int, bool, bool>> types = {
+        {"char", {1, false, true}},    {"int8", {1, false, true}},   {"uchar", {1, false, false}},
+        {"uint8", {1, false, false}},  {"short", {2, false, true}},  {"int16", {2, false, true}},
+        {"ushort", {2, false, false}}, {"uint16", {2, false, false}}, {"int", {4, false, true}},
+        {"int32", {4, false, true}},   {"uint", {4, false, false}},  {"uint32", {4, false, false}},
+        {"float", {4, true, true}},    {"float32", {4, true, true}}, {"double", {8, true, true}},
+        {"float64", {8, true, true}}};
+    auto it = types.find(t);
+    if (it == types.end()) return false;
+    std::tie(bytes, is_float, is_signed) = it->second;
+    return true;
+}
+
+struct PlyReader {
+    const std::string &data, &path;
+    size_t pos;
+    int format;  // 0 ascii, 1 little endian, 2 big endian
+    [[noreturn]] void bad(const std::string &what) const { throw std::logic_error(path + ": PLY: " + what); }
+    double binary(int bytes, bool is_float, bool is_signed) {
+        if (pos + size_t(bytes) > data.size()) bad("unexpected end of data");
+        unsigned char b[8];
+        std::memcpy(b, data.data() + pos, size_t(bytes));
+        pos += size_t(bytes);
+        if (format == 2) std::reverse(b, b + bytes);  // the host is little endian
+        if (is_float) {
+            if (bytes == 4) {
+                float f;
+                std::memcpy(&f, b, 4);
+                return f;
+            }
+            double d;
+            std::memcpy(&d, b, 8);
+            return d;
+        }
+        uint64_t u = 0;
+        std::memcpy(&u, b, size_t(bytes));
+        if (is_signed && bytes < 8 && (u >> (8 * bytes - 1)) & 1u) u |= ~uint64_t(0) << (8 * bytes);
+        return is_signed ? double(int64_t(u)) : double(u);
+    }
+    double ascii() {
+        while (pos < data.size() && std::isspace(static_cast<unsigned char>(data[pos]))) ++pos;
+        if (pos >= data.size()) bad("unexpected end of data");
+        const char *b = data.c_str() + pos;
+        char *e = nullptr;
+        double v = std::strtod(b, &e);
+        if (e == b) bad("bad number");
+        pos += size_t(e - b);
+        return v;
+    }
+    double value(int bytes, bool is_float, bool is_signed) {
+        return format == 0 ? ascii() : binary(bytes, is_float, is_signed);
+    }
+};
+}  // namespace
+
+std::vector<ObjMesh> load_ply(const std::string &path, std::vector<std::string> &material_names) {
+    const std::string data = read_file(path);
+    // header: lines up to "end_header"
+    size_t pos = 0;
+    auto next_line = [&](std::string &line) {
+        if (pos >= data.size()) return false;
+        size_t e = data.find('\n', pos);
+        if (e == std::string::npos) e = data.size();
+        line = data.substr(pos, e - pos);
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        pos = e + 1;
+        return true;
+    };
+    std::string line;
+    if (!next_line(line) || line != "ply") throw std::logic_error(path + ": not a PLY file");
+    int format = -1;
+    std::vector<PlyElem> elems;
+    bool ended = false;
+    while (next_line(line)) {
+        std::istringstream ls(line);
+        std::string kw;
+        if (!(ls >> kw)) continue;
+        if (kw == "format") {
+            std::string f, ver;
+            ls >> f >> ver;
+            format = f == "ascii" ? 0 : f == "binary_little_endian" ? 1 : f == "binary_big_endian" ? 2 : -1;
+            if (format < 0) throw std::logic_error(path + ": PLY: unknown format '" + f + "'");
+        } else if (kw == "element") {
+            PlyElem el;
+            long long n = -1;
+            ls >> el.name >> n;
+            if (n < 0) throw std::logic_error(path + ": PLY: bad element line");
+            el.count = size_t(n);
+            elems.push_back(el);
+        } else if (kw == "property") {
+            if (elems.empty()) throw std::logic_error(path + ": PLY: property before any element");
+            PlyProp pr;
+            std::string t;
+            ls >> t;
+            if (t == "list") {
+                std::string ct, it;
+                ls >> ct >> it >> pr.name;
+                bool cf;
+                if (!ply_type(ct, pr.count_type, cf, pr.count_signed) || cf ||
+                    !ply_type(it, pr.type, pr.is_float, pr.is_signed))
+                    throw std::logic_error(path + ": PLY: bad list property");
+                pr.list = true;
+            } else {
+                if (!ply_type(t, pr.type, pr.is_float, pr.is_signed))
+                    throw std::logic_error(path + ": PLY: unknown property type '" + t + "'");
+                ls >> pr.name;
+            }
+            elems.back().props.push_back(pr);
+        } else if (kw == "end_header") {
+            ended = true;
+            break;
+        }
+        // comment, obj_info: ignored
+    }
+    if (!ended || format < 0) throw std::logic_error(path + ": PLY: incomplete header");
+    PlyReader rd{data, path, pos, format};
+    std::vector<float> V, N;
+    bool has_n = false;
+    ObjMesh m{"", "DefaultMaterial", {}, {}, {}};
+    for (const PlyElem &el : elems) {
+        int ix[6] = {-1, -1, -1, -1, -1, -1};
+        int fl = -1;
+        for (size_t k = 0; k < el.props.size(); ++k) {
+            static const char *names[6] = {"x", "y", "z", "nx", "ny", "nz"};
+            for (int c = 0; c < 6; ++c)
+                if (el.props[k].name == names[c]) ix[c] = int(k);
+            if (el.props[k].list && (el.props[k].name == "vertex_indices" || el.props[k].name == "vertex_index"))
+                fl = int(k);
+        }
+        bool is_vertex = el.name == "vertex", is_face = el.name == "face";
+        if (is_vertex) {
+            if (ix[0] < 0 || ix[1] < 0 || ix[2] < 0) throw std::logic_error(path + ": PLY: vertex without x, y, z");
+            has_n = ix[3] >= 0 && ix[4] >= 0 && ix[5] >= 0;
+            V.reserve(3 * el.count);
+            if (has_n) N.reserve(3 * el.count);
+        }
+        if (is_face && fl < 0) throw std::logic_error(path + ": PLY: face without vertex_indices");
+        std::vector<double> vals(el.props.size());
+        std::vector<uint32_t> poly;
+        for (size_t i = 0; i < el.count; ++i) {
+            for (size_t k = 0; k < el.props.size(); ++k) {
+                const PlyProp &pr = el.props[k];
+                if (!pr.list) {
+                    vals[k] = rd.value(pr.type, pr.is_float, pr.is_signed);
+                    continue;
+                }
+                double cnt = rd.value(pr.count_type, false, pr.count_signed);
+                if (cnt < 0) rd.bad("negative list length");
+                bool keep = is_face && int(k) == fl;
+                if (keep) poly.clear();
+                for (size_t j = 0; j < size_t(cnt); ++j) {
+                    double x = rd.value(pr.type, pr.is_float, pr.is_signed);
+                    if (keep) {
+                        if (x < 0 || x >= double(V.size() / 3)) rd.bad("vertex index out of range");
+                        poly.push_back(uint32_t(x));
+                    }
+                }
+            }
+            if (is_vertex) {
+                for (int c = 0; c < 3; ++c) V.push_back(float(vals[size_t(ix[c])]));
+                if (has_n)
+                    for (int c = 3; c < 6; ++c) N.push_back(float(vals[size_t(ix[c])]));
+            } else if (is_face) {
+                if (poly.size() < 3) rd.bad("face with < 3 vertices");
+                for (size_t k = 1; k + 1 < poly.size(); ++k) m.indices.insert(m.indices.end(), {poly[0], poly[k], poly[k + 1]});
+            }
+        }
+    }
+    m.positions = V;
+    std::vector<char> needs(V.size() / 3, has_n ? 0 : 1);
+    m.normals = has_n ? N : std::vector<float>(V.size(), 0.f);
+    smooth_missing_normals(m, needs);
+    material_names.assign(1, "DefaultMaterial");
+    std::vector<ObjMesh> out;
+    if (!m.indices.empty()) out.push_back(std::move(m));
+    return out;
+}
+
+std::vector<ObjMesh> load_mesh_file(const std::string &path, std::vector<std::string> &material_names) {
+    std::string ext;
+    size_t d = path.find_last_of('.');
+    if (d != std::string::npos)
+        for (char c : path.substr(d + 1)) ext.push_back(char(std::tolower(static_cast<unsigned char>(c))));
+    if (ext == "ply") return load_ply(path, material_names);
+    return load_obj(path, material_names);
 }
 
 namespace {
@@ -359,7 +607,7 @@ LoadedScene load_scene_json(const Json &doc, const std::string &base_dir) {
                     if (!probe) path = base_dir + "/" + path;
                 }
                 std::vector<std::string> mats;
-                auto meshes = load_obj(path, mats);
+                auto meshes = load_mesh_file(path, mats);
                 // assimp's material list comes first; spheres append (scene.cu:52,141-154)
                 names = mats;
                 for (auto &m : meshes) {

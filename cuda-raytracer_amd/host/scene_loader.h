@@ -24,6 +24,10 @@ struct ObjMesh {
 // first (when faces use it) and then every `usemtl` name in order of first
 // mention, with or without faces.
 std::vector<ObjMesh> load_obj(const std::string &path, std::vector<std::string> &material_names);
+// PLY import (ascii and binary); one mesh with the material "DefaultMaterial".
+std::vector<ObjMesh> load_ply(const std::string &path, std::vector<std::string> &material_names);
+// By extension: ".ply" (any case) reads PLY, anything else OBJ.
+std::vector<ObjMesh> load_mesh_file(const std::string &path, std::vector<std::string> &material_names);
 
 struct LoadedScene {
     std::vector<float> positions, normals;

@@ -200,3 +200,138 @@ def test_cr_cli_list_help_and_errors(tmp_path):
     assert out.returncode == 1 and "no such type: OptiX" in out.stdout
     out = subprocess.run([cr, path, "--bogus"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 1 and "does not exist" in out.stdout
+
+
+# ---------------------------------------------------------------- PLY import --
+def _ply_bytes(fmt, pos, faces, nrm=None, extra=False):
+    """A PLY file of the given format with float vertices (x y z [nx ny nz]
+    [uchar red green blue]), uchar/int face lists and, with `extra`, an
+    unused edge element after the faces."""
+    head = ["ply", f"format {fmt} 1.0", "comment test", f"element vertex {len(pos)}",
+            "property float x", "property float y", "property float z"]
+    if nrm is not None:
+        head += ["property float nx", "property float ny", "property float nz"]
+    if extra:
+        head += ["property uchar red", "property uchar green", "property uchar blue"]
+    head += [f"element face {len(faces)}", "property list uchar int vertex_indices"]
+    if extra:
+        head += ["element edge 1", "property int vertex1", "property int vertex2"]
+    head += ["end_header"]
+    out = ("\n".join(head) + "\n").encode()
+    if fmt == "ascii":
+        rows = []
+        for i, p in enumerate(pos):
+            r = [repr(float(np.float32(x))) for x in p]
+            if nrm is not None:
+                r += [repr(float(np.float32(x))) for x in nrm[i]]
+            if extra:
+                r += ["10", "20", "30"]
+            rows.append(" ".join(r))
+        for f in faces:
+            rows.append(" ".join([str(len(f))] + [str(i) for i in f]))
+        if extra:
+            rows.append("0 1")
+        return out + ("\n".join(rows) + "\n").encode()
+    e = "<" if fmt == "binary_little_endian" else ">"
+    for i, p in enumerate(pos):
+        out += struct.pack(e + "3f", *p)
+        if nrm is not None:
+            out += struct.pack(e + "3f", *nrm[i])
+        if extra:
+            out += bytes([10, 20, 30])
+    for f in faces:
+        out += struct.pack(e + "B" + "i" * len(f), len(f), *f)
+    if extra:
+        out += struct.pack(e + "2i", 0, 1)
+    return out
+
+
+def _load_mesh(tmp_path, name, payload):
+    (tmp_path / name).write_bytes(payload) if isinstance(payload, bytes) else (tmp_path / name).write_text(payload)
+    doc = {"scene": {"camera": [{"lens": "pinhole", "fovx": 60, "position": [0, -5, 1], "target": [0, 1, 0]}],
+                     "assets": [["import", {"path": name}]],
+                     "shaders": {"DefaultMaterial": ["Lambert", {"color": [0.5, 0.5, 0.5]}]}},
+           "tracer": ["CPUMulticore", {"kernel": ["Radiance", {"maxBounce": 2}]}]}
+    p = tmp_path / "s.json"
+    p.write_text(json.dumps(doc))
+    ls = host.LoadedScene(str(p))
+    return ls
+
+
+def _corner_geometry(ls):
+    """Triangle corners and normals, and the primitives copied out of the
+    loaded scene (whose arrays die with it)."""
+    pos, nrm, idx, prims, _ = ls.arrays()
+    pr = [(int(p.type), int(p.shader), int(p.tri_begin), int(p.tri_count)) for p in prims]
+    return pos[idx.astype(np.int64)], nrm[idx.astype(np.int64)], pr, ls.shader_names()
+
+
+def test_ply_formats_and_obj_agree(tmp_path):
+    """The same quad + pentagon with normals as ascii, little- and big-endian
+    PLY (with extra color properties and an extra element to skip) and as OBJ:
+    identical triangle corners and normals, one mesh primitive on the shader
+    "DefaultMaterial", polygons split into fans."""
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-1, 1, (9, 3)).astype(np.float32)
+    nrm = rng.normal(size=(9, 3))
+    nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
+    faces = [[0, 1, 2, 3], [4, 5, 6, 7, 8]]
+    got = []
+    for fmt in ("ascii", "binary_little_endian", "binary_big_endian"):
+        ls = _load_mesh(tmp_path, f"m_{fmt}.ply", _ply_bytes(fmt, pos, faces, nrm, extra=True))
+        got.append(_corner_geometry(ls))
+    obj = "".join("v " + " ".join(repr(float(x)) for x in p) + "\n" for p in pos)
+    obj += "".join("vn " + " ".join(repr(float(x)) for x in n) + "\n" for n in nrm)
+    obj += "".join("f " + " ".join(f"{i + 1}//{i + 1}" for i in f) + "\n" for f in faces)
+    got.append(_corner_geometry(_load_mesh(tmp_path, "m.obj", obj)))
+    cp, cn, prims, names = got[0]
+    assert cp.shape == (5, 3, 3)  # 2 + 3 fan triangles
+    fan = [[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7], [4, 7, 8]]
+    assert np.array_equal(cp, pos[np.array(fan)]) and np.array_equal(cn, nrm[np.array(fan)])
+    assert len(prims) == 1 and prims[0][3] == 5 and names[prims[0][1]] == "DefaultMaterial"
+    for p2, n2, pr2, nm2 in got[1:]:
+        assert np.array_equal(p2, cp) and np.array_equal(n2, cn)
+        assert len(pr2) == 1 and nm2[pr2[0][1]] == "DefaultMaterial"
+
+
+@pytest.mark.parametrize("fmt", ["ply", "obj"])
+def test_generated_normals_are_unit_face_normal_sums(tmp_path, fmt):
+    """Without normals in the file, a vertex gets the normalised sum of the
+    UNIT normals of the faces touching its position (assimp's
+    GenSmoothNormals), not an area-weighted sum: a tent of one large and one
+    small triangle sharing an edge has normalize(n1 + n2) on the shared
+    corners, where area weighting would lean towards the large face's normal;
+    the other corners keep their own face's normal."""
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 4, 2], [0.3, -0.5, 0.4]], np.float32)
+    faces = [[0, 1, 2], [1, 0, 3]]
+    if fmt == "ply":
+        ls = _load_mesh(tmp_path, "t.ply", _ply_bytes("ascii", pos, faces))
+    else:
+        obj = "".join("v " + " ".join(repr(float(x)) for x in p) + "\n" for p in pos)
+        obj += "".join("f " + " ".join(str(i + 1) for i in f) + "\n" for f in faces)
+        ls = _load_mesh(tmp_path, "t.obj", obj)
+    cp, cn, _, _ = _corner_geometry(ls)
+    p64 = pos.astype(np.float64)
+
+    def unit(f):
+        n = np.cross(p64[f[1]] - p64[f[0]], p64[f[2]] - p64[f[0]])
+        return n / np.linalg.norm(n)
+    n1, n2 = unit(faces[0]), unit(faces[1])
+    shared = (n1 + n2) / np.linalg.norm(n1 + n2)
+    want = {0: shared, 1: shared, 2: n1, 3: n2}
+    for t, f in enumerate(faces):
+        for k, v in enumerate(f):
+            assert np.allclose(cp[t, k], pos[v])
+            assert np.allclose(cn[t, k], want[v], atol=1e-6), (v, cn[t, k], want[v])
+    a1 = np.linalg.norm(np.cross(p64[1] - p64[0], p64[2] - p64[0]))
+    a2 = np.linalg.norm(np.cross(p64[0] - p64[1], p64[3] - p64[1]))
+    area = (a1 * n1 + a2 * n2) / np.linalg.norm(a1 * n1 + a2 * n2)
+    assert np.abs(area - shared).max() > 1e-2  # the two rules differ here
+
+
+def test_bad_ply_is_rejected(tmp_path):
+    pos = np.zeros((3, 3), np.float32)
+    for payload, word in ((b"plx\n", "not a PLY"), (_ply_bytes("ascii", pos, [[0, 1, 5]]), "out of range"),
+                          (_ply_bytes("binary_little_endian", pos, [[0, 1, 2]])[:-3], "end of data")):
+        with pytest.raises(RuntimeError, match=word):
+            _load_mesh(tmp_path, "b.ply", payload)
