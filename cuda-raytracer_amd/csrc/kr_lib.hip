@@ -1499,19 +1499,58 @@ __global__ __launch_bounds__(64) void k_nrc_fwd_bwd(nrc::Weights W, TrainSource 
             for (int nt = 0; nt < 4; ++nt)
                 if (mt < MT) Ds[mt * 4 + nt][lane] = dz[mt][nt];
         __syncthreads();
-        // dW[i][o] = sum_n H[n][i] dZ[n][o]; db[o] = sum_n dZ[n][o]
-        for (int e = lane; e < IN * OUT; e += 64) {
-            int i = e / OUT, o = e % OUT;
-            float acc = 0.f;
-            int hkt = i >> 4, hq = (i & 15) >> 2, hr = i & 3;
-            int dmt = o >> 4, dq = (o & 15) >> 2, dr = o & 3;
-            for (int n = 0; n < 64; ++n) {
-                int nt = n >> 4, sl = n & 15;
-                float hv = from_bf16(uint16_t(Hs[hbase + hkt * 4 + nt][sl + 16 * hq][hr]));
-                float dv = Ds[dmt * 4 + nt][sl + 16 * dq][dr];
-                acc += hv * dv;
+        // dW[i][o] = sum_n H^T[i][n] dZ[n][o] on MFMA, the 64 samples as two
+        // k = 32 steps (k tiles 2p, 2p + 1 paired as in mfma_pair). A = H^T
+        // (exactly BF16: the activations are), B = dZ split into BF16 high
+        // and low parts, dZ = hi + lo + O(2^-17 dZ), so the FP32 sums keep
+        // FP32-level accuracy. Lane (s, q) needs H^T[16 it + s][n] and
+        // dZ[n][16 ot + s] for n = 32 p + 16 h + 4 q + r, which the forward
+        // layouts of Hs and Ds hold at lane 16 (s >> 2) + 4 q + r, element
+        // s & 3, of tile (2 p + h).
+        {
+            const int src_lane = 16 * (s >> 2) + 4 * q, el = s & 3;
+            v4s bhi[4][2][2], blo[4][2][2];
+#pragma unroll
+            for (int ot = 0; ot < 4; ++ot)
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+                        if (ot < MT) {
+                            float f[4], lo[4];
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                f[r] = Ds[ot * 4 + 2 * p + h][src_lane + r][el];
+                                lo[r] = f[r] - from_bf16(to_bf16(f[r]));
+                            }
+                            bhi[ot][p][h] = pack4(f[0], f[1], f[2], f[3]);
+                            blo[ot][p][h] = pack4(lo[0], lo[1], lo[2], lo[3]);
+                        }
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                if (it >= KT) break;
+                v4s a[2][2];
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const v4s *src = &Hs[hbase + it * 4 + 2 * p + h][src_lane];
+                        a[p][h] = v4s{src[0][el], src[1][el], src[2][el], src[3][el]};
+                    }
+#pragma unroll
+                for (int ot = 0; ot < 4; ++ot) {
+                    if (ot >= MT) break;
+                    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int p = 0; p < 2; ++p) {
+                        acc = mfma_pair(a[p][0], a[p][1], bhi[ot][p][0], bhi[ot][p][1], acc);
+                        acc = mfma_pair(a[p][0], a[p][1], blo[ot][p][0], blo[ot][p][1], acc);
+                    }
+                    // lane (s, q) holds dW[16 it + 4 q + r][16 ot + s]
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) grow[w_off(l) + (16 * it + 4 * q + r) * OUT + 16 * ot + s] = acc[r];
+                }
             }
-            grow[w_off(l) + e] = acc;
         }
         if (lane < OUT) {
             int dmt = lane >> 4, dq = (lane & 15) >> 2, dr = lane & 3;
@@ -1558,14 +1597,23 @@ __global__ __launch_bounds__(64) void k_nrc_fwd_bwd(nrc::Weights W, TrainSource 
 // ema_a = (1 - d) / (1 - d^t) at step t, the bias-corrected exponential
 // moving average of decay d (ema_a = 1 at the first step), and refreshes
 // its BF16 W^T.
-__global__ void k_nrc_adam(float *master, uint16_t *wt, uint16_t *w, float *m, float *v, const float *partial,
-                           uint32_t rows, float lr, float b1, float b2, float eps, float c1, float c2,
-                           const uint32_t *pool_total, float *ema, uint16_t *wt_ema, float ema_a) {
+// Blocks of kAdamParams x kAdamGroups threads: the kAdamGroups threads of a
+// parameter sum every kAdamGroups-th partial row (coalesced across the
+// block's consecutive parameters), the block adds the group sums in LDS in a
+// fixed order, and the first group's thread takes the step.
+constexpr int kAdamParams = 32, kAdamGroups = 8;
+__global__ __launch_bounds__(kAdamParams *kAdamGroups) void k_nrc_adam(
+    float *master, uint16_t *wt, uint16_t *w, float *m, float *v, const float *partial, uint32_t rows, float lr,
+    float b1, float b2, float eps, float c1, float c2, const uint32_t *pool_total, float *ema, uint16_t *wt_ema,
+    float ema_a) {
     using namespace nrc;
     if (pool_total && *pool_total == 0) return;
-    for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < kNumParams; p += gridDim.x * blockDim.x) {
+    __shared__ float gs[kAdamGroups][kAdamParams];
+    const int x = threadIdx.x % kAdamParams, y = threadIdx.x / kAdamParams;
+    auto adam_update = [&](int p, int x) {
         float g = 0.f;
-        for (uint32_t r = 0; r < rows; ++r) g += partial[size_t(r) * kNumParams + p];
+#pragma unroll
+        for (int k = 0; k < kAdamGroups; ++k) g += gs[k][x];
         float mm = b1 * m[p] + (1.f - b1) * g;
         float vv = b2 * v[p] + (1.f - b2) * g * g;
         m[p] = mm;
@@ -1587,6 +1635,18 @@ __global__ void k_nrc_adam(float *master, uint16_t *wt, uint16_t *w, float *m, f
                 if (ema) wt_ema[bw_off(l) + o * in + i] = to_bf16(av);
             }
         }
+    };
+    for (int pb = blockIdx.x * kAdamParams; pb < kNumParams; pb += gridDim.x * kAdamParams) {
+        const int p = pb + x;
+        float part = 0.f;
+        if (p < kNumParams)
+            for (uint32_t r = uint32_t(y); r < rows; r += kAdamGroups) part += partial[size_t(r) * kNumParams + p];
+        gs[y][x] = part;
+        __syncthreads();
+        // every thread reaches both barriers of an iteration; only the first
+        // group's threads of real parameters take the step
+        if (y == 0 && p < kNumParams) adam_update(p, x);
+        __syncthreads();  // gs is rewritten by the next parameter block
     }
 }
 
@@ -2970,7 +3030,8 @@ int nrc_step(kr_nrc *n, const TrainSource &src, uint32_t batch, uint32_t rows) {
     float lr = n->cfg.learning_rate > 0.f ? n->cfg.learning_rate : 1e-2f;
     const double d = n->cfg.ema_decay;
     float ema_a = n->ema ? float((1.0 - d) / (1.0 - std::pow(d, double(n->step)))) : 0.f;
-    hipLaunchKernelGGL(k_nrc_adam, dim3(96), dim3(256), 0, st, n->master, n->wt, n->w, n->adam_m, n->adam_v,
+    hipLaunchKernelGGL(k_nrc_adam, dim3((nrc::kNumParams + kAdamParams - 1) / kAdamParams),
+                       dim3(kAdamParams * kAdamGroups), 0, st, n->master, n->wt, n->w, n->adam_m, n->adam_v,
                        n->partial, rows, lr, b1, b2, eps, c1, c2, src.pool_total, n->ema, n->wt_ema, ema_a);
     KR_HIP(hipGetLastError());
     return KR_OK;
