@@ -349,8 +349,8 @@ struct kr_render {
 
 namespace {
 void resolve_events(kr_render *r);
-void nrc_after_chunk(kr_render *r, Pipe &pp);
-int nrc_after_run(kr_render *r);
+void nrc_after_chunk(kr_render *r, Pipe &pp, kr_nrc *n, const NrcRender &nr);
+int nrc_after_run(kr_render *r, kr_nrc *n);
 }  // namespace
 
 // ------------------------------------------------------------- kernels --
@@ -2580,16 +2580,10 @@ struct JobChunk {
     uint64_t seed;
     double *acc;
     uint32_t *valid;
+    kr_nrc *nrc;   // the job's radiance cache, or null
+    NrcRender nr;  // its view over the pipe's cache buffers (enabled = 0 without one)
 };
 
-// Enqueues one chunk of P paths starting at global path g0 on pipe pp:
-// camera, then maxBounce + 1 rounds of trace / shade / shadow, then the FP64
-// accumulation, which first waits for `acc_after` (the other pipe's last
-// accumulation, or nullptr) and then records pp.acc_done. Only kernel
-// launches and events on the pipe's stream, so the sequence can be captured
-// into a HIP graph. bv != nullptr: a batch chunk that mixes jobs (BATCH =
-// true); jc != nullptr: a batch chunk inside one job, run by the single-render
-// kernels with g0 the job's own path index.
 // Tile order of a chunk's camera rays (TileOrder): 2 x 2 pixels with 16
 // samples each fill a wave when the chunk's sample count allows, else 4 x 4
 // pixels with 4 samples, else 8 x 8 pixels with one (on the 1M-triangle
@@ -2624,6 +2618,14 @@ TileOrder tile_order(const kr_render *r, uint64_t g0, uint32_t P) {
     return to;
 }
 
+// Enqueues one chunk of P paths starting at global path g0 on pipe pp:
+// camera, then maxBounce + 1 rounds of trace / shade / shadow, then the FP64
+// accumulation, which first waits for `acc_after` (the other pipe's last
+// accumulation, or nullptr) and then records pp.acc_done. Only kernel
+// launches and events on the pipe's stream, so the sequence can be captured
+// into a HIP graph. bv != nullptr: a batch chunk that mixes jobs (BATCH =
+// true); jc != nullptr: a batch chunk inside one job, run by the single-render
+// kernels with g0 the job's own path index.
 template <int W, bool BATCH, bool Q>
 int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *bv, hipEvent_t acc_after,
               const JobChunk *jc = nullptr) {
@@ -2639,6 +2641,9 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
     // the profiler (single renders only); the one-hit kernels record no vertices
     ProfView pv = (BATCH || jc) ? ProfView{} : r->pv;
     if (r->prm.kernel != KR_KERNEL_RADIANCE) pv.depth = 0;
+    // the radiance cache of the chunk: the render's, or a batch job's
+    const NrcRender nrv = jc ? jc->nr : (BATCH ? NrcRender{} : pp.nr);
+    kr_nrc *const nrc = jc ? jc->nrc : r->nrc;
     int grid = r->grid;
     int cam_grid = int(std::min<size_t>((P + kBlock - 1) / kBlock, size_t(grid)));
     int cur = 0;
@@ -2649,7 +2654,7 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
         TileOrder to{};
         if (!BATCH && !jc && !pv.enabled) to = tile_order(r, g0, P);
         hipLaunchKernelGGL((k_camera<BATCH>), dim3(cam_grid), dim3(kBlock), 0, st, lens, T, g0, P, npix,
-                           pp.q[0], pp.L, pp.counters, C_RAY0, pp.nr, uint32_t(seed), uint32_t(seed >> 32), pv, to);
+                           pp.q[0], pp.L, pp.counters, C_RAY0, nrv, uint32_t(seed), uint32_t(seed >> 32), pv, to);
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
     bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
@@ -2696,7 +2701,7 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
         a.max_bounce = max_bounce;
         a.seed_lo = uint32_t(seed);
         a.seed_hi = uint32_t(seed >> 32);
-        a.nr = pp.nr;
+        a.nr = nrv;
         a.T = T;
         a.pv = pv;
         {
@@ -2710,14 +2715,14 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
             Timed tm(r, K_SHADOW, st);
             if (dyn)
                 hipLaunchKernelGGL((k_shadow_dyn<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, pp.sq,
-                                   pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, pp.nr, b, pv);
+                                   pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, nrv, b, pv);
             else
                 hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, pp.sq,
-                                   pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, pp.nr, b, pv);
+                                   pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, nrv, b, pv);
         }
         cur = 1 - cur;
     }
-    if (!BATCH && r->nr.enabled && kern == KR_KERNEL_RADIANCE) nrc_after_chunk(r, pp);
+    if (!BATCH && nrv.enabled && nrc && kern == KR_KERNEL_RADIANCE) nrc_after_chunk(r, pp, nrc, nrv);
     if (pv.enabled) {
         Timed tm(r, K_OTHER, st);
         hipLaunchKernelGGL(k_prof_final, dim3(cam_grid), dim3(kBlock), 0, st, pv, pp.L, g0, P);
@@ -2799,7 +2804,7 @@ KR_EXPORT int kr_render_run(kr_render *r, uint32_t spp_begin, uint32_t spp_count
     if (rc) return rc;
     r->paths += gend - g0;
     if (r->nr.enabled) {
-        int rc = nrc_after_run(r);
+        int rc = nrc_after_run(r, r->nrc);
         if (rc) return rc;
     }
     r->last_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -3377,22 +3382,20 @@ __global__ __launch_bounds__(256) void k_nrc_extract(NrcRender nr, const uint32_
 }
 
 namespace {
-void nrc_after_chunk(kr_render *r, Pipe &pp) {
-    kr_nrc *n = r->nrc;
+void nrc_after_chunk(kr_render *r, Pipe &pp, kr_nrc *n, const NrcRender &nr) {
     hipStream_t st = pp.stream;
     {
         Timed tm(r, K_OTHER, st);
         int grid = int(std::min<size_t>((size_t(r->P) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
-        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_render_view(n), pp.nr, pp.counters, pp.L,
+        hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_render_view(n), nr, pp.counters, pp.L,
                            r->stats);
-        int egrid = int(std::min<size_t>((size_t(pp.nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
-        hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, pp.nr, pp.counters, n->pool_raw, n->pool_tgt,
+        int egrid = int(std::min<size_t>((size_t(nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
+        hipLaunchKernelGGL(k_nrc_extract, dim3(egrid), dim3(256), 0, st, nr, pp.counters, n->pool_raw, n->pool_tgt,
                            n->pool_total, n->pool_cap, n->cfg.target_clamp);
     }
 }
 
-int nrc_after_run(kr_render *r) {
-    kr_nrc *n = r->nrc;
+int nrc_after_run(kr_render *r, kr_nrc *n) {
     if (n->cfg.flags & KR_NRC_FROZEN) return KR_OK;
     Timed tm(r, K_OTHER);
     for (uint32_t i = 0; i < n->cfg.steps_per_frame; ++i) {
@@ -3500,6 +3503,12 @@ struct kr_batch {
     double capture_seconds = 0, seconds = 0;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool timed = false;
+    // radiance caches of the jobs (kr_batch_attach_nrc): per job a cache or
+    // null; per pipe one set of cache buffers, sized for the batch's chunks
+    // and shared by the pipe's cached chunks (a pipe's chunks run in order)
+    std::vector<kr_nrc *> nrc;
+    NrcRender pnr[kMaxPipes] = {};
+    std::vector<void *> nrc_allocs;
 };
 
 namespace {
@@ -3603,6 +3612,20 @@ std::vector<std::pair<uint64_t, uint32_t>> batch_chunks(const kr_batch *b, const
 // chunk list. A batch's pipes accumulate into images of their own (read back
 // summed), so the pipes never wait for each other and each pipe's chunks form
 // a graph of their own.
+// A cached job's view over pipe pi's cache buffers: the buffers, and the
+// job's cache's path selection and ending rule (as kr_render_attach_nrc).
+NrcRender batch_nrc_view(const kr_batch *b, int pi, const kr_nrc *n) {
+    NrcRender nr = b->pnr[pi];
+    nr.enabled = 1;
+    nr.train_log2 = n->cfg.train_log2;
+    nr.train_mask = (1u << n->cfg.train_log2) - 1u;
+    nr.rough_threshold = n->cfg.rough_threshold;
+    nr.indirect = (n->cfg.flags & KR_NRC_INDIRECT) ? 1 : 0;
+    nr.end_at = nr.indirect ? 1u : 2u;
+    nr.train_cap = uint32_t(std::min<uint64_t>(b->r->P, (uint64_t(b->r->P) >> n->cfg.train_log2) + 2));
+    return nr;
+}
+
 int batch_enqueue_pipe(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P, int pi) {
     kr_render *r = b->r;
     const std::vector<HostJob> &hj = *bv.jobs;
@@ -3623,6 +3646,10 @@ int batch_enqueue_pipe(kr_batch *b, const BatchView &bv, uint64_t total, uint32_
             const kr_batch_job &J = b->jobs[j];
             JobChunk jc{b->up[J.scene]->view, make_lens(&J.camera, &J.params), hj[j].npix, J.params.max_bounce,
                         J.params.seed, pp.acc + 3 * size_t(hj[j].acc_off), pp.valid + hj[j].acc_off};
+            if (j < b->nrc.size() && b->nrc[j]) {
+                jc.nrc = b->nrc[j];
+                jc.nr = batch_nrc_view(b, pi, b->nrc[j]);
+            }
             return run_chunk<FM::W, false, FM::Q>(r, pp, g - hj[j].begin, n, nullptr, nullptr, &jc);
         });
         if (rc) return rc;
@@ -3688,6 +3715,7 @@ KR_EXPORT int kr_batch_free(kr_batch *b) {
     batch_drop_graph(b);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
+    for (void *p : b->nrc_allocs) (void)hipFree(p);
     if (b->r) kr_render_free(b->r);
     for (kr_scene *s : b->up) kr_scene_free(s);
     for (void *p : b->allocs) (void)hipFree(p);
@@ -3809,6 +3837,55 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
     return KR_OK;
 }
 
+KR_EXPORT int kr_batch_attach_nrc(kr_batch *b, uint32_t job, kr_nrc *n) {
+    if (!b) return fail(KR_E_INVALID, "NULL batch");
+    if (job >= b->jobs.size()) return fail(KR_E_INVALID, "job index out of range");
+    if (n && n->dev != b->dev) return fail(KR_E_INVALID, "cache and batch live on different devices");
+    if (n && (b->tuning.flags & KR_BATCH_MIX_JOBS))
+        return fail(KR_E_UNSUPPORTED, "a batch whose chunks mix jobs takes no radiance cache");
+    (void)hipSetDevice(b->dev->ordinal);
+    (void)hipStreamSynchronize(b->dev->stream);
+    for (int pi = 0; pi < b->r->npipes; ++pi) (void)hipStreamSynchronize(b->r->pipe[pi].stream);
+    b->nrc.resize(b->jobs.size(), nullptr);
+    b->nrc[job] = n;
+    // buffers for the smallest training group any attached cache uses
+    for (void *p : b->nrc_allocs) (void)hipFree(p);
+    b->nrc_allocs.clear();
+    for (NrcRender &x : b->pnr) x = NrcRender{};
+    batch_drop_graph(b);  // the captured chunks carry the cache views
+    uint32_t log2 = 32;
+    for (kr_nrc *c : b->nrc)
+        if (c) log2 = std::min(log2, c->cfg.train_log2);
+    if (log2 == 32) return KR_OK;
+    const uint32_t P = b->r->P;
+    const uint32_t nv = b->max_bounce + 1;
+    const uint32_t cap = uint32_t(std::min<uint64_t>(P, (uint64_t(P) >> log2) + 2));
+    auto get = [&](auto **p, size_t cnt) {
+        int rc = dalloc(p, cnt);
+        if (!rc) b->nrc_allocs.push_back(*p);
+        return rc;
+    };
+    int rc = KR_OK;
+    for (int pi = 0; pi < b->r->npipes && !rc; ++pi) {
+        NrcRender &pn = b->pnr[pi];
+        pn.nv = nv;
+        if (!rc) rc = get(&pn.tslot_of, P);
+        if (!rc) rc = get(&pn.rec, size_t(cap) * nv * kRecFloats);
+        if (!rc) rc = get(&pn.tcount, cap);
+        if (!rc) rc = get(&pn.qslot, P);
+        if (!rc) rc = get(&pn.qbeta, 3 * size_t(P));
+        if (!rc) rc = get(&pn.qraw, size_t(P) * nrc::kRaw);
+    }
+    if (rc) {
+        for (void *p : b->nrc_allocs) (void)hipFree(p);
+        b->nrc_allocs.clear();
+        for (NrcRender &x : b->pnr) x = NrcRender{};
+        std::fill(b->nrc.begin(), b->nrc.end(), nullptr);
+        return rc;
+    }
+    return KR_OK;
+}
+
 KR_EXPORT int kr_batch_run(kr_batch *b) {
     if (!b) return fail(KR_E_INVALID, "NULL batch");
     (void)hipSetDevice(b->dev->ordinal);
@@ -3826,6 +3903,14 @@ KR_EXPORT int kr_batch_run(kr_batch *b) {
         KR_HIP(hipEventRecord(r->fork, r->pipe[pi].stream));
         KR_HIP(hipStreamWaitEvent(st, r->fork, 0));
     }
+    // every attached cache takes its Adam steps on what its jobs gathered
+    std::vector<kr_nrc *> trained;
+    for (kr_nrc *n : b->nrc)
+        if (n && std::find(trained.begin(), trained.end(), n) == trained.end()) {
+            trained.push_back(n);
+            int rc = nrc_after_run(r, n);
+            if (rc) return rc;
+        }
     KR_HIP(hipEventRecord(b->e1, st));
     b->timed = true;
     b->r->paths += b->total_paths;
@@ -3892,6 +3977,8 @@ KR_EXPORT int kr_batch_get_info(kr_batch *b, kr_batch_info *info) {
 
 KR_EXPORT int kr_batch_autotune(kr_batch *b, uint32_t trial_spp, kr_batch_tuning *chosen) {
     if (!b) return fail(KR_E_INVALID, "NULL batch");
+    if (std::any_of(b->nrc.begin(), b->nrc.end(), [](kr_nrc *n) { return n != nullptr; }))
+        return fail(KR_E_UNSUPPORTED, "tune a batch before attaching radiance caches (the trials would train them)");
     (void)hipSetDevice(b->dev->ordinal);
     hipStream_t st = b->dev->stream;
     // trial_spp 0: the same fraction of every job's samples, about 256 M

@@ -149,7 +149,7 @@ EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "
            "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_get_render_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
            "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc", "kr_nrc_read_pool",
            "kr_batch_create", "kr_batch_autotune", "kr_batch_run", "kr_batch_sync", "kr_batch_reset", "kr_batch_read",
-           "kr_batch_get_info", "kr_batch_free", "kr_trace_closest_noalloc"]
+           "kr_batch_get_info", "kr_batch_free", "kr_trace_closest_noalloc", "kr_batch_attach_nrc"]
 
 _lib = None
 
@@ -209,6 +209,7 @@ def load(path: str = LIB_PATH):
     lib.kr_batch_read.argtypes = [vp, u32, vp, vp]
     lib.kr_batch_get_info.argtypes = [vp, C.POINTER(BatchInfo)]
     lib.kr_batch_free.argtypes = [vp]
+    lib.kr_batch_attach_nrc.argtypes = [vp, u32, vp]
     for name in EXPORTS:
         if name not in ("kr_version",):
             getattr(lib, name).restype = C.c_int
@@ -498,6 +499,10 @@ class Batch:
         t = BatchTuning()
         check(self.lib.kr_batch_autotune(self.h, trial_spp, C.byref(t)), "kr_batch_autotune")
         return t
+
+    def attach_nrc(self, job: int, nrc):
+        """kr_batch_attach_nrc: the job's paths use `nrc` (None detaches)."""
+        check(self.lib.kr_batch_attach_nrc(self.h, job, nrc.h if nrc is not None else None), "kr_batch_attach_nrc")
 
     def run(self):
         check(self.lib.kr_batch_run(self.h), "kr_batch_run")

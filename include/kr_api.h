@@ -514,6 +514,17 @@ int kr_batch_reset(kr_batch *batch);
 int kr_batch_read(kr_batch *batch, uint32_t job, double *rgb, uint32_t *valid);
 int kr_batch_get_info(kr_batch *batch, kr_batch_info *info);
 int kr_batch_free(kr_batch *batch);
+/* Attaches a radiance cache to job `job` of the batch (NULL detaches), as
+ * kr_render_attach_nrc does for a render: in every kr_batch_run the job's
+ * paths are selected for training or end early with a cache query by the
+ * cache's configuration, and after the run every attached cache takes its
+ * steps_per_frame Adam steps on the samples its jobs gathered. A cache is
+ * specific to its scene; several jobs of one scene may share one. The next
+ * kr_batch_run re-captures the batch's graphs. KR_E_UNSUPPORTED for batches
+ * whose chunks mix jobs (KR_BATCH_MIX_JOBS); kr_batch_autotune refuses a
+ * batch with caches attached (its trials would train them). No counterpart
+ * in the reference. */
+int kr_batch_attach_nrc(kr_batch *batch, uint32_t job, kr_nrc *nrc);
 
 #ifdef __cplusplus
 }
