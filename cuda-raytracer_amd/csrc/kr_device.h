@@ -40,7 +40,7 @@ __device__ __forceinline__ bool finite3(f3 a) { return isfinite(a.x) && isfinite
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = float(1.0 / 3.14159265358979323846);
-constexpr float kSpawnEps = 4e-3f;  // src/core/meta/varyings.hpp:97,108
+constexpr float kSpawnEps = 4e-3f;  // src/core/meta/varyings.hpp:22,33
 
 // ------------------------------------------------------------------- RNG --
 // Philox4x32-10 (Salmon et al., SC'11). Same counter layout as the oracle:
@@ -141,7 +141,7 @@ __device__ __forceinline__ bool tri_hit(const ShearRay &r, f3 v0, f3 v1, f3 v2, 
     return true;
 }
 
-// Sphere::intersect (src/core/primitive/sphere.hpp:71-94) in FP64: near root,
+// Sphere::intersect (src/core/primitive/sphere.hpp:31-54) in FP64: near root,
 // else far root, hit iff t >= 0.
 __device__ __forceinline__ bool sphere_hit(const double *s, f3 o, f3 d, double &t) {
     double ocx = s[0] - double(o.x), ocy = s[1] - double(o.y), ocz = s[2] - double(o.z);
@@ -742,7 +742,7 @@ __device__ __forceinline__ f3 sample_cos(float r0, float r1) {
     return {r * cp, r * sp, sqrtf(smax(0.f, 1.f - r * r))};
 }
 
-// BeckmannDistribution::sample / f (src/ext/distribution/beckmann.hpp:194-213)
+// BeckmannDistribution::sample / f (src/ext/distribution/beckmann.hpp:19-38)
 __device__ __forceinline__ f3 beckmann_sample(float u0, float u1, float alpha2) {
     float tg2th = -alpha2 * logf(u0);
     float phi = u1 * 2.f * kPi;
@@ -792,7 +792,7 @@ __device__ __forceinline__ f3 dist_sample(const ShaderRec &sh, float u0, float u
 }
 
 // Shader::execute(compute_f_by_wi_wo) of a leaf shader: Lambert / Emission
-// (lambert.hpp:31-33, emission.hpp:117-122), GlossyBSDF (glossy.hpp:77-79),
+// (lambert.hpp:31-33, emission.hpp:22-27), GlossyBSDF (glossy.hpp:28-30),
 // GlassBSDF (glass.hpp:27-34, 48-51; uc is its Fresnel pick), and f = 0 for
 // RefractionBSDF, Transparent, Blackhole.
 __device__ __forceinline__ f3 shader_eval(const ShaderRec &sh, f3 wo, f3 wi, float uc = 0.f) {
@@ -823,7 +823,7 @@ __device__ __forceinline__ void refraction_sample(const ShaderRec &sh, f3 wo, fl
 }
 
 // Shader::execute(sample_wi_f_by_wo) of a leaf shader (lambert.hpp:26-30,
-// glossy.hpp:73-76, glass.hpp:22-46, refraction.hpp:24-34,
+// glossy.hpp:24-27, glass.hpp:22-46, refraction.hpp:24-34,
 // transparent.hpp:20-23, blackhole.hpp:14-21). wi is in/out: a failed
 // refraction keeps the caller's value, as the reference's Varyings does.
 __device__ __forceinline__ void shader_sample(const ShaderRec &sh, f3 wo, float u0, float u1, float uc, f3 &wi,

@@ -28,7 +28,8 @@ constexpr uint32_t kMissCode = 0xFFFFFFFFu;
 
 constexpr uint32_t node_floats(uint32_t w) { return (7 * w + 3) / 4 * 4; }
 
-// Compressed wide node (the default): child boxes quantised to 8 bits per
+// Compressed wide node (KR_SCENE_COMPRESSED_NODES, opt-in; by default nodes
+// hold full FP32 boxes, as above): child boxes quantised to 8 bits per
 // plane against the node's own box, as in Ylitie, Karras and Laine, "Efficient
 // Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs" (HPG 2017):
 //   word 0..2  origin p (float xyz; the node box's lower corner)

@@ -733,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_dyn(DevScene S, JobTable T, ui
 
 // Scene::intersect's shading normal and primitive of a hit
 // (src/core/meta/scene.hpp:48-53): sphere normal in FP64
-// (src/core/primitive/sphere.hpp:92), smooth mesh normal
+// (src/core/primitive/sphere.hpp:52), smooth mesh normal
 // (src/core/primitive/mesh.hpp:85-91).
 __device__ __forceinline__ void hit_surface(const DevScene &S, f3 o, f3 d, float t, uint32_t code, float bu,
                                             float bv, f3 &nrm, uint32_t &prim) {
@@ -757,7 +757,7 @@ __device__ __forceinline__ void hit_surface(const DevScene &S, f3 o, f3 d, float
 
 // Light::sample's point and pdf for shading point p: a point light
 // (pointLight.hpp:20-27; pdf unused), a mesh light's table entry
-// (mesh.hpp:194-200), a sphere's theta-uniform point (sphere.hpp:62-70);
+// (mesh.hpp:92-98), a sphere's theta-uniform point (sphere.hpp:22-30);
 // area pdfs become solid-angle pdfs (areaLight.hpp:26).
 __device__ __forceinline__ void light_point(const DevScene &S, const LightRec &lr, f3 p, float u0, float u1, f3 &lp,
                                             float &pdf) {
@@ -2314,7 +2314,7 @@ KR_EXPORT int kr_render_free(kr_render *r) {
 }
 
 namespace {
-// Lens ctor (src/core/misc/lens.hpp:113-125), FP32 as in the reference.
+// Lens ctor (src/core/misc/lens.hpp:15-27), FP32 as in the reference.
 LensDev make_lens(const kr_camera *cam, const kr_params *prm) {
     LensDev L{};
     L.lens = cam->lens;

@@ -438,7 +438,7 @@ std::vector<ObjMesh> load_mesh_file(const std::string &path, std::vector<std::st
 namespace {
 // "distribution": Config of a SphericalDistribution (distribution.impl.cu:8-11)
 void parse_distribution(const Json &props, kr_shader_desc &d) {
-    const Json &dist = props.at("distribution");  // required (glossy.hpp:62-63)
+    const Json &dist = props.at("distribution");  // required (glossy.hpp:13-14)
     std::string dn;
     const Json *dp = nullptr;
     if (dist.is_string()) {

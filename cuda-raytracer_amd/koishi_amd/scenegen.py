@@ -211,8 +211,8 @@ def _distribution(props, d):
 
 def fill_shader(cfg, table: list, slot: int, depth: int = 0):
     """Reference shader config [type, props] -> kr_shader_desc at table[slot]
-    with the reference's defaults (lambert.hpp:12-19, glossy.hpp:60-66,
-    emission.hpp:107-114, beckmann.hpp:186-192, glass.hpp:12-19,
+    with the reference's defaults (lambert.hpp:12-19, glossy.hpp:11-16,
+    emission.hpp:12-19, beckmann.hpp:11-17, glass.hpp:12-19,
     refraction.hpp:12-19, mixed.hpp:12-18); a Mixed's two inline shaders are
     appended to the table."""
     if isinstance(cfg, str):

@@ -159,7 +159,7 @@ typedef struct kr_scene_info {
 enum kr_lens_type { KR_LENS_PINHOLE = 0, KR_LENS_ORTHOGRAPHIC = 1 };
 
 /* Mirrors CameraConfig (src/util/config.hpp:12-35). "target" is a
- * direction, not a point (src/core/misc/lens.hpp:119). */
+ * direction, not a point (src/core/misc/lens.hpp:21). */
 typedef struct kr_camera {
     uint32_t lens;
     float fovx;           /* degrees */
@@ -171,7 +171,7 @@ typedef struct kr_camera {
 typedef struct kr_params {
     uint32_t width, height;
     uint32_t spp;        /* total samples per pixel; fixes the lens strata
-                            pw = ceil(sqrt(spp)) (src/core/misc/lens.hpp:117) */
+                            pw = ceil(sqrt(spp)) (src/core/misc/lens.hpp:19) */
     uint32_t max_bounce; /* RadianceKernel "maxBounce" (src/core/kernel/radiance.hpp:14) */
     uint64_t seed;       /* counter-based RNG key                               */
     uint32_t flags;      /* KR_RENDER_* bits                                    */
@@ -283,7 +283,7 @@ int kr_scene_light_samples(const kr_scene *scene, uint32_t light, float *p_n, fl
 /* ---- ray queries --------------------------------------------------------
  * kr_trace_closest replaces Scene::intersect(Ray) (src/core/meta/scene.hpp:35-55)
  * over Mesh::intersect(Ray) (src/core/primitive/mesh.cu:200-267) and
- * Sphere::intersect (src/core/primitive/sphere.hpp:71-94).
+ * Sphere::intersect (src/core/primitive/sphere.hpp:31-54).
  * kr_trace_shadow replaces Scene::intersect(Seg) (src/core/meta/scene.hpp:57-67);
  * occluded[i] = 1 when segment i (origin, direction, tmax) is blocked. */
 int kr_trace_closest(kr_scene *scene, const kr_rays *rays, uint32_t n, kr_hits *hits,

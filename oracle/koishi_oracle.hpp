@@ -22,7 +22,7 @@
 // built with the reference's release flags).
 //
 // Deliberate replacement: the reference's Sampler is a time-seeded LCG shared
-// by all threads (src/core/misc/sampler.hpp:191-205), so no run is
+// by all threads (src/core/misc/sampler.hpp:22-36), so no run is
 // reproducible. Here every random number is Philox4x32-10 (Salmon, Moraes,
 // Dror, Shaw, SC'11) of a counter (pixel, sample, bounce, stream) under the
 // key `seed`, and the GPU path draws the very same numbers. The order of
@@ -86,7 +86,7 @@ template <class R> inline V3<R> mk(const float *p) { return {R(p[0]), R(p[1]), R
 template <class R> struct Const {
     static constexpr R pi() { return R(3.14159265358979323846); }
     static constexpr R inv_pi() { return R(1) / R(3.14159265358979323846); }
-    static constexpr R spawn_eps() { return R(4e-3f); }  // src/core/meta/varyings.hpp:97,108
+    static constexpr R spawn_eps() { return R(4e-3f); }  // src/core/meta/varyings.hpp:22,33
 };
 
 // ------------------------------------------------------------------- RNG --
@@ -240,7 +240,7 @@ template <class R> inline V3<R> sample_cos(R rn0, R rn1) {
 // reflect(I, N) = N * dot(N, I) * 2 - I (src/vec/vmath.hpp:470-473)
 template <class R> inline V3<R> reflect(V3<R> I, V3<R> N) { return N * (dot(N, I) * R(2)) - I; }
 
-// BeckmannDistribution::f (src/ext/distribution/beckmann.hpp:194-199),
+// BeckmannDistribution::f (src/ext/distribution/beckmann.hpp:19-24),
 // returned as the scalar multiplying (1,1,1).
 template <class R> inline R beckmann_f(V3<R> w, R alpha2) {
     R cos2 = w.z * w.z;
@@ -249,7 +249,7 @@ template <class R> inline R beckmann_f(V3<R> w, R alpha2) {
     R cos4th = cos2 * cos2;
     return std::exp(-tg2th / alpha2) / (Const<R>::pi() * alpha2 * cos4th) * std::fabs(w.z);
 }
-// BeckmannDistribution::sample (src/ext/distribution/beckmann.hpp:201-213)
+// BeckmannDistribution::sample (src/ext/distribution/beckmann.hpp:26-38)
 template <class R> inline V3<R> beckmann_sample(R u0, R u1, R alpha2) {
     R logu = std::log(u0);
     R tg2th = -alpha2 * logu;
@@ -258,7 +258,7 @@ template <class R> inline V3<R> beckmann_sample(R u0, R u1, R alpha2) {
     R sinth = std::sqrt(smax(R(0), R(1) - costh * costh));
     return from_euler(sinth, costh, phi);
 }
-// CosDistribution::f (src/ext/distribution/cosine.hpp:157-160)
+// CosDistribution::f (src/ext/distribution/cosine.hpp:16-19)
 template <class R> inline R cos_f(V3<R> w) { return w.z >= 0 ? Const<R>::inv_pi() : R(0); }
 
 // ------------------------------------------------------------------ lens --
@@ -481,7 +481,7 @@ template <class R> struct Scene {
         }
         // Light promotion (src/core/meta/scene.cu:200-216): every primitive with an
         // Emission shader becomes an AreaLight with E = color * strength
-        // (src/ext/shader/emission.hpp:107-114), in primitive order.
+        // (src/ext/shader/emission.hpp:12-19), in primitive order.
         for (uint32_t i = 0; i < prims.size() && !(d.flags & KR_SCENE_NO_AREA_LIGHTS); ++i) {
             const kr_shader_desc &sh = shaders[prims[i].shader];
             if (sh.type != KR_SHADER_EMISSION) continue;
@@ -590,7 +590,7 @@ template <class R> struct Scene {
         return false;
     }
 
-    // Sphere::intersect (src/core/primitive/sphere.hpp:71-94).
+    // Sphere::intersect (src/core/primitive/sphere.hpp:31-54).
     static bool sphere_intersect(const Sphere<R> &s, const Ray<R> &ray, Hit<R> &hit) {
         V3<R> oc = s.o - ray.o;
         R poc = dot(ray.d, oc);
@@ -673,7 +673,7 @@ template <class R> inline bool scene_intersect(const Scene<R> &s, const Ray<R> &
     if (!c.hit) return false;
     const Prim &p = s.prims[c.prim];
     if (p.type == KR_PRIM_MESH) {
-        // Mesh::normal + interplot (src/core/primitive/mesh.hpp:187-193, ray.hpp:93-98)
+        // Mesh::normal + interplot (src/core/primitive/mesh.hpp:85-91, ray.hpp:93-98)
         const Nrm<R> &nn = s.meshes[p.index].normals[c.h.id];
         vy.n = normalize(nn.n0 * (R(1) - c.h.u - c.h.v) + nn.n1 * c.h.u + nn.n2 * c.h.v);
     } else {
@@ -756,8 +756,8 @@ template <class R> inline void refraction_sample(const kr_shader_desc &sh, Vary<
 }
 
 // Shader::execute(compute_f_by_wi_wo) of a leaf: Lambert (lambert.hpp:31-33),
-// Glossy (glossy.hpp:77-79), Emission = emission + white Lambert
-// (emission.hpp:117-122), GlassBSDF (glass.hpp:27-34, 48-51: reflection when
+// Glossy (glossy.hpp:28-30), Emission = emission + white Lambert
+// (emission.hpp:22-27), GlassBSDF (glass.hpp:27-34, 48-51: reflection when
 // u < Fresnel(ior)), RefractionBSDF / Transparent / Blackhole: f = 0
 // (refraction.hpp:35-38, transparent.hpp:24-26, blackhole.hpp:18-21).
 // uc: the Glass branch uniform (its draw follows the Mixed draws).
@@ -782,7 +782,7 @@ template <class R> inline void shader_eval(const kr_shader_desc &sh, Vary<R> &vy
 // Shader::execute(sample_wi_f_by_wo) of a leaf with u = sample3(). Lambert:
 // wi from CosDistribution with pdf 1/pi, f = color R f(wi) / pdf
 // (lambert.hpp:26-30). Glossy: wi = reflect(wo, h), f = color when wo, wi lie
-// on one side (glossy.hpp:73-76, H::isSame hemisphere.hpp:14-17). GlassBSDF:
+// on one side (glossy.hpp:24-27, H::isSame hemisphere.hpp:14-17). GlassBSDF:
 // that reflection when u < Fresnel(ior), else the refraction branch
 // (glass.hpp:22-46). Transparent: wi = -wo, f = color (transparent.hpp:20-23).
 // Blackhole: wi = (0, 0, 1), f = 0 (blackhole.hpp:14-21).
@@ -834,13 +834,13 @@ inline V3<R> light_sample(const Scene<R> &s, const Light<R> &L, const Vary<R> &r
     LV<R> lv;
     R pdf;
     if (p.type == KR_PRIM_MESH) {
-        // Mesh::sample(u, pdf): table entry floor(u.x * 1024) (mesh.hpp:194-200)
+        // Mesh::sample(u, pdf): table entry floor(u.x * 1024) (mesh.hpp:92-98)
         const Mesh<R> &m = s.meshes[p.index];
         uint32_t idx = smin(uint32_t(std::floor(u0 * R(1024))), uint32_t(1023));
         pdf = m.pdf;
         lv = m.samples[idx];
     } else {
-        // Sphere::sample (sphere.hpp:62-70): theta-uniform, pdf = 1/(4 pi r^2)
+        // Sphere::sample (sphere.hpp:22-30): theta-uniform, pdf = 1/(4 pi r^2)
         const Sphere<R> &sp = s.spheres[p.index];
         V3<R> w = from_euler(u0 * Const<R>::pi(), u1 * Const<R>::pi() * R(2));
         lv.p = sp.o + std::sqrt(sp.r2) * w;
