@@ -157,32 +157,48 @@ __device__ __forceinline__ bool sphere_hit(const double *s, f3 o, f3 d, double &
 }
 
 // ------------------------------------------------------------- BVH walk --
-// Stack: the top LDS_DEPTH entries of each lane live in LDS ([depth][lane],
-// conflict free), deeper ones spill to a per-lane global area. The spill
-// capacity comes from the tree depth, so the stack cannot overflow.
-template <int LDS_DEPTH>
+// Stack: the top LDS_DEPTH entries of each lane live in LDS, deeper ones
+// spill to a per-lane global area. The spill capacity comes from the tree
+// depth, so the stack cannot overflow. LDS holds two [LDS_DEPTH][LANES] word
+// arrays, refs then entry distances: lane l of every level sits in bank
+// l mod 32, so the lanes of a wave never conflict whatever their depths.
+// A lane's whole state is one byte offset, off = (sp * LANES + lane) * 4:
+// a push or pop is one ds_write2 / ds_read2 at off (the distance at the
+// constant offset LDS_DEPTH * LANES * 4) and an add of LANES * 4, with no
+// lane index to keep in a register (off mod LANES * 4 is 4 * lane).
+template <int LDS_DEPTH, int LANES = 256>
 struct Stack {
-    uint32_t *lds_ref;  // [LDS_DEPTH][blockDim]
-    float *lds_t;
-    uint2 *spill;       // this lane's (ref, t) entries beyond LDS_DEPTH, stride gstride
-    uint32_t lane, lanes, gstride;
-    int sp = 0;
+    static constexpr uint32_t kRow = LANES * 4u;  // bytes of one level
+    static constexpr uint32_t kLds = LDS_DEPTH * kRow;
+    static_assert((kRow & (kRow - 1)) == 0, "a stack level spans a power-of-two number of bytes");
+    char *lds;         // [2][LDS_DEPTH][LANES] words of the block's __shared__ stack
+    uint2 *spill;      // the block's spill area: entry k of lane l at spill[k * gstride + l]
+    uint32_t gstride;  // lanes of the whole launch
+    uint32_t off;      // (sp * LANES + lane) * 4
+    __device__ __forceinline__ void reset() { off &= kRow - 1u; }
+    __device__ __forceinline__ bool empty() const { return off < kRow; }
+    // the spill entry at the current depth (beyond LDS_DEPTH); lds, spill and
+    // gstride are the same for the whole block, so the lane's only register
+    // is off
+    __device__ __forceinline__ uint2 *spilled() const {
+        return spill + size_t(off / kRow - LDS_DEPTH) * gstride + (off & (kRow - 1u)) / 4u;
+    }
     __device__ __forceinline__ void push(uint32_t ref, float t) {
-        if (sp < LDS_DEPTH) {
-            lds_ref[sp * lanes + lane] = ref;
-            lds_t[sp * lanes + lane] = t;
+        if (off < kLds) {
+            *reinterpret_cast<uint32_t *>(lds + off) = ref;
+            *reinterpret_cast<float *>(lds + kLds + off) = t;
         } else {
-            spill[size_t(sp - LDS_DEPTH) * gstride] = make_uint2(ref, __float_as_uint(t));
+            *spilled() = make_uint2(ref, __float_as_uint(t));
         }
-        ++sp;
+        off += kRow;
     }
     __device__ __forceinline__ void pop(uint32_t &ref, float &t) {
-        --sp;
-        if (sp < LDS_DEPTH) {
-            ref = lds_ref[sp * lanes + lane];
-            t = lds_t[sp * lanes + lane];
+        off -= kRow;
+        if (off < kLds) {
+            ref = *reinterpret_cast<const uint32_t *>(lds + off);
+            t = *reinterpret_cast<const float *>(lds + kLds + off);
         } else {
-            uint2 e = spill[size_t(sp - LDS_DEPTH) * gstride];
+            uint2 e = *spilled();
             ref = e.x;
             t = __uint_as_float(e.y);
         }
@@ -382,16 +398,17 @@ __device__ __forceinline__ void sort_children(float (&key)[W], uint32_t (&ref)[W
 // nearest child first, far children pushed with their entry distance and
 // culled on pop. Equivalent to the reference's primitive loop + per-mesh DFS
 // (src/core/meta/scene.hpp:35-47, src/core/primitive/mesh.cu:200-267) up to
-// exact ties.
-template <int W, int LDS_DEPTH, bool Q = false>
-__device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, Stack<LDS_DEPTH> &st,
-                                                uint64_t *counts = nullptr) {
+// exact ties. The walk is cut into the three pieces below so that the
+// refilling traversal kernel runs exactly the same steps as trace_closest.
+
+// The ray's hit against the sphere list (linear, FP64), the starting record
+// of the walk.
+__device__ __forceinline__ HitRec closest_spheres(const DevScene &S, f3 o, f3 d) {
     HitRec h;
     h.t = __builtin_inff();
     h.u = h.v = 0.f;
     h.code = kMissCode;
     h.mat = 0;
-    // spheres (linear, FP64)
     for (uint32_t s = 0; s < S.nspheres; ++s) {
         double ts;
         if (sphere_hit(S.spheres + 4 * s, o, d, ts)) {
@@ -402,75 +419,103 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
             }
         }
     }
+    return h;
+}
+
+// One inner node: returns the nearest child whose box the ray enters before
+// tmax (kEmpty if none) and pushes the other entered children, farthest
+// first, with their entry distances.
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ uint32_t closest_node(const DevScene &S, const RayBox &rb, uint32_t node, float tmax,
+                                                 Stack<LDS_DEPTH> &st) {
+    NodeView<W> nv;
+    if (Q)
+        fetch_node<W, Q>(S.nodes, node, nv);
+    else
+        load_node_nf<W>(S.nodes, node, rb, nv);
+    float key[W];
+    uint32_t ref[W];
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        if (Q) {
+            float tn = box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
+                                 tmax);
+            bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
+            key[c] = ok ? tn : __builtin_inff();
+        } else {
+            // an empty slot holds the box (+inf, -inf), whose entering
+            // plane lies at +inf along every axis: box_enter_nf returns +inf
+            // for it like for any missed child
+            key[c] = box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
+                                  tmax);
+        }
+        ref[c] = nv.ref[c];
+    }
+    sort_children<W>(key, ref);
+    // missed children sort last, with key +inf
+#pragma unroll
+    for (int c = W - 1; c >= 1; --c)
+        if (key[c] != __builtin_inff()) st.push(ref[c], key[c]);
+    return key[0] != __builtin_inff() ? ref[0] : kEmpty;
+}
+
+// One leaf: the triangles of `leaf` against the closest hit so far.
+__device__ __forceinline__ uint32_t closest_leaf(const DevScene &S, const ShearRay &sr, uint32_t leaf, HitRec &h) {
+    uint32_t first = leaf & kLeafFirstMask;
+    uint32_t cnt = ((leaf >> kLeafCountShift) & 0xFu) + 1u;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        const float4 *tp = S.tris + 3 * size_t(first + k);
+        float4 a = tp[0], b = tp[1], c = tp[2];
+        float tt, b1, b2;
+        if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, h.t, tt, b1, b2)) {
+            h.t = tt;
+            h.u = b1;
+            h.v = b2;
+            h.code = __float_as_uint(b.w);
+            h.mat = __float_as_uint(c.w);
+        }
+    }
+    return cnt;
+}
+
+// Pops the next subtree still in front of the closest hit t into `cur`;
+// false when none is left (the walk is done).
+template <int LDS_DEPTH>
+__device__ __forceinline__ bool closest_pop(Stack<LDS_DEPTH> &st, float t, uint32_t &cur) {
+    while (!st.empty()) {
+        float tn;
+        st.pop(cur, tn);
+        if (tn <= t) return true;
+    }
+    return false;
+}
+
+template <int W, int LDS_DEPTH, bool Q = false>
+__device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, Stack<LDS_DEPTH> &st,
+                                                uint64_t *counts = nullptr) {
+    HitRec h = closest_spheres(S, o, d);
     if (S.ntris == 0) return h;
     RayBox rb = make_raybox(o, d);
     ShearRay sr = make_shear(o, d);
     uint32_t cur = 0;  // root is always an inner node
     uint64_t nvis = 0, ntri = 0;
-    st.sp = 0;
+    st.reset();
+    // One node or one leaf per iteration ("if-if"): a lane that reaches a
+    // leaf tests it in the same iteration in which the wave's other lanes
+    // take their next node, instead of waiting at the leaf until every
+    // lane's descent is over, which is what a `continue` into the node
+    // branch compiles to.
     while (true) {
+        bool pop = true;
         if (!(cur & kLeafBit)) {
-            NodeView<W> nv;
-            if (Q)
-                fetch_node<W, Q>(S.nodes, cur, nv);
-            else
-                load_node_nf<W>(S.nodes, cur, rb, nv);
             ++nvis;
-            float key[W];
-            uint32_t ref[W];
-#pragma unroll
-            for (int c = 0; c < W; ++c) {
-                if (Q) {
-                    float tn = box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
-                                         nv.hi[2][c], h.t);
-                    bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
-                    key[c] = ok ? tn : __builtin_inff();
-                } else {
-                    // an empty slot holds the box (+inf, -inf), whose entering
-                    // plane lies at +inf along every axis: box_enter_nf
-                    // returns +inf for it like for any missed child
-                    key[c] = box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
-                                          nv.hi[2][c], h.t);
-                }
-                ref[c] = nv.ref[c];
-            }
-            sort_children<W>(key, ref);
-            // missed children sort last, with key +inf
-#pragma unroll
-            for (int c = W - 1; c >= 1; --c)
-                if (key[c] != __builtin_inff()) st.push(ref[c], key[c]);
-            if (key[0] != __builtin_inff()) {
-                cur = ref[0];
-                continue;
-            }
+            uint32_t next = closest_node<W, LDS_DEPTH, Q>(S, rb, cur, h.t, st);
+            pop = next == kEmpty;
+            if (!pop) cur = next;
         } else {
-            uint32_t first = cur & kLeafFirstMask;
-            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 *tp = S.tris + 3 * size_t(first + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                float tt, b1, b2;
-                ++ntri;
-                if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, h.t, tt, b1, b2)) {
-                    h.t = tt;
-                    h.u = b1;
-                    h.v = b2;
-                    h.code = __float_as_uint(b.w);
-                    h.mat = __float_as_uint(c.w);
-                }
-            }
+            ntri += closest_leaf(S, sr, cur, h);
         }
-        // pop the next subtree still in front of the closest hit
-        bool found = false;
-        while (st.sp > 0) {
-            float tn;
-            st.pop(cur, tn);
-            if (tn <= h.t) {
-                found = true;
-                break;
-            }
-        }
-        if (!found) break;
+        if (pop && !closest_pop(st, h.t, cur)) break;
     }
     if (counts) {
         counts[0] = nvis;
@@ -500,9 +545,9 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
     ShearRay sr = make_shear(o, d);
     // tri_hit accepts t < limit; the segment includes its end point.
     float limit = tmax * 1.00000012f;
-    st.sp = 0;
+    st.reset();
     auto pop_next = [&]() -> uint32_t {
-        if (st.sp == 0) return kEmpty;
+        if (st.empty()) return kEmpty;
         uint32_t r;
         float tn;
         st.pop(r, tn);
@@ -563,97 +608,80 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
     return false;
 }
 
+// The same any-hit walk one node or one leaf per iteration ("if-if"),
+// without postponed leaves.
+template <int W, int LDS_DEPTH, bool Q = false>
+__device__ __forceinline__ bool trace_occluded_steps(const DevScene &S, f3 o, f3 d, float tmax, Stack<LDS_DEPTH> &st) {
+    for (uint32_t s = 0; s < S.nspheres; ++s) {
+        double ts;
+        if (sphere_hit(S.spheres + 4 * s, o, d, ts) && ts <= double(tmax)) return true;
+    }
+    if (S.ntris == 0) return false;
+    RayBox rb = make_raybox(o, d);
+    ShearRay sr = make_shear(o, d);
+    float limit = tmax * 1.00000012f;
+    st.reset();
+    uint32_t cur = 0;
+    while (true) {
+        bool pop = true;
+        if (!(cur & kLeafBit)) {
+            NodeView<W> nv;
+            if (Q)
+                fetch_node<W, Q>(S.nodes, cur, nv);
+            else
+                load_node_nf<W>(S.nodes, cur, rb, nv);
+            uint32_t next = kEmpty;
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                float tn = Q ? box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                         nv.hi[2][c], limit)
+                             : box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
+                                            nv.hi[2][c], limit);
+                if ((Q ? nv.ref[c] != kEmpty : true) && tn != __builtin_inff()) {
+                    if (next == kEmpty)
+                        next = nv.ref[c];
+                    else
+                        st.push(nv.ref[c], tn);
+                }
+            }
+            pop = next == kEmpty;
+            if (!pop) cur = next;
+        } else {
+            uint32_t first = cur & kLeafFirstMask;
+            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 *tp = S.tris + 3 * size_t(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                float tt, b1, b2;
+                if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, limit, tt, b1, b2) &&
+                    tt <= tmax)
+                    return true;
+            }
+        }
+        if (pop) {
+            if (st.empty()) break;
+            float tn;
+            st.pop(cur, tn);
+        }
+    }
+    return false;
+}
+
 // ------------------------------------------------ stepwise traversal --
-// The same two traversals cut into steps (one inner node, or one leaf and
-// the pops after it), so that a persistent kernel can refill lanes whose ray
-// is done with new rays while the rest of the wave keeps traversing
+// The any-hit traversal cut into steps (one inner node, or one leaf and the
+// pop after it), so that a persistent kernel can refill lanes whose segment
+// is done with new segments while the rest of the wave keeps traversing
 // (Aila and Laine, "Understanding the Efficiency of Ray Traversal on GPUs",
-// HPG 2009). Results are identical to trace_closest / trace_occluded.
+// HPG 2009). Results are identical to trace_occluded.
 struct TravState {
     RayBox rb;
     ShearRay sr;
-    HitRec h;
-    float limit;  // shadow: segment end (with the tri_hit margin); tmax
+    float limit;  // segment end (with the tri_hit margin); tmax
     float tmax;
     uint32_t cur;
     bool done;
     bool occluded;
 };
-
-template <int W, int LDS_DEPTH>
-__device__ __forceinline__ void closest_begin(const DevScene &S, f3 o, f3 d, Stack<LDS_DEPTH> &st, TravState &ts) {
-    ts.h.t = __builtin_inff();
-    ts.h.u = ts.h.v = 0.f;
-    ts.h.code = kMissCode;
-    for (uint32_t s = 0; s < S.nspheres; ++s) {
-        double tsph;
-        if (sphere_hit(S.spheres + 4 * s, o, d, tsph)) {
-            float tf = float(tsph);
-            if (tf < ts.h.t) {
-                ts.h.t = tf;
-                ts.h.code = kSphereBit | s;
-            }
-        }
-    }
-    ts.done = S.ntris == 0;
-    ts.rb = make_raybox(o, d);
-    ts.sr = make_shear(o, d);
-    ts.cur = 0;
-    st.sp = 0;
-}
-
-template <int W, int LDS_DEPTH, bool Q>
-__device__ __forceinline__ void closest_step(const DevScene &S, Stack<LDS_DEPTH> &st, TravState &ts, uint32_t &nvis,
-                                             uint32_t &ntri) {
-    uint32_t cur = ts.cur;
-    if (!(cur & kLeafBit)) {
-        NodeView<W> nv;
-        fetch_node<W, Q>(S.nodes, cur, nv);
-        ++nvis;
-        float key[W];
-        uint32_t ref[W];
-#pragma unroll
-        for (int c = 0; c < W; ++c) {
-            float tn = box_enter(ts.rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c], nv.hi[2][c],
-                                 ts.h.t);
-            bool ok = nv.ref[c] != kEmpty && tn != __builtin_inff();
-            key[c] = ok ? tn : __builtin_inff();
-            ref[c] = ok ? nv.ref[c] : kEmpty;
-        }
-        sort_children<W>(key, ref);
-#pragma unroll
-        for (int c = W - 1; c >= 1; --c)
-            if (ref[c] != kEmpty) st.push(ref[c], key[c]);
-        if (ref[0] != kEmpty) {
-            ts.cur = ref[0];
-            return;
-        }
-    } else {
-        uint32_t first = cur & kLeafFirstMask;
-        uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const float4 *tp = S.tris + 3 * size_t(first + k);
-            float4 a = tp[0], b = tp[1], c = tp[2];
-            float tt, b1, b2;
-            ++ntri;
-            if (tri_hit(ts.sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, ts.h.t, tt, b1, b2)) {
-                ts.h.t = tt;
-                ts.h.u = b1;
-                ts.h.v = b2;
-                ts.h.code = __float_as_uint(b.w);
-            }
-        }
-    }
-    while (st.sp > 0) {
-        float tn;
-        st.pop(cur, tn);
-        if (tn <= ts.h.t) {
-            ts.cur = cur;
-            return;
-        }
-    }
-    ts.done = true;
-}
 
 template <int W, int LDS_DEPTH>
 __device__ __forceinline__ void occluded_begin(const DevScene &S, f3 o, f3 d, float tmax, Stack<LDS_DEPTH> &st,
@@ -669,7 +697,7 @@ __device__ __forceinline__ void occluded_begin(const DevScene &S, f3 o, f3 d, fl
     ts.tmax = tmax;
     ts.limit = tmax * 1.00000012f;
     ts.cur = 0;
-    st.sp = 0;
+    st.reset();
 }
 
 template <int W, int LDS_DEPTH, bool Q>
@@ -709,7 +737,7 @@ __device__ __forceinline__ void occluded_step(const DevScene &S, Stack<LDS_DEPTH
             }
         }
     }
-    if (st.sp == 0) {
+    if (st.empty()) {
         ts.done = true;
         return;
     }

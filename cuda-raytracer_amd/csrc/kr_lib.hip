@@ -67,6 +67,9 @@ constexpr int kBlock = 256;
 #define KR_SHADE_OCTANT_SORT 1
 #endif
 
+#ifndef KR_SHADOW_STEPS
+#define KR_SHADOW_STEPS 1
+#endif
 #ifndef KR_SHADE_WAVES
 #define KR_SHADE_WAVES 1
 #endif
@@ -419,16 +422,15 @@ __device__ __forceinline__ void prof_add_light(const ProfView &pv, uint64_t g, u
     pr[11] += cz;
 }
 
+// The traversal stack of a lane of a kBlock-thread block over the block's
+// __shared__ uint32_t[2 * LDS * kBlock] (refs, then entry distances).
 template <int LDS>
-__device__ __forceinline__ Stack<LDS> make_stack(uint32_t *lds_ref, float *lds_t, uint32_t *spill, size_t spill_lanes) {
-    Stack<LDS> st;
-    st.lds_ref = lds_ref;
-    st.lds_t = lds_t;
-    st.lane = threadIdx.x;
-    st.lanes = blockDim.x;
-    size_t g = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ Stack<LDS, kBlock> make_stack(uint32_t *lds, uint32_t *spill, size_t spill_lanes) {
+    Stack<LDS, kBlock> st;
+    st.lds = reinterpret_cast<char *>(lds);
+    st.off = threadIdx.x * 4u;
     st.gstride = uint32_t(spill_lanes);
-    st.spill = reinterpret_cast<uint2 *>(spill) + g;
+    st.spill = reinterpret_cast<uint2 *>(spill) + size_t(blockIdx.x) * kBlock;
     return st;
 }
 
@@ -437,9 +439,8 @@ template <int W, bool Q>
 __global__ __launch_bounds__(kBlock) void k_query_closest(DevScene S, const float *o, const float *d, uint32_t n,
                                                           float *t, uint32_t *prim, uint32_t *tri, float *u, float *v,
                                                           uint32_t *spill, size_t spill_lanes, uint64_t *visits) {
-    __shared__ uint32_t sref[kLdsDepth * kBlock];
-    __shared__ float st_[kLdsDepth * kBlock];
-    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         f3 ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
         f3 rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
@@ -472,9 +473,8 @@ template <int W, bool Q>
 __global__ __launch_bounds__(kBlock) void k_query_shadow(DevScene S, const float *o, const float *d, const float *tmax,
                                                          uint32_t n, uint8_t *occ, uint32_t *spill,
                                                          size_t spill_lanes) {
-    __shared__ uint32_t sref[kLdsDepth * kBlock];
-    __shared__ float st_[kLdsDepth * kBlock];
-    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         f3 ro{o[3 * i], o[3 * i + 1], o[3 * i + 2]};
         f3 rd{d[3 * i], d[3 * i + 1], d[3 * i + 2]};
@@ -616,9 +616,8 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace(DevS
                                                   uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
                                                   unsigned long long *stats, int sort, uint32_t *spill,
                                                   size_t spill_lanes) {
-    __shared__ uint32_t sref[kLdsDepth * kBlock];
-    __shared__ float st_[kLdsDepth * kBlock];
-    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
     uint32_t n = counters[CTR(qi)];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_CLOSEST], (unsigned long long)n);
     uint32_t stride = gridDim.x * blockDim.x;
@@ -662,68 +661,100 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace(DevS
     }
 }
 
-// Persistent closest-hit traversal: every lane keeps a ray until it is done
-// and takes a new one from the queue (one atomic per wave) once more than
-// 64 - kMinActive lanes of its wave are idle, so waves do not idle behind
-// their slowest ray. Same outputs and bucket appends as k_trace.
-constexpr int kMinActive = 32;
+// Closest-hit traversal with lane refill (Aila and Laine, "Understanding the
+// Efficiency of Ray Traversal on GPUs", HPG 2009): a wave walks its rays one
+// step (a node or a leaf) per iteration and, once fewer than kRefillBelow of
+// its lanes still walk, hands the finished rays to their material buckets and
+// gives every idle lane the next ray of the queue (one atomic per wave for
+// each). With
+// incoherent rays a wave otherwise runs until its slowest ray is done while
+// most lanes idle. The walk is trace_closest's, so hits are identical; only
+// the order of the bucket entries differs.
+#ifndef KR_REFILL_BELOW
+#define KR_REFILL_BELOW 48
+#endif
+constexpr int kRefillBelow = KR_REFILL_BELOW;
+static_assert(kRefillBelow >= 1 && kRefillBelow <= 64, "refill threshold in lanes of a wave");
 
-template <int W, bool COUNT, bool BATCH, bool Q>
-__global__ __launch_bounds__(kBlock) void k_trace_dyn(DevScene S, JobTable T, uint64_t g0, RayQ q, int qi, HitQ hit,
-                                                      uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
-                                                      unsigned long long *stats, int sort, uint32_t *spill,
-                                                      size_t spill_lanes) {
-    __shared__ uint32_t sref[kLdsDepth * kBlock];
-    __shared__ float st_[kLdsDepth * kBlock];
-    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+template <int W, bool COUNT, bool Q>
+__global__ __launch_bounds__(kBlock, trace_waves(W, false, Q)) void k_trace_refill(
+    DevScene S, RayQ q, int qi, HitQ hit, uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
+    unsigned long long *stats, int sort, uint32_t *spill, size_t spill_lanes) {
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
     const uint32_t n = counters[CTR(qi)];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_CLOSEST], (unsigned long long)n);
-    uint32_t i = 0, nvis = 0, ntri = 0;
-    bool has = false, exhausted = false;
-    TravState ts;
-    DevScene Sx = S;
+    uint32_t i = kMissCode;  // the lane's ray (queue position), kMissCode when idle
+    bool walking = false;    // false with i set: done, waiting for its bucket
+    bool exhausted = false;  // the queue has no more rays (wave-uniform)
+    HitRec h;
+    h.code = kMissCode;
+    h.mat = 0;
+    RayBox rb;
+    ShearRay sr;
+    uint32_t cur = 0;
+    uint64_t nvis = 0, ntri = 0;
     while (true) {
-        bool want = !has && !exhausted;
-        uint32_t k = wave_append(&counters[CTR(C_FETCH)], want);
-        if (want) {
-            if (k < n) {
-                i = k;
-                has = true;
-                if (BATCH) Sx = T.rec[find_job(T, g0 + q.slot[i])].S;
-                f3 o{q.ox[i], q.oy[i], q.oz[i]}, d{q.dx[i], q.dy[i], q.dz[i]};
-                closest_begin<W, kLdsDepth>(Sx, o, d, st, ts);
+        if (__popcll(__ballot(walking)) < kRefillBelow) {
+            bool fin = !walking && i != kMissCode;
+            bool hitp = fin && h.code != kMissCode;
+            uint32_t mat = 0;
+            if (hitp && sort)
+                mat = (h.code & kSphereBit) ? min(S.shaders[S.prims[S.sphere_prim[h.code & ~kSphereBit]].shader].type, 2u)
+                                            : h.mat;
+            uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
+            if (hitp && mat == 0) b0[p0] = i;
+            if (sort) {
+                uint32_t p1 = wave_append(&counters[CTR(C_BUCKET + 1)], hitp && mat == 1);
+                if (hitp && mat == 1) b1[p1] = i;
+                uint32_t p2 = wave_append(&counters[CTR(C_BUCKET + 2)], hitp && mat == 2);
+                if (hitp && mat == 2) b2[p2] = i;
+            }
+            if (fin) i = kMissCode;
+            if (!exhausted) {
+                bool want = i == kMissCode;
+                uint32_t k = wave_append(&counters[CTR(C_FETCH)], want);
+                exhausted = __ballot(want && k >= n) != 0ull;
+                if (want && k < n) {
+                    i = k;
+                    f3 o{q.ox[i], q.oy[i], q.oz[i]}, d{q.dx[i], q.dy[i], q.dz[i]};
+                    h = closest_spheres(S, o, d);
+                    rb = make_raybox(o, d);
+                    sr = make_shear(o, d);
+                    cur = 0;
+                    st.reset();
+                    walking = S.ntris != 0;
+                    if (!walking) {
+                        hit.t[i] = h.t;
+                        hit.u[i] = h.u;
+                        hit.v[i] = h.v;
+                        hit.code[i] = h.code;
+                    }
+                }
+            }
+            if (__ballot(i != kMissCode) == 0ull) break;
+        }
+        if (walking) {
+            // one step of trace_closest's walk: a node, or a leaf, then the
+            // pop of the next subtree in front of the hit if needed
+            bool pop = true;
+            if (!(cur & kLeafBit)) {
+                if (COUNT) ++nvis;
+                uint32_t next = closest_node<W, kLdsDepth, Q>(S, rb, cur, h.t, st);
+                pop = next == kEmpty;
+                if (!pop) cur = next;
             } else {
-                exhausted = true;
+                uint32_t nt = closest_leaf(S, sr, cur, h);
+                if (COUNT) ntri += nt;
+            }
+            if (pop && !closest_pop(st, h.t, cur)) {
+                walking = false;
+                hit.t[i] = h.t;
+                hit.u[i] = h.u;
+                hit.v[i] = h.v;
+                hit.code[i] = h.code;
             }
         }
-        if (__ballot(has) == 0ull) break;
-        while (true) {
-            if (has && !ts.done) closest_step<W, kLdsDepth, Q>(Sx, st, ts, nvis, ntri);
-            if (__popcll(__ballot(has && !ts.done)) < kMinActive) break;
-        }
-        bool fin = has && ts.done;
-        if (fin) {
-            hit.t[i] = ts.h.t;
-            hit.u[i] = ts.h.u;
-            hit.v[i] = ts.h.v;
-            hit.code[i] = ts.h.code;
-        }
-        bool hitp = fin && ts.h.code != kMissCode;
-        uint32_t mat = 0;
-        if (hitp && sort) {
-            uint32_t c = ts.h.code;
-            uint32_t prim = (c & kSphereBit) ? Sx.sphere_prim[c & ~kSphereBit] : Sx.tri_prim[c];
-            mat = min(Sx.shaders[Sx.prims[prim].shader].type, 2u);  // Lambert, Glossy, everything else
-        }
-        uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
-        if (hitp && mat == 0) b0[p0] = i;
-        if (sort) {
-            uint32_t p1 = wave_append(&counters[CTR(C_BUCKET + 1)], hitp && mat == 1);
-            if (hitp && mat == 1) b1[p1] = i;
-            uint32_t p2 = wave_append(&counters[CTR(C_BUCKET + 2)], hitp && mat == 2);
-            if (hitp && mat == 2) b2[p2] = i;
-        }
-        if (fin) has = false;
     }
     if (COUNT) {
         atomicAdd(&stats[S_NODES], (unsigned long long)nvis);
@@ -1127,9 +1158,8 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(Dev
                                                    float *L, unsigned long long *stats, uint32_t *spill,
                                                    size_t spill_lanes, NrcRender nr, uint32_t bounce,
                                                    ProfView pv) {
-    __shared__ uint32_t sref[kLdsDepth * kBlock];
-    __shared__ float st_[kLdsDepth * kBlock];
-    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
     uint32_t n = counters[CTR(C_SHADOW)];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_SHADOW], (unsigned long long)n);
     uint32_t stride = gridDim.x * blockDim.x;
@@ -1141,7 +1171,9 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(Dev
             uint32_t slot = sq.slot[i];
             DevScene Sx = S;
             if (BATCH) Sx = T.rec[find_job(T, g0 + slot)].S;
-            if (!trace_occluded<W, kLdsDepth, Q>(Sx, o, d, sq.tmax[i], st)) {
+            bool occ = KR_SHADOW_STEPS ? trace_occluded_steps<W, kLdsDepth, Q>(Sx, o, d, sq.tmax[i], st)
+                                       : trace_occluded<W, kLdsDepth, Q>(Sx, o, d, sq.tmax[i], st);
+            if (!occ) {
                 L[3 * slot] += sq.cx[i];
                 L[3 * slot + 1] += sq.cy[i];
                 L[3 * slot + 2] += sq.cz[i];
@@ -1274,16 +1306,18 @@ __global__ __launch_bounds__(kBlock) void k_shade_debug(ShadeArgs a, uint32_t ke
     }
 }
 
-// Persistent shadow traversal (lane refill as in k_trace_dyn); same
+// Persistent shadow traversal: every lane keeps a segment until it is done
+// and takes a new one from the queue (one atomic per wave) once fewer than
+// kMinActive lanes of its wave still walk (Aila and Laine, HPG 2009). Same
 // additions to L and to the training records as k_shadow.
+constexpr int kMinActive = 32;
 template <int W, bool BATCH, bool Q>
 __global__ __launch_bounds__(kBlock) void k_shadow_dyn(DevScene S, JobTable T, uint64_t g0, ShadowQ sq,
                                                        uint32_t *counters, float *L, unsigned long long *stats,
                                                        uint32_t *spill, size_t spill_lanes, NrcRender nr,
                                                        uint32_t bounce, ProfView pv) {
-    __shared__ uint32_t sref[kLdsDepth * kBlock];
-    __shared__ float st_[kLdsDepth * kBlock];
-    auto st = make_stack<kLdsDepth>(sref, st_, spill, spill_lanes);
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
     const uint32_t n = counters[CTR(C_SHADOW)];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_SHADOW], (unsigned long long)n);
     uint32_t i = 0;
@@ -2657,7 +2691,7 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
                            pp.q[0], pp.L, pp.counters, C_RAY0, nrv, uint32_t(seed), uint32_t(seed >> 32), pv, to);
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
-    bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
+    const bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
     const uint32_t kern = r->prm.kernel;
     const uint32_t rounds = kern == KR_KERNEL_RADIANCE ? max_bounce + 1 : 1;
     for (uint32_t b = 0; b < rounds; ++b) {
@@ -2676,8 +2710,12 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
                                    pp.bucket[0], pp.bucket[1], pp.bucket[2], pp.counters, r->stats, sort, pp.spill,
                                    pp.spill_lanes);
             };
-            if (dyn)
-                count ? trace(k_trace_dyn<W, true, BATCH, Q>) : trace(k_trace_dyn<W, false, BATCH, Q>);
+            auto trace_refill = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, S, pp.q[cur], qi, pp.hit, pp.bucket[0],
+                                   pp.bucket[1], pp.bucket[2], pp.counters, r->stats, sort, pp.spill, pp.spill_lanes);
+            };
+            if (!BATCH && dyn)
+                count ? trace_refill(k_trace_refill<W, true, Q>) : trace_refill(k_trace_refill<W, false, Q>);
             else
                 count ? trace(k_trace<W, true, BATCH, Q>) : trace(k_trace<W, false, BATCH, Q>);
         }

@@ -193,7 +193,10 @@ enum kr_kernel_type {
 #define KR_RENDER_NO_MATERIAL_SORT 0x1u /* shade with one mixed-material queue */
 #define KR_RENDER_PERSISTENT_TRAVERSAL 0x2u /* traversal kernels refill lanes whose ray
                                                is done from the queue (persistent
-                                               threads) instead of a fixed share */
+                                               threads) instead of taking a fixed
+                                               share of 64 rays per pass (same
+                                               image; slower on the measured
+                                               scenes) */
 #define KR_RENDER_ONE_STREAM 0x4u /* run every chunk on the device's stream; by
                                      default a render of >= 8 M paths cycles its
                                      chunks through four streams (same image)   */

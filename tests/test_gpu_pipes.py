@@ -100,3 +100,36 @@ def test_tile_ordered_camera_queue_is_bit_identical(device, spheres, which, monk
         out.append(_render(device, scene))
     for rgb, val in out[1:]:
         assert np.array_equal(val, out[0][1]) and np.array_equal(rgb, out[0][0])
+
+
+def _render_stats(device, scene, flags):
+    gs = kr.GpuScene(device, scene.to_desc())
+    r = kr.Render(gs, scene.camera_struct(), scene.params(seed=5, flags=flags))
+    r.set_profiling(kr.KR_PROFILE_COUNTERS)
+    r.run(0, scene.spp)
+    r.sync()
+    st = r.stats()
+    counts = (st.closest_rays, st.node_visits, st.tri_tests)
+    out = r.read()
+    r.free()
+    gs.free()
+    return out, counts
+
+
+@pytest.mark.parametrize("which", ["mesh", "spheres"])
+def test_lane_refill_is_bit_identical(device, spheres, which):
+    """Traversal waves that refill finished lanes from the queue
+    (KR_RENDER_PERSISTENT_TRAVERSAL) walk each ray exactly as waves with a
+    fixed share of rays do: the same node visits and triangle tests in total
+    and the same hits, so the image is the same bit for bit; only the order
+    in which hits enter the shading queues differs. The spheres-only scene has
+    no BVH: every ray is done at its fetch."""
+    scene = sg.mesh_scene(width=256, height=128, spp=16) if which == "mesh" else spheres
+    flags = (0, kr.KR_RENDER_PERSISTENT_TRAVERSAL)
+    runs = [_render_stats(device, scene, f) for f in flags]
+    (rgb0, val0), c0 = runs[0]
+    print(which, "closest rays, node visits, triangle tests:", c0, flush=True)
+    assert c0[0] > 0 and (which == "spheres" or c0[1] > c0[0])
+    for f, ((rgb, val), c) in zip(flags[1:], runs[1:]):
+        assert c == c0, (f, c, c0)
+        assert np.array_equal(val, val0) and np.array_equal(rgb, rgb0), f
