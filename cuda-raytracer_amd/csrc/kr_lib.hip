@@ -835,9 +835,12 @@ struct ShadeArgs {
 };
 
 // RadianceKernel::execute loop body (src/core/kernel/radiance.cu:26-71) for
-// one path vertex.
-template <bool BATCH>
+// one path vertex. NRC: a radiance cache may be attached (a.nr); PROF: a
+// profiler may record vertices (a.pv). The plain instantiation carries
+// neither's code or registers.
+template <bool BATCH, bool NRC, bool PROF>
 __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
+    const bool nr_on = NRC && a.nr.enabled, pv_on = PROF && a.pv.enabled;
     uint32_t n0 = a.counters[CTR(C_BUCKET + 0)], n1 = a.counters[CTR(C_BUCKET + 1)], n2 = a.counters[CTR(C_BUCKET + 2)];
     uint32_t total = n0 + n1 + n2;
     uint32_t stride = gridDim.x * blockDim.x;
@@ -889,7 +892,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
             uint32_t ndr = (flags >> 1) & 3u;
             bool terminate = false;
             float *rec = nullptr;
-            if (a.nr.enabled) {
+            if (nr_on) {
                 bool dr = sh.type == KR_SHADER_LAMBERT || sh.type == KR_SHADER_EMISSION ||
                           sh.type == KR_SHADER_MIXED ||
                           ((sh.type == KR_SHADER_GLOSSY || sh.type == KR_SHADER_GLASS ||
@@ -997,7 +1000,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
             }
             // profiler record of this vertex (radiance.cu:38-50); k_shadow adds
             // the light sample's contribution to its L once it is unoccluded
-            int64_t psl = a.pv.enabled ? prof_slice(a.pv, a.g0 + slot) : -1;
+            int64_t psl = pv_on ? prof_slice(a.pv, a.g0 + slot) : -1;
             if (psl >= 0 && a.bounce < a.pv.depth) {
                 float *pr = a.pv.rec + (size_t(psl) * a.pv.depth + a.bounce) * kProfFloats;
                 pr[0] = o.x;
@@ -1060,7 +1063,7 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
             }  // continued
             }  // not terminated before the light sample
         }
-        if (a.nr.enabled) {
+        if (nr_on) {
             uint32_t pq = wave_append(&a.counters[CTR(C_NRCQ)], push_query);
             if (push_query) {
                 a.nr.qslot[pq] = slot;
@@ -2744,9 +2747,14 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
         a.pv = pv;
         {
             Timed tm(r, K_SHADE, st);
-            if (kern == KR_KERNEL_RADIANCE)
-                hipLaunchKernelGGL((k_shade<BATCH>), dim3(grid), dim3(kBlock), 0, st, a);
-            else
+            auto shade = [&](auto kfn) { hipLaunchKernelGGL(kfn, dim3(grid), dim3(kBlock), 0, st, a); };
+            if (kern == KR_KERNEL_RADIANCE) {
+                // the profiler records single renders only
+                if (nrv.enabled)
+                    (!BATCH && pv.enabled) ? shade(k_shade<false, true, true>) : shade(k_shade<BATCH, true, false>);
+                else
+                    (!BATCH && pv.enabled) ? shade(k_shade<false, false, true>) : shade(k_shade<BATCH, false, false>);
+            } else
                 hipLaunchKernelGGL((k_shade_debug<BATCH>), dim3(grid), dim3(kBlock), 0, st, a, kern);
         }
         {
