@@ -255,10 +255,13 @@ struct NodeView {
     uint32_t ref[W];
 };
 
+// A child ref of an inner node is the node's offset in 16-byte units from
+// the start of the node array (kr_scene_upload rewrites the builder's node
+// indices so), so that its address is a shift, not a multiply.
 template <int W>
-__device__ __forceinline__ void load_node(const float4 *nodes, uint32_t idx, NodeView<W> &nv) {
+__device__ __forceinline__ void load_node(const float4 *nodes, uint32_t ref, NodeView<W> &nv) {
     constexpr int NF4 = int(node_floats(W)) / 4;
-    const float4 *p = nodes + size_t(idx) * NF4;
+    const float4 *p = nodes + ref;
     float f[NF4 * 4];
 #pragma unroll
     for (int i = 0; i < NF4; ++i) {
@@ -282,9 +285,9 @@ __device__ __forceinline__ void load_node(const float4 *nodes, uint32_t idx, Nod
 // Compressed node (qnode_bytes(W) bytes, layout in kr_layout.h): decoded
 // with the same fma(q, s, p) the encoder used to make the boxes conservative.
 template <int W>
-__device__ __forceinline__ void load_qnode(const float4 *nodes, uint32_t idx, NodeView<W> &nv) {
+__device__ __forceinline__ void load_qnode(const float4 *nodes, uint32_t ref, NodeView<W> &nv) {
     constexpr int NU4 = int(qnode_bytes(W)) / 16;
-    const uint4 *p = reinterpret_cast<const uint4 *>(nodes) + size_t(idx) * NU4;
+    const uint4 *p = reinterpret_cast<const uint4 *>(nodes) + ref;
     uint32_t u[NU4 * 4];
 #pragma unroll
     for (int i = 0; i < NU4; ++i) {
@@ -322,11 +325,12 @@ __device__ __forceinline__ void load_qnode(const float4 *nodes, uint32_t idx, No
 // (the empty asm keeps the compiler from holding them in registers across
 // the traversal loop, where they would spill).
 template <int W>
-__device__ __forceinline__ void load_node_nf(const float4 *nodes, uint32_t idx, const RayBox &rb, NodeView<W> &nv) {
-    // 32-bit byte offsets from the array base (node arrays stay far below
-    // 4 GB), so the loads address from a scalar base plus one VGPR
+__device__ __forceinline__ void load_node_nf(const float4 *nodes, uint32_t ref, const RayBox &rb, NodeView<W> &nv) {
+    // 32-bit byte offsets from the array base (kr_scene_upload refuses node
+    // arrays of 4 GB or more), so the loads address from a scalar base plus
+    // one VGPR
     const char *base = reinterpret_cast<const char *>(nodes);
-    const uint32_t nb = idx * uint32_t(node_floats(W) * 4);
+    const uint32_t nb = ref << 4;
     constexpr uint32_t AB = W * 4;  // bytes of one plane array
     uint32_t bits[3] = {__float_as_uint(rb.inv.x), __float_as_uint(rb.inv.y), __float_as_uint(rb.inv.z)};
     asm volatile("" : "+v"(bits[0]), "+v"(bits[1]), "+v"(bits[2]));
@@ -364,11 +368,11 @@ __device__ __forceinline__ void load_node_nf(const float4 *nodes, uint32_t idx, 
 }
 
 template <int W, bool Q>
-__device__ __forceinline__ void fetch_node(const float4 *nodes, uint32_t idx, NodeView<W> &nv) {
+__device__ __forceinline__ void fetch_node(const float4 *nodes, uint32_t ref, NodeView<W> &nv) {
     if (Q)
-        load_qnode<W>(nodes, idx, nv);
+        load_qnode<W>(nodes, ref, nv);
     else
-        load_node<W>(nodes, idx, nv);
+        load_node<W>(nodes, ref, nv);
 }
 
 // Sorts the children's (entry distance, ref) pairs ascending by distance with

@@ -4,8 +4,10 @@
 //
 //  Wide BVH node (W = 2, 4 or 8 children), 16-byte aligned, NODE_FLOATS(W)
 //  floats each:  lo.x[W] hi.x[W] lo.y[W] hi.y[W] lo.z[W] hi.z[W] ref[W] pad
-//  A child ref is an inner-node index, a leaf (LEAF bit | (count-1) << 27 |
-//  first triangle) or EMPTY.
+//  A child ref is an inner node, a leaf (LEAF bit | (count-1) << 27 |
+//  first triangle) or EMPTY. The builder (WideBvh) refers to inner nodes by
+//  index; in device memory the ref is the node's offset in 16-byte units
+//  (index * node bytes / 16), so the kernels address it with a shift.
 //
 //  Triangles in leaf order, 3 float4 each (48 B):
 //    {v0.xyz, prim index bits}, {v1.xyz, global triangle index bits},

@@ -2078,6 +2078,12 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     DevScene &v = s->view;
     int rc = KR_OK;
     s->qnodes = (d->flags & KR_SCENE_COMPRESSED_NODES) != 0;
+    // the kernels address nodes with 32-bit byte offsets
+    const uint32_t node_bytes = s->qnodes ? qnode_bytes(width) : node_floats(width) * 4;
+    if (uint64_t(bvh.num_nodes) * node_bytes >= (uint64_t(1) << 32)) {
+        delete s;
+        return fail(KR_E_INVALID, "BVH of " + std::to_string(bvh.num_nodes) + " nodes exceeds 4 GB");
+    }
     if (s->qnodes) {
         std::vector<uint32_t> qn;
         compress_wide_nodes(bvh, qn);
@@ -2094,10 +2100,25 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
                         rc = fail(KR_E_OVERFLOW, "compressed BVH node does not contain its child box");
             }
         }
+        // inner-node refs become offsets in 16-byte units (see node_ref_units)
+        for (uint32_t n = 0; n < bvh.num_nodes; ++n)
+            for (uint32_t c = 0; c < width; ++c) {
+                uint32_t &ref = qn[size_t(n) * NW + 4 + c];
+                if (ref != kEmpty && !(ref & kLeafBit)) ref *= node_bytes / 16;
+            }
         const uint32_t *nodes_q = nullptr;
         rc = rc ? rc : upload(s, qn.data(), qn.size(), &nodes_q);
         v.nodes = reinterpret_cast<const float4 *>(nodes_q);
     } else {
+        const uint32_t NF = node_floats(width);
+        for (uint32_t n = 0; n < bvh.num_nodes; ++n)
+            for (uint32_t c = 0; c < width; ++c) {
+                float &f = bvh.nodes[size_t(n) * NF + 6 * width + c];
+                uint32_t ref;
+                std::memcpy(&ref, &f, 4);
+                if (ref != kEmpty && !(ref & kLeafBit)) ref *= node_bytes / 16;
+                std::memcpy(&f, &ref, 4);
+            }
         const float *nodes_f = nullptr;
         rc = rc ? rc : upload(s, bvh.nodes.data(), bvh.nodes.size(), &nodes_f);
         v.nodes = reinterpret_cast<const float4 *>(nodes_f);
