@@ -613,7 +613,11 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
 }
 
 // The same any-hit walk one node or one leaf per iteration ("if-if"),
-// without postponed leaves.
+// without postponed leaves, entering the children farthest first: a segment
+// runs from a surface towards a light, and an occluder found early ends it.
+// Measured by a CPU count on the 1M-triangle frame's light segments:
+// farthest-first visits 8 % fewer nodes per 64-segment wave than the
+// children's slot order, nearest-first 1 % more.
 template <int W, int LDS_DEPTH, bool Q = false>
 __device__ __forceinline__ bool trace_occluded_steps(const DevScene &S, f3 o, f3 d, float tmax, Stack<LDS_DEPTH> &st) {
     for (uint32_t s = 0; s < S.nspheres; ++s) {
@@ -634,22 +638,27 @@ __device__ __forceinline__ bool trace_occluded_steps(const DevScene &S, f3 o, f3
                 fetch_node<W, Q>(S.nodes, cur, nv);
             else
                 load_node_nf<W>(S.nodes, cur, rb, nv);
-            uint32_t next = kEmpty;
+            // sort key: minus the entry distance (farthest first), +inf for a
+            // child the segment misses (last)
+            float key[W];
+            uint32_t ref[W];
 #pragma unroll
             for (int c = 0; c < W; ++c) {
                 float tn = Q ? box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
                                          nv.hi[2][c], limit)
                              : box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
                                             nv.hi[2][c], limit);
-                if ((Q ? nv.ref[c] != kEmpty : true) && tn != __builtin_inff()) {
-                    if (next == kEmpty)
-                        next = nv.ref[c];
-                    else
-                        st.push(nv.ref[c], tn);
-                }
+                // (box_enter_nf never enters an empty slot's (+inf, -inf) box)
+                bool in = (Q ? nv.ref[c] != kEmpty : true) && tn != __builtin_inff();
+                key[c] = in ? -tn : __builtin_inff();
+                ref[c] = nv.ref[c];
             }
-            pop = next == kEmpty;
-            if (!pop) cur = next;
+            sort_children<W>(key, ref);
+#pragma unroll
+            for (int c = W - 1; c >= 1; --c)
+                if (key[c] != __builtin_inff()) st.push(ref[c], -key[c]);
+            pop = key[0] == __builtin_inff();
+            if (!pop) cur = ref[0];
         } else {
             uint32_t first = cur & kLeafFirstMask;
             uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
