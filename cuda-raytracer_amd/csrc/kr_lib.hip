@@ -21,7 +21,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "bvh_build.h"
@@ -118,17 +121,25 @@ inline H3 hcross(H3 a, H3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * 
 inline H3 hnorm(H3 a) { return hmul(a, 1.0f / std::sqrt(hdot(a, a))); }
 inline H3 hload(const float *p) { return {p[0], p[1], p[2]}; }
 
-template <class T>
-int dalloc(T **p, size_t n) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
-    if (e != hipSuccess) return fail(KR_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    return KR_OK;
-}
 }  // namespace
 
 // ---------------------------------------------------------- handle types --
+// Device memory cache: every buffer of the library's scenes, renders,
+// batches and radiance caches comes from its device's cache. hipFree waits
+// for the whole device and unmaps the pages (40 to 300 ms for the ~60
+// buffers of a 1080p render), so freed buffers are kept, keyed by size, and
+// handed to the next request they fit (at most 1/8 larger). kr_device_trim
+// and kr_device_close give them back to HIP, and a request that HIP cannot
+// serve first releases the idle buffers and then tries once more. Callers
+// free a buffer only after the work that uses it is done (every *_free
+// synchronises its streams first).
+struct MemCache {
+    std::mutex mu;
+    std::multimap<size_t, void *> idle;           // bytes -> idle buffer
+    std::unordered_map<void *, size_t> bytes_of;  // every buffer made, idle or in use
+    size_t idle_bytes = 0;
+};
+
 struct kr_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;
@@ -136,7 +147,73 @@ struct kr_device {
     std::string name;
     // device address of this device's copy of `scratch` (kr_trace_closest_noalloc)
     uint2 *scratch = nullptr;
+    mutable MemCache mem;
 };
+
+namespace {
+// requests round up to 256 B, and from 64 KiB on to 64 KiB
+size_t cache_class(size_t bytes) {
+    if (bytes == 0) bytes = 1;
+    const size_t g = bytes >= (size_t(64) << 10) ? (size_t(64) << 10) : 256;
+    return (bytes + g - 1) / g * g;
+}
+
+void cache_release(const kr_device *dev) {
+    std::lock_guard<std::mutex> lk(dev->mem.mu);
+    for (auto &kv : dev->mem.idle) {
+        dev->mem.bytes_of.erase(kv.second);
+        (void)hipFree(kv.second);
+    }
+    dev->mem.idle.clear();
+    dev->mem.idle_bytes = 0;
+}
+
+int cache_alloc(const kr_device *dev, size_t bytes, void **p) {
+    *p = nullptr;
+    const size_t want = cache_class(bytes);
+    {
+        std::lock_guard<std::mutex> lk(dev->mem.mu);
+        auto it = dev->mem.idle.lower_bound(want);
+        if (it != dev->mem.idle.end() && it->first <= want + want / 8) {
+            *p = it->second;
+            dev->mem.idle_bytes -= it->first;
+            dev->mem.idle.erase(it);
+            return KR_OK;
+        }
+    }
+    hipError_t e = hipMalloc(p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        cache_release(dev);
+        e = hipMalloc(p, want);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(KR_E_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> lk(dev->mem.mu);
+    dev->mem.bytes_of[*p] = want;
+    return KR_OK;
+}
+
+template <class T>
+int dalloc(const kr_device *dev, T **p, size_t n) {
+    void *v = nullptr;
+    int rc = cache_alloc(dev, (n ? n : 1) * sizeof(T), &v);
+    *p = static_cast<T *>(v);
+    return rc;
+}
+
+// returns a buffer of dalloc to the device's cache
+void dfree(const kr_device *dev, void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(dev->mem.mu);
+    auto it = dev->mem.bytes_of.find(p);
+    if (it == dev->mem.bytes_of.end()) return;
+    dev->mem.idle.emplace(it->second, p);
+    dev->mem.idle_bytes += it->second;
+}
+}  // namespace
 
 // Scratch memory of kr_trace_closest_noalloc: the traversal spill stacks of
 // the lanes of one launch, (ref, t) pairs, entry k of lane g at k * lanes + g
@@ -1782,7 +1859,21 @@ KR_EXPORT int kr_device_close(kr_device *dev) {
     (void)hipSetDevice(dev->ordinal);
     (void)hipStreamSynchronize(dev->stream);
     (void)hipStreamDestroy(dev->stream);
+    cache_release(dev);
     delete dev;
+    return KR_OK;
+}
+
+KR_EXPORT int kr_device_trim(kr_device *dev, size_t *released) {
+    if (!dev) return fail(KR_E_INVALID, "NULL device");
+    (void)hipSetDevice(dev->ordinal);
+    size_t idle;
+    {
+        std::lock_guard<std::mutex> lk(dev->mem.mu);
+        idle = dev->mem.idle_bytes;
+    }
+    cache_release(dev);
+    if (released) *released = idle;
     return KR_OK;
 }
 
@@ -1791,11 +1882,11 @@ KR_EXPORT int kr_device_bandwidth(kr_device *dev, size_t bytes, uint32_t iters, 
     (void)hipSetDevice(dev->ordinal);
     size_t n = bytes / 16;
     float4 *a = nullptr, *b = nullptr;
-    int rc = dalloc(&a, n);
+    int rc = dalloc(dev, &a, n);
     if (rc) return rc;
-    rc = dalloc(&b, n);
+    rc = dalloc(dev, &b, n);
     if (rc) {
-        (void)hipFree(a);
+        dfree(dev, a);
         return rc;
     }
     hipStream_t st = dev->stream;
@@ -1813,8 +1904,8 @@ KR_EXPORT int kr_device_bandwidth(kr_device *dev, size_t bytes, uint32_t iters, 
     (void)hipEventElapsedTime(&ms, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
+    dfree(dev, a);
+    dfree(dev, b);
     if (e != hipSuccess) return fail(KR_E_HIP, std::string("k_copy: ") + hipGetErrorString(e));
     *gbps = 2.0 * double(n * 16) * iters / (double(ms) * 1e-3) / 1e9;
     return KR_OK;
@@ -1824,7 +1915,7 @@ KR_EXPORT int kr_scene_free(kr_scene *s) {
     if (!s) return KR_OK;
     (void)hipSetDevice(s->dev->ordinal);
     (void)hipStreamSynchronize(s->dev->stream);
-    for (void *p : s->allocs) (void)hipFree(p);
+    for (void *p : s->allocs) dfree(s->dev, p);
     delete s;
     return KR_OK;
 }
@@ -1833,7 +1924,7 @@ namespace {
 template <class T>
 int upload(kr_scene *s, const T *host, size_t n, const T **dst) {
     T *p = nullptr;
-    int rc = dalloc(&p, n);
+    int rc = dalloc(s->dev, &p, n);
     if (rc) return rc;
     s->allocs.push_back(p);
     if (n) KR_HIP(hipMemcpyAsync(p, host, n * sizeof(T), hipMemcpyHostToDevice, s->dev->stream));
@@ -1869,7 +1960,7 @@ int ensure_spill(uint32_t *&spill, size_t &lanes, std::vector<void *> &allocs, c
     if (spill) return KR_OK;
     lanes = want_lanes ? want_lanes : grid_lanes(dev);
     if (depth == 0 || depth > uint32_t(kSpillDepth)) depth = kSpillDepth;
-    int rc = dalloc(&spill, lanes * depth * 2);
+    int rc = dalloc(dev, &spill, lanes * depth * 2);
     if (rc) return rc;
     allocs.push_back(spill);
     return KR_OK;
@@ -2201,13 +2292,15 @@ int dispatch_fmt(uint32_t w, bool q, F f) {
 
 
 struct DevBufs {
+    const kr_device *dev;
     std::vector<void *> ptrs;
+    explicit DevBufs(const kr_device *d) : dev(d) {}
     ~DevBufs() {
-        for (void *p : ptrs) (void)hipFree(p);
+        for (void *p : ptrs) dfree(dev, p);
     }
     template <class T>
     int get(T **p, size_t n) {
-        int rc = dalloc(p, n);
+        int rc = dalloc(dev, p, n);
         if (!rc) ptrs.push_back(*p);
         return rc;
     }
@@ -2229,7 +2322,7 @@ KR_EXPORT int kr_trace_closest(kr_scene *s, const kr_rays *rays, uint32_t n, kr_
     hipStream_t stream = s->dev->stream;
     int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev, 0, spill_depth(s->view.stack_cap));
     if (rc) return rc;
-    DevBufs b;
+    DevBufs b(s->dev);
     const float *o = rays->origin, *d = rays->direction;
     float *t = hits->t, *u = hits->u, *v = hits->v;
     uint32_t *prim = hits->prim, *tri = hits->tri;
@@ -2313,7 +2406,7 @@ KR_EXPORT int kr_trace_shadow(kr_scene *s, const kr_rays *rays, uint32_t n, uint
     hipStream_t stream = s->dev->stream;
     int rc = ensure_spill(s->spill, s->spill_lanes, s->allocs, s->dev, 0, spill_depth(s->view.stack_cap));
     if (rc) return rc;
-    DevBufs b;
+    DevBufs b(s->dev);
     const float *o = rays->origin, *d = rays->direction, *tm = rays->tmax;
     uint8_t *occ = occluded;
     if (mem == KR_MEM_HOST) {
@@ -2354,14 +2447,14 @@ KR_EXPORT int kr_render_free(kr_render *r) {
     (void)hipStreamSynchronize(r->scene->dev->stream);
     for (Pipe &pp : r->pipe)
         if (pp.own_stream) (void)hipStreamSynchronize(pp.stream);
-    for (void *p : r->allocs) (void)hipFree(p);
+    for (void *p : r->allocs) dfree(r->scene->dev, p);
     for (auto &pe : r->pending) {
         (void)hipEventDestroy(pe.second.first);
         (void)hipEventDestroy(pe.second.second);
     }
     for (hipEvent_t e : r->event_pool) (void)hipEventDestroy(e);
-    for (void *p : r->nrc_allocs) (void)hipFree(p);
-    for (void *p : r->prof_allocs) (void)hipFree(p);
+    for (void *p : r->nrc_allocs) dfree(r->scene->dev, p);
+    for (void *p : r->prof_allocs) dfree(r->scene->dev, p);
     for (Pipe &pp : r->pipe) {
         if (pp.acc_done) (void)hipEventDestroy(pp.acc_done);
         if (pp.own_stream) (void)hipStreamDestroy(pp.stream);
@@ -2417,7 +2510,7 @@ int check_view(const kr_camera *cam, const kr_params *prm) {
 int render_buffers(kr_render *r, size_t lanes, uint32_t spill_entries) {
     size_t P = r->P;
     auto get = [&](auto **p, size_t cnt) {
-        int rc = dalloc(p, cnt);
+        int rc = dalloc(r->scene->dev, p, cnt);
         if (!rc) r->allocs.push_back(*p);
         return rc;
     };
@@ -2702,8 +2795,10 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
     // the radiance cache of the chunk: the render's, or a batch job's
     const NrcRender nrv = jc ? jc->nr : (BATCH ? NrcRender{} : pp.nr);
     kr_nrc *const nrc = jc ? jc->nrc : r->nrc;
-    int grid = r->grid;
-    int cam_grid = int(std::min<size_t>((P + kBlock - 1) / kBlock, size_t(grid)));
+    // no more lanes than the chunk has paths: a small chunk's launches would
+    // otherwise start blocks that find no work
+    int grid = int(std::min<size_t>((P + kBlock - 1) / kBlock, size_t(r->grid)));
+    int cam_grid = grid;
     int cur = 0;
     bool count = (r->profile & KR_PROFILE_COUNTERS) != 0;
     {
@@ -2904,7 +2999,7 @@ KR_EXPORT int kr_render_set_profiler(kr_render *r, const kr_profiler_config *cfg
     if (cfg && r->nrc) return fail(KR_E_UNSUPPORTED, "a render with a radiance cache takes no profiler");
     (void)hipSetDevice(r->scene->dev->ordinal);
     KR_HIP(hipStreamSynchronize(r->scene->dev->stream));
-    for (void *p : r->prof_allocs) (void)hipFree(p);
+    for (void *p : r->prof_allocs) dfree(r->scene->dev, p);
     r->prof_allocs.clear();
     r->pv = ProfView{};
     r->prof_slices = 0;
@@ -2927,12 +3022,12 @@ KR_EXPORT int kr_render_set_profiler(kr_render *r, const kr_profiler_config *cfg
     pv.depth = cfg->depth;
     size_t nrec = std::max<size_t>(1, size_t(slices) * cfg->depth * kProfFloats);
     size_t nfin = std::max<size_t>(1, size_t(slices) * 4);
-    int rc = dalloc(&pv.rec, nrec);
+    int rc = dalloc(r->scene->dev, &pv.rec, nrec);
     if (!rc) r->prof_allocs.push_back(pv.rec);
-    if (!rc) rc = dalloc(&pv.fin, nfin);
+    if (!rc) rc = dalloc(r->scene->dev, &pv.fin, nfin);
     if (!rc) r->prof_allocs.push_back(pv.fin);
     if (rc) {
-        for (void *p : r->prof_allocs) (void)hipFree(p);
+        for (void *p : r->prof_allocs) dfree(r->scene->dev, p);
         r->prof_allocs.clear();
         return rc;
     }
@@ -3111,7 +3206,7 @@ int nrc_step(kr_nrc *n, const TrainSource &src, uint32_t batch, uint32_t rows) {
 
 template <class T>
 int nrc_alloc(kr_nrc *n, T **p, size_t cnt) {
-    int rc = dalloc(p, cnt);
+    int rc = dalloc(n->dev, p, cnt);
     if (!rc) n->allocs.push_back(*p);
     return rc;
 }
@@ -3121,7 +3216,7 @@ KR_EXPORT int kr_nrc_free(kr_nrc *n) {
     if (!n) return KR_OK;
     (void)hipSetDevice(n->dev->ordinal);
     (void)hipStreamSynchronize(n->dev->stream);
-    for (void *p : n->allocs) (void)hipFree(p);
+    for (void *p : n->allocs) dfree(n->dev, p);
     delete n;
     return KR_OK;
 }
@@ -3221,7 +3316,7 @@ KR_EXPORT int kr_nrc_set_params(kr_nrc *n, const float *params) {
 KR_EXPORT int kr_nrc_encode(kr_nrc *n, const float *raw, uint32_t cnt, float *features) {
     if (!n || !raw || !features) return fail(KR_E_INVALID, "NULL argument");
     if (!cnt) return KR_OK;
-    DevBufs b;
+    DevBufs b(n->dev);
     float *r, *f;
     int rc;
     if ((rc = b.get(&r, size_t(cnt) * nrc::kRaw)) || (rc = b.get(&f, size_t(cnt) * nrc::kIn))) return rc;
@@ -3237,7 +3332,7 @@ KR_EXPORT int kr_nrc_encode(kr_nrc *n, const float *raw, uint32_t cnt, float *fe
 KR_EXPORT int kr_nrc_predict(kr_nrc *n, const float *raw, uint32_t cnt, float *rgb) {
     if (!n || !raw || !rgb) return fail(KR_E_INVALID, "NULL argument");
     if (!cnt) return KR_OK;
-    DevBufs b;
+    DevBufs b(n->dev);
     float *r, *o;
     int rc;
     if ((rc = b.get(&r, size_t(cnt) * nrc::kRaw)) || (rc = b.get(&o, size_t(cnt) * 3))) return rc;
@@ -3280,7 +3375,7 @@ int nrc_explicit_source(kr_nrc *n, DevBufs &b, const float *raw, const float *ta
 KR_EXPORT int kr_nrc_gradient(kr_nrc *n, const float *raw, const float *target, uint32_t cnt, float *grad,
                               float *loss) {
     if (!n || !raw || !target || !grad || !cnt) return fail(KR_E_INVALID, "bad argument");
-    DevBufs b;
+    DevBufs b(n->dev);
     TrainSource src;
     float *part;
     uint32_t rows;
@@ -3301,7 +3396,7 @@ KR_EXPORT int kr_nrc_gradient(kr_nrc *n, const float *raw, const float *target, 
 
 KR_EXPORT int kr_nrc_train(kr_nrc *n, const float *raw, const float *target, uint32_t cnt, uint32_t steps, float *loss) {
     if (!n || !raw || !target || !cnt) return fail(KR_E_INVALID, "bad argument");
-    DevBufs b;
+    DevBufs b(n->dev);
     TrainSource src;
     float *part;
     uint32_t rows;
@@ -3488,7 +3583,7 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     (void)hipSetDevice(r->scene->dev->ordinal);
     (void)hipStreamSynchronize(r->scene->dev->stream);
     for (int pi = 0; pi < r->npipes; ++pi) (void)hipStreamSynchronize(r->pipe[pi].stream);
-    for (void *p : r->nrc_allocs) (void)hipFree(p);
+    for (void *p : r->nrc_allocs) dfree(r->scene->dev, p);
     r->nrc_allocs.clear();
     r->nr = NrcRender{};
     for (Pipe &pp : r->pipe) pp.nr = NrcRender{};
@@ -3506,7 +3601,7 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
     // groups a chunk of P paths can touch: P / 2^log2 + 2 partial ones
     nr.train_cap = uint32_t(std::min<uint64_t>(r->P, (uint64_t(r->P) >> log2) + 2));
     auto get = [&](auto **p, size_t cnt) {
-        int rc = dalloc(p, cnt);
+        int rc = dalloc(r->scene->dev, p, cnt);
         if (!rc) r->nrc_allocs.push_back(*p);
         return rc;
     };
@@ -3524,7 +3619,7 @@ KR_EXPORT int kr_render_attach_nrc(kr_render *r, kr_nrc *n) {
         if (!rc) rc = get(&pn.qraw, size_t(r->P) * nrc::kRaw);
     }
     if (rc) {
-        for (void *p : r->nrc_allocs) (void)hipFree(p);
+        for (void *p : r->nrc_allocs) dfree(r->scene->dev, p);
         r->nrc_allocs.clear();
         r->nr = NrcRender{};
         for (Pipe &pp : r->pipe) pp.nr = NrcRender{};
@@ -3782,10 +3877,10 @@ KR_EXPORT int kr_batch_free(kr_batch *b) {
     batch_drop_graph(b);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
-    for (void *p : b->nrc_allocs) (void)hipFree(p);
+    for (void *p : b->nrc_allocs) dfree(b->dev, p);
     if (b->r) kr_render_free(b->r);
     for (kr_scene *s : b->up) kr_scene_free(s);
-    for (void *p : b->allocs) (void)hipFree(p);
+    for (void *p : b->allocs) dfree(b->dev, p);
     delete b;
     return KR_OK;
 }
@@ -3862,7 +3957,7 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
         c.desc.shaders = c.shaders.data();
     }
     auto get = [&](auto **p, size_t cnt) {
-        int rc = dalloc(p, cnt);
+        int rc = dalloc(b->dev, p, cnt);
         if (!rc) b->allocs.push_back(*p);
         return rc;
     };
@@ -3886,9 +3981,9 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
         // pipes 1.. accumulate into images of their own (kr_batch_read sums them)
         for (int pi = 1; pi < r->npipes && !rc; ++pi) {
             Pipe &pp = r->pipe[pi];
-            rc = dalloc(&pp.acc, 3 * size_t(r->npix));
+            rc = dalloc(b->dev, &pp.acc, 3 * size_t(r->npix));
             if (!rc) r->allocs.push_back(pp.acc);
-            if (!rc) rc = dalloc(&pp.valid, size_t(r->npix));
+            if (!rc) rc = dalloc(b->dev, &pp.valid, size_t(r->npix));
             if (!rc) r->allocs.push_back(pp.valid);
         }
         if (!rc) rc = kr_render_reset(r);
@@ -3916,7 +4011,7 @@ KR_EXPORT int kr_batch_attach_nrc(kr_batch *b, uint32_t job, kr_nrc *n) {
     b->nrc.resize(b->jobs.size(), nullptr);
     b->nrc[job] = n;
     // buffers for the smallest training group any attached cache uses
-    for (void *p : b->nrc_allocs) (void)hipFree(p);
+    for (void *p : b->nrc_allocs) dfree(b->dev, p);
     b->nrc_allocs.clear();
     for (NrcRender &x : b->pnr) x = NrcRender{};
     batch_drop_graph(b);  // the captured chunks carry the cache views
@@ -3928,7 +4023,7 @@ KR_EXPORT int kr_batch_attach_nrc(kr_batch *b, uint32_t job, kr_nrc *n) {
     const uint32_t nv = b->max_bounce + 1;
     const uint32_t cap = uint32_t(std::min<uint64_t>(P, (uint64_t(P) >> log2) + 2));
     auto get = [&](auto **p, size_t cnt) {
-        int rc = dalloc(p, cnt);
+        int rc = dalloc(b->dev, p, cnt);
         if (!rc) b->nrc_allocs.push_back(*p);
         return rc;
     };
@@ -3944,7 +4039,7 @@ KR_EXPORT int kr_batch_attach_nrc(kr_batch *b, uint32_t job, kr_nrc *n) {
         if (!rc) rc = get(&pn.qraw, size_t(P) * nrc::kRaw);
     }
     if (rc) {
-        for (void *p : b->nrc_allocs) (void)hipFree(p);
+        for (void *p : b->nrc_allocs) dfree(b->dev, p);
         b->nrc_allocs.clear();
         for (NrcRender &x : b->pnr) x = NrcRender{};
         std::fill(b->nrc.begin(), b->nrc.end(), nullptr);

@@ -149,7 +149,8 @@ EXPORTS = ["kr_version", "kr_last_error", "kr_device_count", "kr_device_open", "
            "kr_nrc_get_params", "kr_nrc_set_params", "kr_nrc_get_render_params", "kr_nrc_encode", "kr_nrc_predict", "kr_nrc_gradient",
            "kr_nrc_train", "kr_nrc_get_stats", "kr_render_attach_nrc", "kr_nrc_read_pool",
            "kr_batch_create", "kr_batch_autotune", "kr_batch_run", "kr_batch_sync", "kr_batch_reset", "kr_batch_read",
-           "kr_batch_get_info", "kr_batch_free", "kr_trace_closest_noalloc", "kr_batch_attach_nrc"]
+           "kr_batch_get_info", "kr_batch_free", "kr_trace_closest_noalloc", "kr_batch_attach_nrc",
+           "kr_device_trim"]
 
 _lib = None
 
@@ -188,6 +189,7 @@ def load(path: str = LIB_PATH):
     lib.kr_render_profiler_size.argtypes = [vp, C.POINTER(u64), C.POINTER(u32)]
     lib.kr_render_read_slices.argtypes = [vp, u64, u64, vp, vp, vp]
     lib.kr_device_bandwidth.argtypes = [vp, C.c_size_t, u32, C.POINTER(C.c_double)]
+    lib.kr_device_trim.argtypes = [vp, C.POINTER(C.c_size_t)]
     lib.kr_nrc_create.argtypes = [vp, C.POINTER(NrcConfig), C.POINTER(vp)]
     lib.kr_nrc_free.argtypes = [vp]
     lib.kr_nrc_get_params.argtypes = [vp, vp]
@@ -248,6 +250,13 @@ class Device:
         g = C.c_double()
         check(self.lib.kr_device_bandwidth(self.h, nbytes, iters, C.byref(g)), "kr_device_bandwidth")
         return g.value
+
+    def trim(self) -> int:
+        """Gives the device buffers that freed scenes, renders, batches and
+        caches left cached back to HIP; returns their bytes."""
+        n = C.c_size_t()
+        check(self.lib.kr_device_trim(self.h, C.byref(n)), "kr_device_trim")
+        return n.value
 
     def close(self):
         if self.h:

@@ -257,9 +257,19 @@ int kr_last_error(char *buf, size_t len);
 /* Number of visible HIP devices. */
 int kr_device_count(int *count);
 
-/* Opens device `ordinal` and creates its stream. */
+/* Opens device `ordinal` and creates its stream. Closing it releases the
+ * device memory its scenes, renders, batches and caches left cached. */
 int kr_device_open(int ordinal, kr_device **out);
 int kr_device_close(kr_device *dev);
+
+/* The library keeps the device buffers of freed scenes, renders, batches and
+ * caches for later ones of a size they fit (hipFree waits for the whole
+ * device and unmaps pages: tens to hundreds of ms per 1080p render).
+ * kr_device_trim gives the idle ones back to HIP and reports their bytes.
+ * No reference counterpart: the reference allocates and frees device memory
+ * with each launch's arguments (cudaMallocManaged / cudaFree,
+ * src/poly/kernel.hpp:83-116). */
+int kr_device_trim(kr_device *dev, size_t *released);
 
 /* Streams a plain device-to-device copy of `bytes` bytes (16-byte vector
  * loads and stores, kernel k_copy) `iters` times on the device's stream and
