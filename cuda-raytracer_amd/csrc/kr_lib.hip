@@ -2743,9 +2743,9 @@ struct JobChunk {
 // and the tiles divide it evenly, else the queue stays in row-major order.
 // KR_CAMERA_TILE="WxHxS" (e.g. "8x8x1") overrides the shape; "0" turns the
 // tile order off.
-TileOrder tile_order(const kr_render *r, uint64_t g0, uint32_t P) {
+TileOrder tile_order(uint32_t w, uint32_t h, uint64_t g0, uint32_t P) {
     TileOrder to{};
-    const uint32_t w = r->prm.width, h = r->prm.height, npix = r->npix;
+    const uint32_t npix = w * h;
     if (g0 % npix || P % npix) return to;
     const uint32_t sc = P / npix;  // samples in the chunk
     uint32_t tw, th, sg;
@@ -2805,7 +2805,8 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
         Timed tm(r, K_OTHER, st);
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, pp.counters, 0xFFFFFFFFu);
         TileOrder to{};
-        if (!BATCH && !jc && !pv.enabled) to = tile_order(r, g0, P);
+        if (!BATCH && !pv.enabled)
+            to = jc ? tile_order(jc->lens.w, jc->lens.h, g0, P) : tile_order(r->prm.width, r->prm.height, g0, P);
         hipLaunchKernelGGL((k_camera<BATCH>), dim3(cam_grid), dim3(kBlock), 0, st, lens, T, g0, P, npix,
                            pp.q[0], pp.L, pp.counters, C_RAY0, nrv, uint32_t(seed), uint32_t(seed >> 32), pv, to);
     }
@@ -3763,17 +3764,23 @@ std::vector<std::pair<uint64_t, uint32_t>> batch_chunks(const kr_batch *b, const
     if (b->tuning.flags & KR_BATCH_MIX_JOBS) {
         for (uint64_t g = 0; g < total; g += P) c.push_back({g, uint32_t(std::min<uint64_t>(P, total - g))});
     } else {
-        for (const HostJob &J : *bv.jobs)
-            for (uint64_t g = J.begin; g < J.begin + J.paths; g += P)
-                c.push_back({g, uint32_t(std::min<uint64_t>(P, J.begin + J.paths - g))});
+        // a job's chunks hold whole samples of its frame, as many as fit in P,
+        // spread evenly over the fewest chunks (so that their camera rays
+        // enter the queue in tile order, as a single render's)
+        for (const HostJob &J : *bv.jobs) {
+            uint64_t step = P;
+            if (J.npix <= P && J.paths % J.npix == 0) {
+                const uint64_t spp = J.paths / J.npix, fit = P / J.npix;
+                const uint64_t m = (spp + fit - 1) / fit;
+                step = (spp + m - 1) / m * J.npix;
+            }
+            for (uint64_t g = J.begin; g < J.begin + J.paths; g += step)
+                c.push_back({g, uint32_t(std::min<uint64_t>(step, J.begin + J.paths - g))});
+        }
     }
     return c;
 }
 
-// Enqueues on pipe pi the chunks k = pi, pi + npipes, ... of the batch's
-// chunk list. A batch's pipes accumulate into images of their own (read back
-// summed), so the pipes never wait for each other and each pipe's chunks form
-// a graph of their own.
 // A cached job's view over pipe pi's cache buffers: the buffers, and the
 // job's cache's path selection and ending rule (as kr_render_attach_nrc).
 NrcRender batch_nrc_view(const kr_batch *b, int pi, const kr_nrc *n) {
@@ -3788,6 +3795,10 @@ NrcRender batch_nrc_view(const kr_batch *b, int pi, const kr_nrc *n) {
     return nr;
 }
 
+// Enqueues on pipe pi the chunks k = pi, pi + npipes, ... of the batch's
+// chunk list. A batch's pipes accumulate into images of their own (read back
+// summed), so the pipes never wait for each other and each pipe's chunks form
+// a graph of their own.
 int batch_enqueue_pipe(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P, int pi) {
     kr_render *r = b->r;
     const std::vector<HostJob> &hj = *bv.jobs;
