@@ -3795,16 +3795,21 @@ NrcRender batch_nrc_view(const kr_batch *b, int pi, const kr_nrc *n) {
     return nr;
 }
 
-// Enqueues on pipe pi the chunks k = pi, pi + npipes, ... of the batch's
-// chunk list. A batch's pipes accumulate into images of their own (read back
-// summed), so the pipes never wait for each other and each pipe's chunks form
-// a graph of their own.
+// Enqueues on pipe pi its share of the batch's chunk list. A batch's pipes
+// accumulate into images of their own (read back summed), so the pipes never
+// wait for each other and each pipe's chunks form a graph of their own.
 int batch_enqueue_pipe(kr_batch *b, const BatchView &bv, uint64_t total, uint32_t P, int pi) {
     kr_render *r = b->r;
     const std::vector<HostJob> &hj = *bv.jobs;
     Pipe &pp = r->pipe[pi];
     auto chunks = batch_chunks(b, bv, total, P);
-    for (size_t k = size_t(pi); k < chunks.size(); k += size_t(r->npipes)) {
+    // each chunk in list order goes to the pipe with the fewest paths so far,
+    // so that the pipes, which run concurrently, end together
+    std::vector<uint64_t> load(size_t(r->npipes), 0);
+    for (size_t k = 0; k < chunks.size(); ++k) {
+        const int owner = int(std::min_element(load.begin(), load.end()) - load.begin());
+        load[size_t(owner)] += chunks[k].second;
+        if (owner != pi) continue;
         uint64_t g = chunks[k].first;
         uint32_t n = chunks[k].second;
         // a chunk inside one job runs the single-render kernels (scene views
