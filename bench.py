@@ -637,8 +637,14 @@ def main():
             "launches": int(st.trace_launches),
             "launch_time": "mean of HIP-event durations of every k_trace launch over the timed steps, each "
                            "recorded on the stream the launch runs on (four streams per frame share the GPU)",
-            "limiter": "VALU issue of divergent traversal waves and L2 misses to the MALL, not HBM bandwidth "
-                       "(SQ and TCC counters of this kernel)",
+            # the HBM roofline is the reference line, not the limiter: on one
+            # stream the kernel's real HBM + MALL traffic is 3.18 GB per launch
+            # (23 % of peak) against 15 GB of algorithmic bytes, and it issues
+            # VALU instructions in 79 % of the SIMDs' cycles with its vector
+            # memory units 89-91 % busy (rocprofv3 FETCH_SIZE / WRITE_SIZE, SQ,
+            # TA and TD counters of this frame)
+            "limiter": "VALU issue (79 % of SIMD cycles) and the vector memory pipeline (TA/TD 89-91 % busy) of "
+                       "divergent traversal waves (30 % of lanes active); 79 % of the algorithmic bytes hit in L2",
         },
         "cpu_baseline": cpu,
     }
