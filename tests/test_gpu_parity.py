@@ -3,7 +3,9 @@ oracle (oracle/liboracle.so) on identical inputs.
 
 Bars (the hot path's contract):
   * closest hit: primitive (and triangle) agreement >= 99.99% on identical
-    rays; hit t within 1e-5 relative to max(t, 1) where both agree;
+    rays; where both agree, hit t within 1e-5 relative to t on camera rays
+    and to max(t, 1) on rays from surfaces or anywhere in the scene, on all
+    but 1e-4 of the hits;
   * radiance: the GPU draws the oracle's Philox numbers, so a path's radiance
     is compared path by path; FP32 shading vs FP64 makes a small fraction of
     paths branch differently (Russian roulette threshold, grazing hits), so
@@ -50,7 +52,7 @@ def _random_rays(scene, n, seed, spread=6.0):
     return o.astype(np.float32), d.astype(np.float32)
 
 
-def _compare_closest(g, c, min_agree=0.9999):
+def _compare_closest(g, c, min_agree=0.9999, t_floor=1.0):
     gt, gp, gtri, gu, gv = g
     ct, cp, ctri, cu, cv = c
     n = gt.shape[0]
@@ -58,9 +60,12 @@ def _compare_closest(g, c, min_agree=0.9999):
     frac = agree.mean()
     assert frac >= min_agree, f"closest-hit agreement {frac:.6f} over {n} rays"
     both = agree & (gp != kr.KR_MISS)
-    # relative to max(t, 1): below t = 1 the FP32 resolution of O(1-10) world
-    # coordinates (~1e-6 absolute) bounds any FP32 intersection
-    rel = np.abs(gt[both].astype(np.float64) - ct[both]) / np.maximum(np.abs(ct[both]), 1.0)
+    # relative to max(t, t_floor): camera rays (t_floor 0) are held to t
+    # itself; rays that start on a surface or anywhere in the scene often hit
+    # at t << 1, where the FP32 resolution of O(1-10) world coordinates
+    # (~1e-6 absolute) bounds any FP32 intersection, so they are held to
+    # max(t, 1)
+    rel = np.abs(gt[both].astype(np.float64) - ct[both]) / np.maximum(np.abs(ct[both]), t_floor)
     bad = (rel > REL_T).mean() if both.any() else 0.0
     return frac, rel, bad
 
@@ -80,8 +85,8 @@ def test_closest_hit_sphere_scene(device, spheres):
     orc = ol.OracleScene(spheres, "f64")
     p = spheres.params()
     rays = [_camera_rays(spheres, p, 0), _camera_rays(spheres, p, 15), _random_rays(spheres, 100000, 1)]
-    for o, d in rays:
-        frac, rel, bad = _compare_closest(gs.trace_closest(o, d), orc.trace_closest(o, d))
+    for k, (o, d) in enumerate(rays):
+        frac, rel, bad = _compare_closest(gs.trace_closest(o, d), orc.trace_closest(o, d), t_floor=0.0 if k < 2 else 1.0)
         assert bad <= 1e-4, f"t beyond {REL_T} relative on {bad:.2e} of hits (max {rel.max():.2e})"
     gs.free()
 
@@ -91,10 +96,10 @@ def test_closest_hit_mesh_scene(device, mesh):
     orc = ol.OracleScene(mesh, "f64", traversal=1)
     p = mesh.params(width=256, height=128)
     rays = [_camera_rays(mesh, p, 0), _surface_rays(mesh, orc, 50000, 2), _random_rays(mesh, 50000, 3, 2.0)]
-    for o, d in rays:
+    for k, (o, d) in enumerate(rays):
         g = gs.trace_closest(o, d)
         c = orc.trace_closest(o, d)
-        frac, rel, bad = _compare_closest(g, c)
+        frac, rel, bad = _compare_closest(g, c, t_floor=0.0 if k == 0 else 1.0)
         assert bad <= 1e-4, f"t beyond {REL_T} relative on {bad:.2e} of hits (max {rel.max():.2e})"
         both = (g[1] == c[1]) & (g[1] != kr.KR_MISS)
         assert np.abs(g[3][both] - c[3][both]).max() < 1e-3
