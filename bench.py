@@ -450,8 +450,8 @@ def main():
     ap.add_argument("--workload", default="large", choices=["mesh", "spheres", "large", "room", "batch"])
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--chunk-paths", type=int, default=0)
-    ap.add_argument("--compressed-nodes", action="store_true",
-                    help="8-bit quantised BVH child boxes instead of FP32")
+    ap.add_argument("--full-nodes", action="store_true",
+                    help="full FP32 BVH child boxes instead of the default 8-bit quantised ones")
     ap.add_argument("--no-cache", action="store_true",
                     help="skip the radiance-cache leg (plain vs cached wall-clock to the plain frame's RMSE "
                          "against the FP64 oracle), which runs by default on workloads with an oracle fixture")
@@ -499,7 +499,7 @@ def main():
     scene = sg.WORKLOADS[args.workload]()
     dev = kr.Device(0)
     gs = kr.GpuScene(dev, scene.to_desc(bvh_width=args.bvh_width,
-                                        flags=kr.KR_SCENE_COMPRESSED_NODES if args.compressed_nodes else 0))
+                                        flags=kr.KR_SCENE_FULL_NODES if args.full_nodes else 0))
     info = gs.info()
     log(f"[bench] scene: {info.num_triangles} tris, {info.num_spheres} spheres, {info.num_lights} lights, "
         f"BVH{info.bvh_width} {info.bvh_nodes} nodes depth {info.bvh_depth}, build {info.build_seconds:.2f}s")

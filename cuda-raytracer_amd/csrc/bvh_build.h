@@ -13,7 +13,7 @@ struct WideBvh {
     uint32_t num_nodes = 0, num_leaves = 0, depth = 0;
 };
 
-// Binned-SAH binary build (32 bins, leaves of at most leaf_size triangles)
+// Binned-SAH binary build (32 bins on each axis, leaves of at most leaf_size triangles)
 // collapsed to `width` children per node by repeatedly opening the child of
 // largest surface area. tri_verts holds 9 floats (v0, v1, v2) per triangle.
 bool build_wide_bvh(const float *tri_verts, uint32_t num_tris, uint32_t width, uint32_t leaf_size, WideBvh &out,

@@ -114,6 +114,8 @@ __device__ __forceinline__ ShearRay make_shear(f3 o, f3 d) {
     return r;
 }
 
+// LE: accept t == tmax as well (closest hit breaks exact ties itself).
+template <bool LE = false>
 __device__ __forceinline__ bool tri_hit(const ShearRay &r, f3 v0, f3 v1, f3 v2, float tmax, float &t, float &b1,
                                         float &b2) {
     f3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
@@ -134,7 +136,7 @@ __device__ __forceinline__ bool tri_hit(const ShearRay &r, f3 v0, f3 v1, f3 v2, 
     float T = U * dot(r.mz, A) + V * dot(r.mz, B) + W * dot(r.mz, C);
     float inv = __builtin_amdgcn_rcpf(det);  // v_rcp_f32, 1 ulp
     float tt = T * inv;
-    if (!(tt > 0.f && tt < tmax)) return false;
+    if (!(tt > 0.f && (LE ? tt <= tmax : tt < tmax))) return false;
     t = tt;
     b1 = V * inv;
     b2 = W * inv;
@@ -202,6 +204,16 @@ struct Stack {
             ref = e.x;
             t = __uint_as_float(e.y);
         }
+    }
+    // entries on the stack
+    __device__ __forceinline__ uint32_t depth() const { return off / kRow; }
+    // entry k counted from the bottom (k < depth()), as (ref, entry distance bits)
+    __device__ __forceinline__ uint2 entry(uint32_t k) const {
+        uint32_t o = k * kRow + (off & (kRow - 1u));
+        if (o < kLds)
+            return make_uint2(*reinterpret_cast<const uint32_t *>(lds + o),
+                              *reinterpret_cast<const uint32_t *>(lds + kLds + o));
+        return spill[size_t(k - LDS_DEPTH) * gstride + (off & (kRow - 1u)) / 4u];
     }
 };
 
@@ -367,6 +379,65 @@ __device__ __forceinline__ void load_node_nf(const float4 *nodes, uint32_t ref, 
     for (int c = 0; c < W; ++c) nv.ref[c] = r[c];
 }
 
+// Entry distances of the children of a compressed node (W = 4 or 8): key[c]
+// = the distance at which the ray enters child c before tmax, +inf for a
+// missed or empty child. The decode is folded into the slab test: along axis
+// a a plane is p = q s_a + o_node (s_a a power of two), so its distance is
+//   (p - o) inv_a = q (s_a inv_a) + (o_node inv_a - o inv_a),
+// one conversion of the byte and one fma per plane, with s_a inv_a exact and
+// the second term computed once per node and axis. As for the FP32 nodes, the
+// ray's direction signs pick each axis' entering and leaving planes (whole
+// words of W bytes), and the leaving distance is widened by 1 + 2^-22.
+template <int W>
+__device__ __forceinline__ void qnode_enter(const float4 *nodes, uint32_t ref, const RayBox &rb, float tmax,
+                                            float (&key)[W], uint32_t (&cref)[W]) {
+    static_assert(W == 4 || W == 8, "byte planes of whole words");
+    constexpr int NU4 = int(qnode_bytes(W)) / 16;
+    constexpr int QW = 4 + W;  // first word of the planes
+    constexpr int AW = W / 4;  // words per plane array
+    const uint4 *p = reinterpret_cast<const uint4 *>(nodes) + ref;
+    uint32_t u[NU4 * 4];
+#pragma unroll
+    for (int i = 0; i < NU4; ++i) {
+        uint4 q = p[i];
+        u[4 * i + 0] = q.x;
+        u[4 * i + 1] = q.y;
+        u[4 * i + 2] = q.z;
+        u[4 * i + 3] = q.w;
+    }
+    const float inv[3] = {rb.inv.x, rb.inv.y, rb.inv.z}, oinv[3] = {rb.oinv.x, rb.oinv.y, rb.oinv.z};
+    float A[3], B[3];
+    uint32_t nw[3][AW], fw[3][AW];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int e = int(int8_t(uint8_t(u[3] >> (8 * a))));
+        A[a] = __uint_as_float(uint32_t(e + 127) << 23) * inv[a];
+        B[a] = fmaf(__uint_as_float(u[a]), inv[a], -oinv[a]);
+        const bool neg = __float_as_uint(inv[a]) >> 31;
+#pragma unroll
+        for (int w = 0; w < AW; ++w) {
+            const uint32_t lo = u[QW + 2 * a * AW + w], hi = u[QW + (2 * a + 1) * AW + w];
+            nw[a][w] = neg ? hi : lo;
+            fw[a][w] = neg ? lo : hi;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        float tn3[3], tf3[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float qn = float((nw[a][c / 4] >> (8 * (c % 4))) & 0xFFu);
+            const float qf = float((fw[a][c / 4] >> (8 * (c % 4))) & 0xFFu);
+            tn3[a] = fmaf(qn, A[a], B[a]);
+            tf3[a] = fmaf(qf, A[a], B[a]);
+        }
+        float tn = fmaxf(fmaxf(tn3[0], tn3[1]), fmaxf(tn3[2], 0.f));
+        float tf = fminf(fminf(tf3[0], tf3[1]), fminf(tf3[2], tmax)) * 1.00000024f;
+        cref[c] = u[4 + c];
+        key[c] = (tn <= tf && cref[c] != kEmpty) ? tn : __builtin_inff();
+    }
+}
+
 template <int W, bool Q>
 __device__ __forceinline__ void fetch_node(const float4 *nodes, uint32_t ref, NodeView<W> &nv) {
     if (Q)
@@ -426,19 +497,20 @@ __device__ __forceinline__ HitRec closest_spheres(const DevScene &S, f3 o, f3 d)
     return h;
 }
 
-// One inner node: returns the nearest child whose box the ray enters before
-// tmax (kEmpty if none) and pushes the other entered children, farthest
-// first, with their entry distances.
-template <int W, int LDS_DEPTH, bool Q>
-__device__ __forceinline__ uint32_t closest_node(const DevScene &S, const RayBox &rb, uint32_t node, float tmax,
-                                                 Stack<LDS_DEPTH> &st) {
+// The children of inner node `node`: key[c] = the distance at which the ray
+// enters child c before tmax, +inf for a missed or empty child; ref[c] its ref.
+template <int W, bool Q>
+__device__ __forceinline__ void node_keys(const float4 *nodes, uint32_t node, const RayBox &rb, float tmax,
+                                          float (&key)[W], uint32_t (&ref)[W]) {
+    if constexpr (Q && W != 2) {
+        qnode_enter<W>(nodes, node, rb, tmax, key, ref);
+        return;
+    }
     NodeView<W> nv;
     if (Q)
-        fetch_node<W, Q>(S.nodes, node, nv);
+        fetch_node<W, Q>(nodes, node, nv);
     else
-        load_node_nf<W>(S.nodes, node, rb, nv);
-    float key[W];
-    uint32_t ref[W];
+        load_node_nf<W>(nodes, node, rb, nv);
 #pragma unroll
     for (int c = 0; c < W; ++c) {
         if (Q) {
@@ -455,6 +527,17 @@ __device__ __forceinline__ uint32_t closest_node(const DevScene &S, const RayBox
         }
         ref[c] = nv.ref[c];
     }
+}
+
+// One inner node: returns the nearest child whose box the ray enters before
+// tmax (kEmpty if none) and pushes the other entered children, farthest
+// first, with their entry distances.
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ uint32_t closest_node(const DevScene &S, const RayBox &rb, uint32_t node, float tmax,
+                                                 Stack<LDS_DEPTH> &st) {
+    float key[W];
+    uint32_t ref[W];
+    node_keys<W, Q>(S.nodes, node, rb, tmax, key, ref);
     sort_children<W>(key, ref);
     // missed children sort last, with key +inf
 #pragma unroll
@@ -463,7 +546,12 @@ __device__ __forceinline__ uint32_t closest_node(const DevScene &S, const RayBox
     return key[0] != __builtin_inff() ? ref[0] : kEmpty;
 }
 
-// One leaf: the triangles of `leaf` against the closest hit so far.
+// One leaf: the triangles of `leaf` against the closest hit so far. Hits at
+// exactly the same t (coplanar, overlapping triangles, such as two emitters
+// in one plane) go to the lowest global triangle index, i.e. to the first
+// primitive in scene order, as the reference's loop over primitives with a
+// strict `<` decides them (src/core/meta/scene.hpp:39-46), whatever order
+// the tree visits the leaves in.
 __device__ __forceinline__ uint32_t closest_leaf(const DevScene &S, const ShearRay &sr, uint32_t leaf, HitRec &h) {
     uint32_t first = leaf & kLeafFirstMask;
     uint32_t cnt = ((leaf >> kLeafCountShift) & 0xFu) + 1u;
@@ -471,11 +559,13 @@ __device__ __forceinline__ uint32_t closest_leaf(const DevScene &S, const ShearR
         const float4 *tp = S.tris + 3 * size_t(first + k);
         float4 a = tp[0], b = tp[1], c = tp[2];
         float tt, b1, b2;
-        if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, h.t, tt, b1, b2)) {
+        const uint32_t code = __float_as_uint(b.w);
+        if (tri_hit<true>(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, h.t, tt, b1, b2) &&
+            (tt < h.t || code < h.code)) {
             h.t = tt;
             h.u = b1;
             h.v = b2;
-            h.code = __float_as_uint(b.w);
+            h.code = code;
             h.mat = __float_as_uint(c.w);
         }
     }
@@ -526,6 +616,30 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
         counts[1] = ntri;
     }
     return h;
+}
+
+// trace_closest's walk resumed from a state (the next ref `cur`, the hit so
+// far `h` and the stack), for at most `steps` steps of one node or one leaf;
+// true when the walk is over. A walk run in such pieces visits the same nodes
+// and leaves in the same order as one uninterrupted walk, so its hit is the
+// same. Visited nodes and tested triangles are added to nvis and ntri.
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ bool closest_walk(const DevScene &S, const RayBox &rb, const ShearRay &sr,
+                                             Stack<LDS_DEPTH> &st, uint32_t &cur, HitRec &h, uint32_t steps,
+                                             uint32_t &nvis, uint32_t &ntri) {
+    for (uint32_t k = 0; k < steps; ++k) {
+        bool pop = true;
+        if (!(cur & kLeafBit)) {
+            ++nvis;
+            uint32_t next = closest_node<W, LDS_DEPTH, Q>(S, rb, cur, h.t, st);
+            pop = next == kEmpty;
+            if (!pop) cur = next;
+        } else {
+            ntri += closest_leaf(S, sr, cur, h);
+        }
+        if (pop && !closest_pop(st, h.t, cur)) return true;
+    }
+    return false;
 }
 
 // Any hit on the segment (o, d, tmax): Scene::intersect(Seg)
@@ -633,26 +747,13 @@ __device__ __forceinline__ bool trace_occluded_steps(const DevScene &S, f3 o, f3
     while (true) {
         bool pop = true;
         if (!(cur & kLeafBit)) {
-            NodeView<W> nv;
-            if (Q)
-                fetch_node<W, Q>(S.nodes, cur, nv);
-            else
-                load_node_nf<W>(S.nodes, cur, rb, nv);
             // sort key: minus the entry distance (farthest first), +inf for a
             // child the segment misses (last)
             float key[W];
             uint32_t ref[W];
+            node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
 #pragma unroll
-            for (int c = 0; c < W; ++c) {
-                float tn = Q ? box_enter(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
-                                         nv.hi[2][c], limit)
-                             : box_enter_nf(rb, nv.lo[0][c], nv.hi[0][c], nv.lo[1][c], nv.hi[1][c], nv.lo[2][c],
-                                            nv.hi[2][c], limit);
-                // (box_enter_nf never enters an empty slot's (+inf, -inf) box)
-                bool in = (Q ? nv.ref[c] != kEmpty : true) && tn != __builtin_inff();
-                key[c] = in ? -tn : __builtin_inff();
-                ref[c] = nv.ref[c];
-            }
+            for (int c = 0; c < W; ++c) key[c] = key[c] != __builtin_inff() ? -key[c] : __builtin_inff();
             sort_children<W>(key, ref);
 #pragma unroll
             for (int c = W - 1; c >= 1; --c)
@@ -678,6 +779,51 @@ __device__ __forceinline__ bool trace_occluded_steps(const DevScene &S, f3 o, f3
         }
     }
     return false;
+}
+
+// trace_occluded_steps' walk over the triangles resumed from a state (the
+// next ref `cur` and the stack), for at most `steps` steps: 1 when a
+// triangle blocks the segment, 0 when the walk ends without one, -1 when the
+// steps ran out first. Pieces of the walk visit what the whole walk visits,
+// in the same order. (The spheres are tested once, before the walk.)
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ int occluded_walk(const DevScene &S, const RayBox &rb, const ShearRay &sr, float limit,
+                                             float tmax, Stack<LDS_DEPTH> &st, uint32_t &cur, uint32_t steps) {
+    for (uint32_t s = 0; s < steps; ++s) {
+        bool pop = true;
+        if (!(cur & kLeafBit)) {
+            // sort key: minus the entry distance (farthest first), +inf for a
+            // child the segment misses (last)
+            float key[W];
+            uint32_t ref[W];
+            node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
+#pragma unroll
+            for (int c = 0; c < W; ++c) key[c] = key[c] != __builtin_inff() ? -key[c] : __builtin_inff();
+            sort_children<W>(key, ref);
+#pragma unroll
+            for (int c = W - 1; c >= 1; --c)
+                if (key[c] != __builtin_inff()) st.push(ref[c], -key[c]);
+            pop = key[0] == __builtin_inff();
+            if (!pop) cur = ref[0];
+        } else {
+            uint32_t first = cur & kLeafFirstMask;
+            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                const float4 *tp = S.tris + 3 * size_t(first + k);
+                float4 a = tp[0], b = tp[1], c = tp[2];
+                float tt, b1, b2;
+                if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, limit, tt, b1, b2) &&
+                    tt <= tmax)
+                    return 1;
+            }
+        }
+        if (pop) {
+            if (st.empty()) return 0;
+            float tn;
+            st.pop(cur, tn);
+        }
+    }
+    return -1;
 }
 
 // ------------------------------------------------ stepwise traversal --

@@ -30,8 +30,8 @@ constexpr uint32_t kMissCode = 0xFFFFFFFFu;
 
 constexpr uint32_t node_floats(uint32_t w) { return (7 * w + 3) / 4 * 4; }
 
-// Compressed wide node (KR_SCENE_COMPRESSED_NODES, opt-in; by default nodes
-// hold full FP32 boxes, as above): child boxes quantised to 8 bits per
+// Compressed wide node (the default; KR_SCENE_FULL_NODES selects the full
+// FP32 boxes above): child boxes quantised to 8 bits per
 // plane against the node's own box, as in Ylitie, Karras and Laine, "Efficient
 // Incoherent Ray Traversal on GPUs Through Compressed Wide BVHs" (HPG 2017):
 //   word 0..2  origin p (float xyz; the node box's lower corner)
@@ -41,7 +41,8 @@ constexpr uint32_t node_floats(uint32_t w) { return (7 * w + 3) / 4 * 4; }
 // A child's box is lo = fma(q_lo, s, p), hi = fma(q_hi, s, p) in FP32; the
 // encoder rounds q_lo down and q_hi up until the decoded box contains the
 // exact one, so traversal only ever visits a superset of nodes. 64 B for
-// W = 4 (128 B uncompressed), 96 B for W = 8 (224 B).
+// W = 4 (112 B uncompressed), 96 B for W = 8 (224 B). The W = 4 and W = 8
+// kernels fold the decode into the slab test (qnode_enter in kr_device.h).
 constexpr uint32_t qnode_bytes(uint32_t w) { return (16 + 4 * w + 6 * w + 15) / 16 * 16; }
 
 inline uint32_t make_leaf(uint32_t first, uint32_t count) {

@@ -77,10 +77,10 @@ constexpr int kBlock = 256;
 #define KR_SHADE_WAVES 1
 #endif
 constexpr int kLdsDepth = KR_LDS_DEPTH;
-// The wave hint applies to the plain BVH2 / BVH4 traversals; the BVH8,
-// batch, compressed-node and persistent variants need more registers and
-// would spill.
-constexpr int trace_waves(int w, bool batch, bool q) { return (w <= 4 && !batch && !q) ? KR_TRACE_WAVES : 1; }
+// The wave hint applies to the BVH2 / BVH4 traversals of single renders
+// (full or compressed nodes); the BVH8 and job-mixing batch variants need
+// more registers and would spill.
+constexpr int trace_waves(int w, bool batch, bool) { return (w <= 4 && !batch) ? KR_TRACE_WAVES : 1; }
 constexpr int kLdsDepthMin = kLdsDepth;
 // traversal stack entries per lane beyond the LDS part, in HBM (108 in all:
 // BVH8 trees to depth 14, BVH4 to depth 34)
@@ -275,6 +275,18 @@ struct HitQ {
     float *t, *u, *v;
     uint32_t *code;
 };
+// Rays or segments whose traversal used up its step budget in a launch, with
+// what the next launch needs to resume it (SoA, `cap` entries): the ray's
+// index in its queue, the next node or leaf ref, the stack depth (and, for a
+// closest hit, the material bucket of the hit so far in bits 16-17; the hit
+// itself waits in the hit queue at the ray's index) and the stack entries,
+// entry k of slot j at stk[k * cap + j].
+struct TailQ {
+    uint32_t *idx, *cur, *info;
+    uint2 *stk;
+    uint32_t cap;      // slots
+    uint32_t stk_cap;  // stack entries per slot: a walk with a deeper stack finishes in its launch
+};
 struct LensDev {
     float n[3], u[3], v[3], du[3], dv[3], o[3];
     uint32_t lens, pw, w, h;
@@ -330,8 +342,12 @@ enum : int {
     C_TRAIN = 7,  // training paths
     C_FETCH = 8,  // persistent closest-hit traversal: next ray to fetch
     C_FETCH_SH = 9,  // persistent shadow traversal: next segment to fetch
-    C_NUM = 10
+    C_TAIL = 10,     // closest-hit walks handed to tail round r: C_TAIL + r (kMaxTailRounds + 1 entries)
+    C_TAIL_SH = 15,  // the same for shadow segments
+    C_NUM = 20
 };
+constexpr int kMaxTailRounds = 4;
+static_assert(C_TAIL + kMaxTailRounds < C_TAIL_SH && C_TAIL_SH + kMaxTailRounds < C_NUM, "tail counters");
 // Each counter sits on its own 128-byte line, so that the wave-aggregated
 // atomics of different queues (fetch, buckets, shadow, next rays) do not
 // serialise on one L2 channel.
@@ -388,6 +404,7 @@ struct Pipe {
     hipStream_t stream = nullptr;  // pipe 0: the device's stream
     bool own_stream = false;
     hipEvent_t acc_done = nullptr;  // recorded after each of the pipe's accumulations
+    TailQ tail[2]{};                // walks handed between tail rounds (cap 0: no tail rounds)
     NrcRender nr{};                 // radiance-cache buffers of the pipe's chunks
     double *acc = nullptr;          // the image the pipe accumulates into: the render's
     uint32_t *valid = nullptr;      // (shared, ordered by acc_done) or, in a batch, its own
@@ -407,6 +424,10 @@ struct kr_render {
     double *acc = nullptr;
     uint32_t *valid = nullptr;
     int grid = 0;
+    // tail rounds (k_trace_tail, k_shadow_tail): step budget of a launch,
+    // follow-up launches per queue (the last one finishes every walk), first
+    // bounce whose closest-hit walks use them (shadow segments: every bounce)
+    uint32_t tail_steps = 0, tail_rounds = 0, tail_from = 1;
     std::vector<void *> allocs;
     double last_seconds = 0;
     uint64_t paths = 0;
@@ -727,6 +748,102 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace(DevS
             mat = (h.code & kSphereBit) ? min(Sx.shaders[Sx.prims[Sx.sphere_prim[h.code & ~kSphereBit]].shader].type, 2u)
                                         : h.mat;
         }
+        uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
+        if (hitp && mat == 0) b0[p0] = i;
+        if (sort) {
+            uint32_t p1 = wave_append(&counters[CTR(C_BUCKET + 1)], hitp && mat == 1);
+            if (hitp && mat == 1) b1[p1] = i;
+            uint32_t p2 = wave_append(&counters[CTR(C_BUCKET + 2)], hitp && mat == 2);
+            if (hitp && mat == 2) b2[p2] = i;
+        }
+    }
+}
+
+// Closest-hit traversal in rounds ("tail rounds"). A wave of incoherent rays
+// runs until its slowest ray is done: on the 1M-triangle frame a bounce ray
+// takes 17 steps (one node or one leaf each) at the median and the slowest of
+// 64 about 48, so most lanes idle through the end of every wave. Here a walk
+// that has taken `steps` steps leaves its state in the tail queue `tout` and
+// the lane moves on; the next launch (FIRST = false) resumes the queued walks
+// packed into full waves from `tin`. A walk whose stack is deeper than the
+// queue keeps, or that finds the queue full, finishes in its launch, and so
+// does every walk of a launch with steps = ~0u. Walks are resumed exactly
+// where they stopped, so hits are those of k_trace; only the order of the
+// bucket entries differs.
+template <int W, bool COUNT, bool Q, bool FIRST>
+__global__ __launch_bounds__(kBlock, trace_waves(W, false, Q)) void k_trace_tail(
+    DevScene S, RayQ q, int qi, HitQ hit, uint32_t *b0, uint32_t *b1, uint32_t *b2, uint32_t *counters,
+    unsigned long long *stats, int sort, uint32_t *spill, size_t spill_lanes, TailQ tin, int cin, TailQ tout,
+    int cout, uint32_t steps) {
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
+    const uint32_t n = FIRST ? counters[CTR(qi)] : min(counters[CTR(cin)], tin.cap);
+    if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_CLOSEST], (unsigned long long)n);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t j = base + threadIdx.x;
+        const bool active = j < n;
+        uint32_t i = 0;
+        HitRec h;
+        h.code = kMissCode;
+        h.mat = 0;
+        bool done = false;
+        if (active) {
+            i = FIRST ? j : tin.idx[j];
+            f3 o{q.ox[i], q.oy[i], q.oz[i]}, d{q.dx[i], q.dy[i], q.dz[i]};
+            RayBox rb = make_raybox(o, d);
+            ShearRay sr = make_shear(o, d);
+            uint32_t cur = 0;
+            st.reset();
+            if (FIRST) {
+                h = closest_spheres(S, o, d);
+                done = S.ntris == 0;
+            } else {
+                h.t = hit.t[i];
+                h.u = hit.u[i];
+                h.v = hit.v[i];
+                h.code = hit.code[i];
+                cur = tin.cur[j];
+                const uint32_t info = tin.info[j];
+                h.mat = info >> 16;
+                const uint32_t dep = info & 0xFFFFu;
+                for (uint32_t k = 0; k < dep; ++k) {
+                    uint2 e = tin.stk[size_t(k) * tin.cap + j];
+                    st.push(e.x, __uint_as_float(e.y));
+                }
+            }
+            uint32_t nvis = 0, ntri = 0, budget = steps;
+            while (!done) {
+                done = closest_walk<W, kLdsDepth, Q>(S, rb, sr, st, cur, h, budget, nvis, ntri);
+                if (done) break;
+                // out of steps: queue the walk for the next round if its stack fits
+                const uint32_t dep = st.depth();
+                const bool fits = dep <= tout.stk_cap;
+                const uint32_t pos = wave_append(&counters[CTR(cout)], fits);
+                if (fits && pos < tout.cap) {
+                    tout.idx[pos] = i;
+                    tout.cur[pos] = cur;
+                    tout.info[pos] = dep | (h.mat << 16);
+                    for (uint32_t k = 0; k < dep; ++k) tout.stk[size_t(k) * tout.cap + pos] = st.entry(k);
+                    break;
+                }
+                budget = ~0u;
+            }
+            if (COUNT) {
+                atomicAdd(&stats[S_NODES], (unsigned long long)nvis);
+                atomicAdd(&stats[S_TRIS], (unsigned long long)ntri);
+            }
+            // the hit so far (a queued walk resumes from it) or the final one
+            hit.t[i] = h.t;
+            hit.u[i] = h.u;
+            hit.v[i] = h.v;
+            hit.code[i] = h.code;
+        }
+        const bool hitp = active && done && h.code != kMissCode;
+        uint32_t mat = 0;
+        if (hitp && sort)
+            mat = (h.code & kSphereBit) ? min(S.shaders[S.prims[S.sphere_prim[h.code & ~kSphereBit]].shader].type, 2u)
+                                        : h.mat;
         uint32_t p0 = wave_append(&counters[CTR(C_BUCKET + 0)], hitp && mat == 0);
         if (hitp && mat == 0) b0[p0] = i;
         if (sort) {
@@ -1233,6 +1350,26 @@ __global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
     }
 }
 
+// An unoccluded light sample: its contribution goes into the path's L (and
+// into the vertex record of a training path and the profiler's record).
+__device__ __forceinline__ void shadow_unoccluded(const ShadowQ &sq, uint32_t i, uint32_t slot, float *L,
+                                                  const NrcRender &nr, uint32_t bounce, const ProfView &pv,
+                                                  uint64_t g0) {
+    L[3 * slot] += sq.cx[i];
+    L[3 * slot + 1] += sq.cy[i];
+    L[3 * slot + 2] += sq.cz[i];
+    if (nr.enabled) {
+        uint32_t ts = nr.tslot_of[slot];
+        if (ts != kMissCode) {
+            float *rec = nr.rec + (size_t(ts) * nr.nv + bounce) * kRecFloats;
+            rec[19] += sq.cx[i];
+            rec[20] += sq.cy[i];
+            rec[21] += sq.cz[i];
+        }
+    }
+    if (pv.enabled) prof_add_light(pv, g0 + slot, bounce, sq.cx[i], sq.cy[i], sq.cz[i]);
+}
+
 template <int W, bool BATCH, bool Q>
 __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(DevScene S, JobTable T, uint64_t g0, ShadowQ sq, uint32_t *counters,
                                                    float *L, unsigned long long *stats, uint32_t *spill,
@@ -1253,22 +1390,69 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(Dev
             if (BATCH) Sx = T.rec[find_job(T, g0 + slot)].S;
             bool occ = KR_SHADOW_STEPS ? trace_occluded_steps<W, kLdsDepth, Q>(Sx, o, d, sq.tmax[i], st)
                                        : trace_occluded<W, kLdsDepth, Q>(Sx, o, d, sq.tmax[i], st);
-            if (!occ) {
-                L[3 * slot] += sq.cx[i];
-                L[3 * slot + 1] += sq.cy[i];
-                L[3 * slot + 2] += sq.cz[i];
-                if (nr.enabled) {
-                    uint32_t ts = nr.tslot_of[slot];
-                    if (ts != kMissCode) {
-                        float *rec = nr.rec + (size_t(ts) * nr.nv + bounce) * kRecFloats;
-                        rec[19] += sq.cx[i];
-                        rec[20] += sq.cy[i];
-                        rec[21] += sq.cz[i];
-                    }
-                }
-                if (pv.enabled) prof_add_light(pv, g0 + slot, bounce, sq.cx[i], sq.cy[i], sq.cz[i]);
+            if (!occ) shadow_unoccluded(sq, i, slot, L, nr, bounce, pv, g0);
+        }
+    }
+}
+
+// Shadow segments in tail rounds, as k_trace_tail does for closest hits: a
+// walk that has taken `steps` steps is queued in `tout` with its state, and
+// the next launch (FIRST = false) resumes the queued walks packed into full
+// waves. Single renders and one-job batch chunks only.
+template <int W, bool Q, bool FIRST>
+__global__ __launch_bounds__(kBlock, trace_waves(W, false, Q)) void k_shadow_tail(
+    DevScene S, uint64_t g0, ShadowQ sq, uint32_t *counters, float *L, unsigned long long *stats, uint32_t *spill,
+    size_t spill_lanes, NrcRender nr, uint32_t bounce, ProfView pv, TailQ tin, int cin, TailQ tout, int cout,
+    uint32_t steps) {
+    __shared__ uint32_t stk[2 * kLdsDepth * kBlock];
+    auto st = make_stack<kLdsDepth>(stk, spill, spill_lanes);
+    const uint32_t n = FIRST ? counters[CTR(C_SHADOW)] : min(counters[CTR(cin)], tin.cap);
+    if (FIRST && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_SHADOW], (unsigned long long)n);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t j = base + threadIdx.x;
+        if (j >= n) continue;
+        const uint32_t i = FIRST ? j : tin.idx[j];
+        f3 o{sq.ox[i], sq.oy[i], sq.oz[i]}, d{sq.dx[i], sq.dy[i], sq.dz[i]};
+        const float tmax = sq.tmax[i];
+        RayBox rb = make_raybox(o, d);
+        ShearRay sr = make_shear(o, d);
+        // tri_hit accepts t < limit; the segment includes its end point
+        const float limit = tmax * 1.00000012f;
+        uint32_t cur = 0;
+        int occ = -1;
+        st.reset();
+        if (FIRST) {
+            for (uint32_t s = 0; s < S.nspheres && occ < 0; ++s) {
+                double ts;
+                if (sphere_hit(S.spheres + 4 * s, o, d, ts) && ts <= double(tmax)) occ = 1;
+            }
+            if (occ < 0 && S.ntris == 0) occ = 0;
+        } else {
+            cur = tin.cur[j];
+            const uint32_t dep = tin.info[j];
+            for (uint32_t k = 0; k < dep; ++k) {
+                uint2 e = tin.stk[size_t(k) * tin.cap + j];
+                st.push(e.x, __uint_as_float(e.y));
             }
         }
+        uint32_t budget = steps;
+        while (occ < 0) {
+            occ = occluded_walk<W, kLdsDepth, Q>(S, rb, sr, limit, tmax, st, cur, budget);
+            if (occ >= 0) break;
+            const uint32_t dep = st.depth();
+            const bool fits = dep <= tout.stk_cap;
+            const uint32_t pos = wave_append(&counters[CTR(cout)], fits);
+            if (fits && pos < tout.cap) {
+                tout.idx[pos] = i;
+                tout.cur[pos] = cur;
+                tout.info[pos] = dep;
+                for (uint32_t k = 0; k < dep; ++k) tout.stk[size_t(k) * tout.cap + pos] = st.entry(k);
+                break;
+            }
+            budget = ~0u;
+        }
+        if (occ == 0) shadow_unoccluded(sq, i, sq.slot[i], L, nr, bounce, pv, g0);
     }
 }
 
@@ -2168,7 +2352,7 @@ KR_EXPORT int kr_scene_upload(kr_device *dev, const kr_scene_desc *d, kr_scene *
     (void)hipSetDevice(dev->ordinal);
     DevScene &v = s->view;
     int rc = KR_OK;
-    s->qnodes = (d->flags & KR_SCENE_COMPRESSED_NODES) != 0;
+    s->qnodes = (d->flags & KR_SCENE_FULL_NODES) == 0;
     // the kernels address nodes with 32-bit byte offsets
     const uint32_t node_bytes = s->qnodes ? qnode_bytes(width) : node_floats(width) * 4;
     if (uint64_t(bvh.num_nodes) * node_bytes >= (uint64_t(1) << 32)) {
@@ -2502,6 +2686,29 @@ int check_view(const kr_camera *cam, const kr_params *prm) {
     return KR_OK;
 }
 
+// Tail rounds (k_trace_tail): a launch's step budget, the follow-up launches
+// per queue and the stack entries a queued walk keeps; KR_TAIL_STEPS,
+// KR_TAIL_ROUNDS, KR_TAIL_STACK and KR_TAIL_FROM override them.
+uint32_t env_u32(const char *name, uint32_t dflt) {
+    const char *e = std::getenv(name);
+    return e && *e ? uint32_t(std::strtoul(e, nullptr, 10)) : dflt;
+}
+#ifndef KR_TAIL_STEPS
+#define KR_TAIL_STEPS 28
+#endif
+#ifndef KR_TAIL_ROUNDS
+#define KR_TAIL_ROUNDS 2
+#endif
+uint32_t tail_stack_entries() { return std::max(1u, std::min(64u, env_u32("KR_TAIL_STACK", 12))); }
+void tail_config(kr_render *r) {
+    r->tail_steps = (r->prm.flags & (KR_RENDER_NO_TAIL_ROUNDS | KR_RENDER_PERSISTENT_TRAVERSAL))
+                        ? 0u
+                        : env_u32("KR_TAIL_STEPS", KR_TAIL_STEPS);
+    r->tail_rounds = std::min<uint32_t>(env_u32("KR_TAIL_ROUNDS", KR_TAIL_ROUNDS), kMaxTailRounds);
+    r->tail_from = env_u32("KR_TAIL_FROM", 1);
+    if (r->tail_rounds == 0) r->tail_steps = 0;
+}
+
 // Queues for r->P paths per pipe (r->npipes of them), accumulators for
 // r->npix pixels, counters, the traversal spill stack for `lanes` lanes per
 // pipe (spill_entries per lane, 0 = the most any scene may need); r->grid =
@@ -2540,6 +2747,19 @@ int render_buffers(kr_render *r, size_t lanes, uint32_t spill_entries) {
         if (!rc) rc = get(&pp.L, 3 * P);
         if (!rc) rc = get(&pp.counters, size_t(C_NUM) * kCtrStride);
         if (!rc) rc = ensure_spill(pp.spill, pp.spill_lanes, r->allocs, r->scene->dev, lanes, spill_entries);
+        if (r->tail_steps) {
+            // room for half the queue per round; walks beyond it finish in place
+            const uint32_t cap = uint32_t((P + 1) / 2), scap = tail_stack_entries();
+            for (int k = 0; k < 2 && !rc; ++k) {
+                TailQ &t = pp.tail[k];
+                t.cap = cap;
+                t.stk_cap = scap;
+                rc = get(&t.idx, cap);
+                if (!rc) rc = get(&t.cur, cap);
+                if (!rc) rc = get(&t.info, cap);
+                if (!rc) rc = get(&t.stk, size_t(cap) * scap);
+            }
+        }
         if (!rc) {
             if (pi == 0) {
                 pp.stream = r->scene->dev->stream;
@@ -2597,6 +2817,7 @@ KR_EXPORT int kr_render_create(kr_scene *s, const kr_camera *cam, const kr_param
         uint64_t m = (per + P - 1) / P;
         r->P = uint32_t((per + m - 1) / m);
     }
+    tail_config(r);
     rc = render_buffers(r, 0, spill_depth(s->view.stack_cap));
     if (rc) {
         kr_render_free(r);
@@ -2812,6 +3033,7 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
     }
     int sort = (r->prm.flags & KR_RENDER_NO_MATERIAL_SORT) ? 0 : 1;
     const bool dyn = (r->prm.flags & KR_RENDER_PERSISTENT_TRAVERSAL) != 0;
+    const bool tails = r->tail_steps != 0 && pp.tail[0].cap != 0;
     const uint32_t kern = r->prm.kernel;
     const uint32_t rounds = kern == KR_KERNEL_RADIANCE ? max_bounce + 1 : 1;
     for (uint32_t b = 0; b < rounds; ++b) {
@@ -2820,7 +3042,8 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
         {
             // clear buckets, shadow count and the next ray queue's count
             Timed tm(r, K_OTHER, st);
-            uint32_t mask = (0x7u << C_BUCKET) | (1u << C_SHADOW) | (1u << qni) | (1u << C_FETCH) | (1u << C_FETCH_SH);
+            uint32_t mask = (0x7u << C_BUCKET) | (1u << C_SHADOW) | (1u << qni) | (1u << C_FETCH) | (1u << C_FETCH_SH) |
+                            (0x1Fu << C_TAIL) | (0x1Fu << C_TAIL_SH);
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, pp.counters, mask);
         }
         {
@@ -2834,10 +3057,27 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
                 hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, S, pp.q[cur], qi, pp.hit, pp.bucket[0],
                                    pp.bucket[1], pp.bucket[2], pp.counters, r->stats, sort, pp.spill, pp.spill_lanes);
             };
-            if (!BATCH && dyn)
+            // tail rounds: the first launch walks every ray for up to tail_steps
+            // steps, each follow-up resumes the walks queued by the previous
+            // one, the last without a step limit
+            auto trace_tail = [&](auto kern, int round) {
+                const TailQ &tin = pp.tail[(round + 1) & 1], &tout = pp.tail[round & 1];
+                uint32_t steps = round < int(r->tail_rounds) ? r->tail_steps : ~0u;
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, S, pp.q[cur], qi, pp.hit, pp.bucket[0],
+                                   pp.bucket[1], pp.bucket[2], pp.counters, r->stats, sort, pp.spill,
+                                   pp.spill_lanes, tin, C_TAIL + std::max(round - 1, 0), tout, C_TAIL + round,
+                                   steps);
+            };
+            if (!BATCH && dyn) {
                 count ? trace_refill(k_trace_refill<W, true, Q>) : trace_refill(k_trace_refill<W, false, Q>);
-            else
+            } else if (!BATCH && tails && b >= r->tail_from) {
+                count ? trace_tail(k_trace_tail<W, true, Q, true>, 0) : trace_tail(k_trace_tail<W, false, Q, true>, 0);
+                for (int rd = 1; rd <= int(r->tail_rounds); ++rd)
+                    count ? trace_tail(k_trace_tail<W, true, Q, false>, rd)
+                          : trace_tail(k_trace_tail<W, false, Q, false>, rd);
+            } else {
                 count ? trace(k_trace<W, true, BATCH, Q>) : trace(k_trace<W, false, BATCH, Q>);
+            }
         }
         ShadeArgs a;
         a.S = S;
@@ -2876,10 +3116,20 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
         }
         {
             Timed tm(r, K_SHADOW, st);
-            if (dyn)
+            auto shadow_tail = [&](auto kern, int round) {
+                const TailQ &tin = pp.tail[(round + 1) & 1], &tout = pp.tail[round & 1];
+                uint32_t steps = round < int(r->tail_rounds) ? r->tail_steps : ~0u;
+                hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, st, S, g0, pp.sq, pp.counters, pp.L, r->stats,
+                                   pp.spill, pp.spill_lanes, nrv, b, pv, tin, C_TAIL_SH + std::max(round - 1, 0),
+                                   tout, C_TAIL_SH + round, steps);
+            };
+            if (dyn) {
                 hipLaunchKernelGGL((k_shadow_dyn<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, pp.sq,
                                    pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, nrv, b, pv);
-            else
+            } else if (!BATCH && tails) {
+                shadow_tail(k_shadow_tail<W, Q, true>, 0);
+                for (int rd = 1; rd <= int(r->tail_rounds); ++rd) shadow_tail(k_shadow_tail<W, Q, false>, rd);
+            } else
                 hipLaunchKernelGGL((k_shadow<W, BATCH, Q>), dim3(grid), dim3(kBlock), 0, st, S, T, g0, pp.sq,
                                    pp.counters, pp.L, r->stats, pp.spill, pp.spill_lanes, nrv, b, pv);
         }
@@ -3740,7 +3990,8 @@ int batch_set_width(kr_batch *b, uint32_t width) {
         kr_scene_desc d = b->scenes[i].desc;
         d.bvh_width = width;
         // one node format for all scenes: the kernels are compiled per format
-        d.flags = (d.flags & ~KR_SCENE_COMPRESSED_NODES) | (b->scenes[0].desc.flags & KR_SCENE_COMPRESSED_NODES);
+        const uint32_t node_fmt = KR_SCENE_COMPRESSED_NODES | KR_SCENE_FULL_NODES;
+        d.flags = (d.flags & ~node_fmt) | (b->scenes[0].desc.flags & node_fmt);
         int rc = kr_scene_upload(b->dev, &d, &b->up[i]);
         if (rc) {
             // leave no half-uploaded set behind: the next call re-uploads all
@@ -3992,6 +4243,7 @@ KR_EXPORT int kr_batch_create(kr_device *dev, const kr_scene_desc *scenes, uint3
         r->npix = uint32_t(pix);
         r->P = uint32_t(std::max<uint64_t>(1, std::min<uint64_t>(std::max(tu.chunk_paths, kBatchMaxChunk), total)));
         r->npipes = total >= kPipeMinPaths ? KR_RENDER_PIPES : 1;
+        tail_config(r);
         // the autotuner may re-upload the scenes with deeper trees: room for any
         rc = render_buffers(r, size_t(std::max(dev->num_cu, 1)) * kMaxBlocksPerCu * kBlock, 0);
         // pipes 1.. accumulate into images of their own (kr_batch_read sums them)

@@ -23,6 +23,7 @@ KR_MEM_DEVICE = 1
 KR_RENDER_NO_MATERIAL_SORT = 0x1
 KR_RENDER_PERSISTENT_TRAVERSAL = 0x2
 KR_RENDER_ONE_STREAM = 0x4
+KR_RENDER_NO_TAIL_ROUNDS = 0x8
 
 SHADER_LAMBERT, SHADER_GLOSSY, SHADER_EMISSION = 0, 1, 2
 SHADER_GLASS, SHADER_REFRACTION, SHADER_TRANSPARENT, SHADER_BLACKHOLE, SHADER_MIXED = 3, 4, 5, 6, 7
@@ -60,7 +61,8 @@ class PointLight(C.Structure):
 
 
 KR_SCENE_NO_AREA_LIGHTS = 0x1
-KR_SCENE_COMPRESSED_NODES = 0x2
+KR_SCENE_COMPRESSED_NODES = 0x2  # the default node format
+KR_SCENE_FULL_NODES = 0x4
 
 
 class SceneInfo(C.Structure):

@@ -140,10 +140,12 @@ typedef struct kr_point_light {
 /* No area lights are derived: the reference skips light promotion when any
  * shader name has no configuration (src/core/meta/scene.cu:193-216). */
 #define KR_SCENE_NO_AREA_LIGHTS 0x1u
-/* Store BVH child boxes quantised to 8 bits per plane (16 + 10W bytes per
- * node) instead of the default full FP32 boxes (28W bytes, 16-byte padded).
- * Traversal decodes them conservatively: same hits, fewer bytes, more ALU. */
+/* BVH child boxes quantised to 8 bits per plane (16 + 10W bytes per node),
+ * decoded conservatively by traversal: same hits as full boxes, fewer bytes
+ * per node visit. The default; the flag is accepted and changes nothing. */
 #define KR_SCENE_COMPRESSED_NODES 0x2u
+/* Full FP32 child boxes instead (28W bytes per node, 16-byte padded). */
+#define KR_SCENE_FULL_NODES 0x4u
 
 typedef struct kr_scene_info {
     uint32_t num_prims, num_triangles, num_spheres, num_lights;
@@ -200,6 +202,11 @@ enum kr_kernel_type {
 #define KR_RENDER_ONE_STREAM 0x4u /* run every chunk on the device's stream; by
                                      default a render of >= 8 M paths cycles its
                                      chunks through four streams (same image)   */
+#define KR_RENDER_NO_TAIL_ROUNDS 0x8u /* traverse each bounce's queue in one launch;
+                                         by default walks that outlast a step
+                                         budget are resumed, packed into full
+                                         waves, by follow-up launches (same
+                                         image)                                 */
 
 /* ---- ray query buffers -------------------------------------------------- */
 

@@ -110,7 +110,7 @@ def test_closest_hit_mesh_scene(device, mesh):
 @pytest.mark.parametrize("width", [2, 4, 8])
 @pytest.mark.parametrize("nodes", ["compressed", "full"])
 def test_closest_hit_bvh_widths(device, mesh, width, nodes):
-    flags = kr.KR_SCENE_COMPRESSED_NODES if nodes == "compressed" else 0
+    flags = kr.KR_SCENE_FULL_NODES if nodes == "full" else 0
     gs = kr.GpuScene(device, mesh.to_desc(bvh_width=width, flags=flags))
     info = gs.info()
     assert info.bvh_width == width
@@ -121,9 +121,11 @@ def test_closest_hit_bvh_widths(device, mesh, width, nodes):
     gs.free()
 
 
-def test_shadow_segments(device, mesh, spheres):
+@pytest.mark.parametrize("width,nodes", [(4, "full"), (4, "compressed"), (8, "compressed"), (2, "compressed")])
+def test_shadow_segments(device, mesh, spheres, width, nodes):
+    flags = kr.KR_SCENE_FULL_NODES if nodes == "full" else 0
     for scene, seed in ((mesh, 5), (spheres, 6)):
-        gs = kr.GpuScene(device, scene.to_desc())
+        gs = kr.GpuScene(device, scene.to_desc(bvh_width=width, flags=flags))
         orc = ol.OracleScene(scene, "f64", traversal=1)
         o, d = _surface_rays(scene, orc, 40000, seed)
         tmax = np.random.default_rng(seed).uniform(0.01, 8.0, o.shape[0]).astype(np.float32)

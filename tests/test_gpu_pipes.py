@@ -133,3 +133,30 @@ def test_lane_refill_is_bit_identical(device, spheres, which):
     for f, ((rgb, val), c) in zip(flags[1:], runs[1:]):
         assert c == c0, (f, c, c0)
         assert np.array_equal(val, val0) and np.array_equal(rgb, rgb0), f
+
+
+@pytest.mark.parametrize("steps,rounds,stack", [("24", "2", "12"), ("1", "1", "12"), ("4", "4", "2"),
+                                                ("8", "3", "64")])
+def test_tail_rounds_are_bit_identical(device, monkeypatch, steps, rounds, stack):
+    """Walks cut after a step budget and resumed by follow-up launches from
+    the tail queues (k_trace_tail, k_shadow_tail) visit what one launch's
+    walks visit: the same node visits and triangle tests in total and the
+    same hits, so the image is the same bit for bit. The budgets force every
+    path: queues that fill up (1 step: more walks outlast it than half a
+    queue holds), stacks too deep for the queue (2 entries), several rounds,
+    and tails from the camera rays on (KR_TAIL_FROM=0)."""
+    scene = sg.mesh_scene(width=1024, height=512, spp=16)  # 8.4 M paths: four streams
+    ref = _render_stats(device, scene, kr.KR_RENDER_NO_TAIL_ROUNDS)
+    ref_plain = _render(device, scene, flags=kr.KR_RENDER_NO_TAIL_ROUNDS)
+    monkeypatch.setenv("KR_TAIL_STEPS", steps)
+    monkeypatch.setenv("KR_TAIL_ROUNDS", rounds)
+    monkeypatch.setenv("KR_TAIL_STACK", stack)
+    monkeypatch.setenv("KR_TAIL_FROM", "0")
+    (rgb, val), c = _render_stats(device, scene, 0)
+    (rgb0, val0), c0 = ref
+    print("closest rays, node visits, triangle tests:", c0, c, flush=True)
+    assert c == c0
+    assert np.array_equal(val, val0) and np.array_equal(rgb, rgb0)
+    rgb1, val1 = _render(device, scene)
+    assert np.array_equal(val1, ref_plain[1]) and np.array_equal(rgb1, ref_plain[0])
+    assert np.array_equal(val1, val0) and np.array_equal(rgb1, rgb0)
