@@ -423,7 +423,8 @@ def bench_batch(args, kr, sg, barrier):
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_trace (closest-hit wide-BVH traversal), timed on the jobs rendered one by one",
+            "kernel": "closest-hit wide-BVH traversal of one bounce's ray queue (k_trace, or k_trace_tail and its "
+                      "follow-up launches), timed on the jobs rendered one by one",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -432,7 +433,7 @@ def bench_batch(args, kr, sg, barrier):
             "bytes_per_query": bytes_per_query,
             "queries_per_launch": closest / max(1, trace_launches),
             "avg_launch_ms": trace_busy / max(1, trace_launches),
-            "launch_time": "union of the k_trace launch intervals per launch (four streams per render)",
+            "launch_time": "union of the traversal intervals per queue (four streams per render)",
             "avg_overlapped_launch_ms": trace_ms / max(1, trace_launches),
         },
         "cpu_baseline": cpu,
@@ -544,10 +545,11 @@ def main():
     log(f"[bench] {args.steps} steps in {elapsed:.3f}s")
     rgb, valid = rend.read()
     value = paths_per_step * args.steps / elapsed
-    # the kernel's average launch duration: HIP events around every k_trace
-    # launch, on the stream it runs on, over the timed steps (a frame's chunks
-    # run on four streams, so a launch shares the GPU with the other streams'
-    # kernels for part of its duration)
+    # the closest-hit traversal of one bounce's queue: one k_trace launch for
+    # the camera rays, a k_trace_tail launch and its follow-up launches for
+    # the later bounces. HIP events around it, on the stream it runs on, over
+    # the timed steps (a frame's chunks run on four streams, so it shares the
+    # GPU with the other streams' kernels for part of that time)
     avg_launch_ms = st.trace_ms / max(1, st.trace_launches)
     queries_per_launch = st.closest_rays / max(1, st.trace_launches)
     achieved = queries_per_launch * bytes_per_query / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
@@ -623,7 +625,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_trace (closest-hit wide-BVH traversal)",
+            "kernel": "closest-hit wide-BVH traversal of one bounce's ray queue (k_trace for camera rays; "
+                      "k_trace_tail and its follow-up launches for later bounces)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -635,16 +638,18 @@ def main():
             "queries_per_launch": queries_per_launch,
             "avg_launch_ms": avg_launch_ms,
             "launches": int(st.trace_launches),
-            "launch_time": "mean of HIP-event durations of every k_trace launch over the timed steps, each "
-                           "recorded on the stream the launch runs on (four streams per frame share the GPU)",
-            # the HBM roofline is the reference line, not the limiter: on one
-            # stream the kernel's real HBM + MALL traffic is 3.18 GB per launch
-            # (23 % of peak) against 15 GB of algorithmic bytes, and it issues
-            # VALU instructions in 79 % of the SIMDs' cycles with its vector
-            # memory units 89-91 % busy (rocprofv3 FETCH_SIZE / WRITE_SIZE, SQ,
-            # TA and TD counters of this frame)
-            "limiter": "VALU issue (79 % of SIMD cycles) and the vector memory pipeline (TA/TD 89-91 % busy) of "
-                       "divergent traversal waves (30 % of lanes active); 79 % of the algorithmic bytes hit in L2",
+            "launch_time": "mean duration of the traversal of one queue (its first launch to its last follow-up), "
+                           "HIP events recorded around it on the stream it runs on over the timed steps (four "
+                           "streams per frame share the GPU)",
+            # the HBM roofline is the reference line, not the limiter: the
+            # first k_trace_tail launch of a bounce moves 1.72 GB of HBM + MALL
+            # traffic against 8.97 GB of algorithmic bytes, issues VALU
+            # instructions in ~100 % of the SIMDs' cycles with 36 % of their
+            # lanes active, and keeps the texture address unit 82 % busy
+            # (rocprofv3 FETCH_SIZE / WRITE_SIZE, SQ and TA counters of this frame)
+            "limiter": "VALU issue (the first traversal launch of a bounce issues VALU in ~100 % of SIMD cycles, "
+                       "36 % of lanes active) with the vector memory address unit 82 % busy; ~70 % of the "
+                       "algorithmic bytes hit in L2",
         },
         "cpu_baseline": cpu,
     }
