@@ -325,6 +325,16 @@ void compress_wide_nodes(const WideBvh &bvh, std::vector<uint32_t> &out) {
         }
         o[3] = uint32_t(uint8_t(int8_t(e[0]))) | uint32_t(uint8_t(int8_t(e[1]))) << 8 |
                uint32_t(uint8_t(int8_t(e[2]))) << 16;
+        if (W == 4) {
+            // the 8 bytes of padding hold the scales themselves, which the
+            // kernels read instead of building 2^e from the exponent bytes:
+            // word 14 = the upper halves of s_x and s_y (a power of two is
+            // exact in them), word 15 = s_z
+            uint32_t sb[3];
+            std::memcpy(sb, sc, sizeof sb);
+            o[14] = (sb[0] >> 16) | (sb[1] & 0xFFFF0000u);
+            o[15] = sb[2];
+        }
         for (uint32_t c = 0; c < W; ++c) o[4 + c] = ref[c];
         uint8_t *q = reinterpret_cast<uint8_t *>(o + 4 + W);
         for (uint32_t c = 0; c < W; ++c) {
@@ -354,6 +364,13 @@ void decode_qnode_box(const uint32_t *node, uint32_t W, uint32_t child, float lo
         std::memcpy(&p, &node[a], 4);
         int e = int(int8_t(uint8_t(node[3] >> (8 * a))));
         float s = qscale(e);
+        if (W == 4) {
+            // the scale words the W = 4 kernels read; they must equal 2^e
+            const uint32_t bits = a == 0 ? node[14] << 16 : (a == 1 ? node[14] & 0xFFFF0000u : node[15]);
+            float sw;
+            std::memcpy(&sw, &bits, 4);
+            if (sw != s) s = std::numeric_limits<float>::quiet_NaN();  // fails every containment check
+        }
         lo[a] = qdecode(q[(2 * a) * W + child], s, p);
         hi[a] = qdecode(q[(2 * a + 1) * W + child], s, p);
     }

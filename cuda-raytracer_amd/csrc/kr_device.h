@@ -410,8 +410,15 @@ __device__ __forceinline__ void qnode_enter(const float4 *nodes, uint32_t ref, c
     uint32_t nw[3][AW], fw[3][AW];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int e = int(int8_t(uint8_t(u[3] >> (8 * a))));
-        A[a] = __uint_as_float(uint32_t(e + 127) << 23) * inv[a];
+        float s;
+        if (W == 4) {
+            // scales stored as floats in the node's padding (s_x, s_y as upper halves)
+            s = __uint_as_float(a == 0 ? u[14] << 16 : (a == 1 ? u[14] & 0xFFFF0000u : u[15]));
+        } else {
+            const int e = int(int8_t(uint8_t(u[3] >> (8 * a))));
+            s = __uint_as_float(uint32_t(e + 127) << 23);
+        }
+        A[a] = s * inv[a];
         B[a] = fmaf(__uint_as_float(u[a]), inv[a], -oinv[a]);
         const bool neg = __float_as_uint(inv[a]) >> 31;
 #pragma unroll

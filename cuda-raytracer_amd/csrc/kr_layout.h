@@ -38,6 +38,8 @@ constexpr uint32_t node_floats(uint32_t w) { return (7 * w + 3) / 4 * 4; }
 //   word 3     per-axis scale exponents e (int8 x 3): scale s_a = 2^e_a
 //   word 4..   child refs (W x uint32, same encoding as above)
 //   then       6W bytes: lo_x[W] hi_x[W] lo_y[W] hi_y[W] lo_z[W] hi_z[W]
+//   W = 4 only: word 14 = upper halves of s_x (low 16 bits) and s_y (high 16
+//              bits), word 15 = s_z (FP32), the scales the kernels read
 // A child's box is lo = fma(q_lo, s, p), hi = fma(q_hi, s, p) in FP32; the
 // encoder rounds q_lo down and q_hi up until the decoded box contains the
 // exact one, so traversal only ever visits a superset of nodes. 64 B for
