@@ -16,7 +16,10 @@ counts, seeds, spp and a hash of the scene arrays.
 
 `--batch-job K` renders job K of the batched uneven workload
 (scenegen.batch_workload, at the job's own resolution and spp) instead, into
-tests/golden/oracle_batch<K>.npz.
+tests/golden/oracle_batch<K>.npz. `--crop` renders the 64 x 64 block of
+pixels centred in the frame instead of the subset, into
+tests/golden/oracle_<workload>_crop.npz; `--pixels 0` the full frame, into
+tests/golden/oracle_<workload>_full.npz.
 """
 import argparse
 import hashlib
@@ -52,6 +55,13 @@ def subset(s, n=4096, seed=99):
     return np.sort(rng.choice(npix, n, replace=False)).astype(np.uint32)
 
 
+def crop(s, size=64):
+    """The size x size block of pixels centred in the frame, row by row."""
+    x0, y0 = (s.width - size) // 2, (s.height - size) // 2
+    ys, xs = np.meshgrid(np.arange(y0, y0 + size), np.arange(x0, x0 + size), indexing="ij")
+    return (ys * s.width + xs).reshape(-1).astype(np.uint32)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="mesh", choices=list(sg.WORKLOADS))
@@ -59,6 +69,7 @@ def main():
     ap.add_argument("--pixels", type=int, default=4096, help="0 = full frame")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--batch-job", type=int, default=-1, help="job of the batched workload instead")
+    ap.add_argument("--crop", action="store_true", help="the 64 x 64 block centred in the frame")
     args = ap.parse_args()
     if args.batch_job >= 0:
         s = sg.batch_workload()[0][args.batch_job]
@@ -66,7 +77,13 @@ def main():
     else:
         s = sg.WORKLOADS[args.workload]()
         name = args.workload
-    pix = subset(s, args.pixels)
+    if args.crop:
+        pix = crop(s)
+        name += "_crop"
+    else:
+        pix = subset(s, args.pixels)
+        if pix.size == s.width * s.height and args.batch_job < 0:
+            name += "_full"
     orc = ol.OracleScene(s, "f64", traversal=1)
     acc = np.zeros((pix.size, 3))
     valid = np.zeros(pix.size, np.uint64)
