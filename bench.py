@@ -92,14 +92,29 @@ def cpu_baseline(scene, target_seconds):
     }
 
 
+def oracle_fixtures(workload):
+    """The committed converged FP64 oracle images of a workload: {"subset":
+    a fixed 4096-pixel subset, "crop": the 64 x 64 block centred in the frame,
+    "full": every pixel}, those that exist (tests/golden/oracle_<workload>
+    [_crop|_full].npz), each with the workload's lens strata and 65536 samples
+    per pixel."""
+    out = {}
+    for key, suffix in (("subset", ""), ("crop", "_crop"), ("full", "_full")):
+        path = os.path.join(ROOT, "tests", "golden", f"oracle_{workload}{suffix}.npz")
+        if os.path.exists(path):
+            out[key] = path
+    return out
+
+
 def rmse_vs_oracle(workload, rgb, valid, width):
     """RMSE of the rendered frame against the committed converged FP64 oracle
-    image (tests/golden/oracle_<workload>.npz: a fixed pixel subset, same lens
-    strata, 65536 samples per pixel), on linear radiance, unclamped and
-    clamped to [0, 1] like the reference's 8-bit output."""
+    image (the full frame where its fixture exists, else the fixed pixel
+    subset; same lens strata, 65536 samples per pixel), on linear radiance,
+    unclamped and clamped to [0, 1] like the reference's 8-bit output."""
     import numpy as np
-    path = os.path.join(ROOT, "tests", "golden", f"oracle_{workload}.npz")
-    if not os.path.exists(path):
+    fx = oracle_fixtures(workload)
+    path = fx.get("full", fx.get("subset"))
+    if path is None:
         return None
     f = np.load(path)
     pix = f["pixels"].astype(np.int64)
@@ -109,6 +124,75 @@ def rmse_vs_oracle(workload, rgb, valid, width):
     dc = np.clip(mine, 0, 1) - np.clip(ref, 0, 1)
     return {"unclamped": float(np.sqrt((d * d).mean())), "clamped": float(np.sqrt((dc * dc).mean())),
             "pixels": int(pix.size), "oracle_samples_per_pixel": int(f["seeds"]) * int(f["spp"])}
+
+
+def fit_noise_bias(ms, mse, oracle_frames):
+    """Least-squares fit of mse(m) = b^2 + sigma^2 / m over the frame counts
+    ms; the oracle's own noise (sigma^2 / oracle_frames, an average of that
+    many frames) is taken off b^2. Returns the signed bias b, sigma of one
+    frame, the fitted b^2 and the oracle noise^2."""
+    import numpy as np
+    A = np.stack([np.ones(len(ms)), 1.0 / np.asarray(ms, float)], axis=1)
+    (b2, s2), *_ = np.linalg.lstsq(A, np.asarray(mse, float), rcond=None)
+    b2t = b2 - s2 / oracle_frames
+    return {"bias": float(np.sign(b2t) * np.sqrt(abs(b2t))), "sigma_per_frame": float(np.sqrt(max(s2, 0.0))),
+            "bias_squared_fit": float(b2), "oracle_noise_squared": float(s2 / oracle_frames)}
+
+
+def fit_text(fit):
+    return ", ".join(f"{k}: b {v['clamped']['bias']:.4f} / sigma {v['clamped']['sigma_per_frame']:.4f} clamped, "
+                     f"b {v['unclamped']['bias']:.3f} / sigma {v['unclamped']['sigma_per_frame']:.3f}"
+                     for k, v in fit.items() if isinstance(v, dict))
+
+
+def rmse_fit(kr, scene, gs, workload, spp, nrc=None, frames=16):
+    """Noise and bias against the FP64 oracle, separated by fitting
+    RMSE^2(m) = b^2 + sigma^2 / m, where m independent frames of the
+    workload's spp (seeds 101, 102, ...) are averaged: m = 1, 2, 4, 8, 16,
+    i.e. 64 ... 1024 samples per pixel at 64 spp, every group of m frames
+    disjoint from the others and RMSE^2(m) their mean. The reference's lens
+    places sample k of a pixel at a fixed point of a ceil(sqrt(spp))^2 grid
+    (no jitter), so renders at other spp converge to other expectations:
+    only whole frames of the oracle's own spp share its expectation. The
+    oracle image averages 1024 such frames, so its own noise sigma^2 / 1024
+    is taken off b^2. Per oracle fixture (subset, crop, full), unclamped and
+    clamped to [0, 1]; nrc: the cached path with that (trained) cache."""
+    import numpy as np
+    fx = {k: np.load(v) for k, v in oracle_fixtures(workload).items()}
+    if not fx:
+        return None
+    cam = scene.camera_struct()
+    imgs = {k: [] for k in fx}
+    for i in range(frames):
+        r = kr.Render(gs, cam, scene.params(seed=101 + i, spp=spp))
+        if nrc is not None:
+            r.attach_nrc(nrc)
+        r.run(0, spp)
+        r.sync()
+        rgb, val = r.read()
+        r.free()
+        for k, f in fx.items():
+            pix = f["pixels"].astype(np.int64)
+            imgs[k].append(rgb[pix] / np.maximum(val[pix], 1)[:, None])
+    out = {"frames": frames, "spp_per_frame": spp}
+    ms = [m for m in (1, 2, 4, 8, 16, 32) if m <= frames]
+    for k, f in fx.items():
+        nseeds = int(f["seeds"])
+        res = {"pixels": int(f["pixels"].size)}
+        for key, g in (("unclamped", lambda x: x), ("clamped", lambda x: np.clip(x, 0.0, 1.0))):
+            ref = g(f["mean"])
+            im = np.stack([g(x) for x in imgs[k]])
+            y = []
+            for m in ms:
+                groups = im[: (frames // m) * m].reshape(frames // m, m, *im.shape[1:]).mean(axis=1)
+                y.append(float(((groups - ref) ** 2).mean()))
+            res[key] = {"samples_per_pixel": [m * spp for m in ms], "rmse": [float(np.sqrt(v)) for v in y],
+                        **fit_noise_bias(ms, y, nseeds)}
+            # the noise of the mean of all the frames: a b^2 well below it is
+            # not resolved by this many frames
+            res[key]["noise_squared_of_all_frames"] = res[key]["sigma_per_frame"] ** 2 / frames
+        out[k] = res
+    return out
 
 
 # The cache's two ways of ending paths (kr_nrc_config.flags) and their
@@ -282,6 +366,9 @@ def bench_cache(args, kr, scene, gs, p, dev):
                                      "speedup": plain_time / dt if rm <= targets["clamped"] else None}
             log(f"[bench] {m} cached + batched, {len(jobs)} frames of {sb} spp in one launch: "
                 f"{dt * 1e3:.1f} ms per frame, rmse {rm:.4f} clamped")
+        if not args.no_fit:
+            out["rmse_fit"] = rmse_fit(kr, scene, gs, args.workload, p.spp, nrc)
+            log(f"[bench] {m} bias fit: " + fit_text(out["rmse_fit"]))
         nrc.free()
         out["by_spp"] = [{"spp": r["spp"], "seconds": r["cached"]["seconds"], "rmse": r["cached"]["rmse"],
                           "rmse_clamped": r["cached"]["rmse_clamped"],
@@ -473,6 +560,8 @@ def main():
                     help="traversal kernels refill finished lanes from the queue")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fit", action="store_true",
+                    help="skip the noise / bias fit RMSE^2(m) = b^2 + sigma^2 / m over m = 1..16 frames")
     ap.add_argument("--no-torch", action="store_true", help="skip torch.cuda.synchronize (profiling runs)")
     args = ap.parse_args()
 
@@ -555,6 +644,12 @@ def main():
     achieved = queries_per_launch * bytes_per_query / (avg_launch_ms * 1e-3) / 1e9 if avg_launch_ms > 0 else 0.0
 
     rmse = rmse_vs_oracle(args.workload, rgb, valid, p.width)
+    fit = None
+    if not args.no_fit:
+        rend.set_profiling(0)
+        fit = rmse_fit(kr, scene, gs, args.workload, p.spp)
+        if fit:
+            log("[bench] plain bias fit: " + fit_text(fit))
     # every launch timed, over as many frames as the timed region, after one
     # frame that creates the events
     rend.set_profiling(kr.KR_PROFILE_EVENTS)
@@ -609,6 +704,8 @@ def main():
             # the timed steps' frames accumulated (steps x spp samples per pixel)
             "rmse_vs_fp64_oracle": rmse,
             "rmse_samples_per_pixel": p.spp * args.steps,
+            # noise and bias of the plain path against the oracle fixtures
+            "rmse_fit": fit,
             "wall_clock_per_frame_s": elapsed / args.steps,
             # frames after the timed region, every launch timed: per kernel kind
             # the busy time (union of its launches' intervals over the four

@@ -3714,6 +3714,16 @@ KR_EXPORT int kr_nrc_get_stats(kr_nrc *n, uint64_t *samples_trained, float *last
 __global__ __launch_bounds__(256, KR_NRC_INFER_WAVES) void k_nrc_infer(nrc::Weights W, NrcRender nr, const uint32_t *counters, float *L,
                                                                          unsigned long long *stats) {
     using namespace nrc;
+    // The block's copy of the weights and biases: every layer's fragments
+    // are read from LDS instead of from L1/L2 per 64 queries.
+    __shared__ __align__(16) uint16_t s_wt[kNumWeights];
+    __shared__ float s_bias[kNumBiases];
+    static_assert(kNumWeights % 8 == 0, "weights copied in 16-byte pieces");
+    for (int i = threadIdx.x; i < kNumWeights / 8; i += blockDim.x)
+        reinterpret_cast<uint4 *>(s_wt)[i] = reinterpret_cast<const uint4 *>(W.wt)[i];
+    for (int l = 0; l < kLayers; ++l)
+        for (int i = threadIdx.x; i < kDims[l + 1]; i += blockDim.x) s_bias[bb_off(l) + i] = W.master[b_off(l) + i];
+    __syncthreads();
     int lane = __lane_id(), s = lane & 15, q = lane >> 4;
     uint32_t n = counters[CTR(C_NRCQ)];
     if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats[S_NRCQ], (unsigned long long)n);
@@ -3724,7 +3734,7 @@ __global__ __launch_bounds__(256, KR_NRC_INFER_WAVES) void k_nrc_infer(nrc::Weig
             return base + k < n ? nr.qraw + size_t(base + k) * kRaw : nullptr;
         }, W);
         v4f out[4];
-        forward(in, out, W);
+        forward_from<true>(in, out, s_wt, s_bias);
         if (q == 0)
             for (int nt = 0; nt < 4; ++nt) {
                 uint32_t i = base + nt * 16 + s;
@@ -3803,7 +3813,8 @@ void nrc_after_chunk(kr_render *r, Pipe &pp, kr_nrc *n, const NrcRender &nr) {
     hipStream_t st = pp.stream;
     {
         Timed tm(r, K_OTHER, st);
-        int grid = int(std::min<size_t>((size_t(r->P) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));
+        // three resident blocks per CU (48 KB of LDS weights each, 158 VGPRs), looping
+        int grid = int(std::min<size_t>((size_t(r->P) + 255) / 256, size_t(r->scene->dev->num_cu) * 3));
         hipLaunchKernelGGL(k_nrc_infer, dim3(grid), dim3(256), 0, st, nrc_render_view(n), nr, pp.counters, pp.L,
                            r->stats);
         int egrid = int(std::min<size_t>((size_t(nr.train_cap) + 255) / 256, size_t(r->scene->dev->num_cu) * 8));

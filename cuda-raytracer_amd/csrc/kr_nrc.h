@@ -45,6 +45,13 @@ constexpr int bw_off(int l) {
     return o;
 }
 constexpr int kNumWeights = bw_off(kLayers);
+// biases alone, layer after layer: layer l starts at bb_off(l)
+constexpr int bb_off(int l) {
+    int o = 0;
+    for (int i = 0; i < l; ++i) o += kDims[i + 1];
+    return o;
+}
+constexpr int kNumBiases = bb_off(kLayers);
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef short v8s __attribute__((ext_vector_type(8)));
@@ -216,22 +223,34 @@ __device__ __forceinline__ void output_layer(const v4s x[4][4], v4f out[4], cons
 }
 
 // Forward of a wave's 64 samples; out[nt] in lanes q = 0 holds the RGB
-// prediction of sample nt*16 + s.
-__device__ __forceinline__ void forward(const v4s in[6][4], v4f out[4], const Weights &W) {
-    // The empty asm makes the weight pointers opaque on every call, so the
-    // compiler cannot hoist all 48 KB of weight fragments out of a caller's
-    // sample loop into registers (which leaves one wave per SIMD, or spills);
-    // they are reloaded from L1/L2 per layer instead.
-    const uint16_t *wt = W.wt;
-    const float *ms = W.master;
-    asm volatile("" : "+s"(wt), "+s"(ms));
+// prediction of sample nt*16 + s. wt: the BF16 W^T of every layer
+// (bw_off); bias: the FP32 biases, layer l at b_off(l) of the parameter
+// vector (PACKED = false) or at bb_off(l) of a biases-only copy (PACKED =
+// true, the LDS copy of k_nrc_infer).
+template <bool PACKED>
+__device__ __forceinline__ void forward_from(const v4s in[6][4], v4f out[4], const uint16_t *wt, const float *bias) {
+    // The empty asm makes the offsets opaque on every call, so the compiler
+    // cannot hoist all 48 KB of weight fragments out of a caller's sample
+    // loop into registers (which leaves one wave per SIMD, or spills); they
+    // are reloaded per layer instead (from L1/L2, or from LDS). The pointers
+    // themselves stay visible, so a __shared__ copy is still read with LDS
+    // instructions.
+    uint32_t z = 0;
+    asm volatile("" : "+s"(z));
+    wt += z;
+    bias += z;
+    auto bo = [](int l) { return PACKED ? bb_off(l) : b_off(l); };
     v4s h[4][4], g[4][4];
-    hidden_layer<6, 4>(in, h, wt + bw_off(0), ms + b_off(0));
-    hidden_layer<4, 4>(h, g, wt + bw_off(1), ms + b_off(1));
-    hidden_layer<4, 4>(g, h, wt + bw_off(2), ms + b_off(2));
-    hidden_layer<4, 4>(h, g, wt + bw_off(3), ms + b_off(3));
-    hidden_layer<4, 4>(g, h, wt + bw_off(4), ms + b_off(4));
-    output_layer(h, out, wt + bw_off(5), ms + b_off(5));
+    hidden_layer<6, 4>(in, h, wt + bw_off(0), bias + bo(0));
+    hidden_layer<4, 4>(h, g, wt + bw_off(1), bias + bo(1));
+    hidden_layer<4, 4>(g, h, wt + bw_off(2), bias + bo(2));
+    hidden_layer<4, 4>(h, g, wt + bw_off(3), bias + bo(3));
+    hidden_layer<4, 4>(g, h, wt + bw_off(4), bias + bo(4));
+    output_layer(h, out, wt + bw_off(5), bias + bo(5));
+}
+
+__device__ __forceinline__ void forward(const v4s in[6][4], v4f out[4], const Weights &W) {
+    forward_from<false>(in, out, W.wt, W.master);
 }
 
 }  // namespace nrc

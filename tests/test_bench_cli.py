@@ -24,3 +24,18 @@ def test_help_lists_workloads_and_cache_modes():
     assert r.returncode == 0
     for word in ("mesh", "spheres", "large", "room", "batch", "--cache-mode", "--cache-ladder"):
         assert word in r.stdout, word
+
+
+def test_noise_bias_fit_recovers_known_terms():
+    """The fit of mse(m) = b^2 + sigma^2 / m behind config.rmse_fit: on exact
+    data it returns b and sigma, with the oracle's own noise sigma^2 / 1024
+    taken off b^2; on data with b = 0 it reports no bias."""
+    sys.path.insert(0, ROOT)
+    import bench
+    ms = [1, 2, 4, 8, 16]
+    b, sigma = 0.03, 0.09
+    mse = [b * b + sigma * sigma / m + sigma * sigma / 1024 for m in ms]
+    fit = bench.fit_noise_bias(ms, mse, 1024)
+    assert abs(fit["bias"] - b) < 1e-9 and abs(fit["sigma_per_frame"] - sigma) < 1e-9
+    fit0 = bench.fit_noise_bias(ms, [sigma * sigma / m for m in ms], 1024)
+    assert abs(fit0["sigma_per_frame"] - sigma) < 1e-9 and abs(fit0["bias"]) < 3e-3

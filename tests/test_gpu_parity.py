@@ -264,14 +264,18 @@ def test_render_stats_and_profiling(device, mesh):
     gs.free()
 
 
-@pytest.mark.parametrize("workload", ["spheres", "mesh", "large"])
-def test_unbiased_against_converged_oracle(device, workload):
+@pytest.mark.parametrize("workload,part", [("spheres", ""), ("mesh", ""), ("large", ""), ("mesh", "_crop"),
+                                           ("large", "_crop"), ("spheres", "_full")])
+def test_unbiased_against_converged_oracle(device, workload, part):
     """Full-size frame vs the committed FP64 oracle image (same lens strata,
-    65536 samples per pixel on a fixed 4096-pixel subset): the squared error
-    against the oracle must equal the render's own Monte Carlo variance,
-    estimated from two independent seeds (bias^2 ~ 0)."""
+    65536 samples per pixel on a fixed 4096-pixel subset, on the 64 x 64
+    block centred in the frame, or on every pixel of the sphere frame): the
+    squared error against the oracle must equal the render's own Monte Carlo
+    variance, estimated from independent seeds (bias^2 ~ 0)."""
     import os
-    f = np.load(os.path.join(ol.ROOT, "tests", "golden", f"oracle_{workload}.npz"))
+    path = os.path.join(ol.ROOT, "tests", "golden", f"oracle_{workload}{part}.npz")
+    assert os.path.exists(path), f"missing oracle image {path}"
+    f = np.load(path)
     import sys
     sys.path.insert(0, os.path.join(ol.ROOT, "tests", "golden"))
     from make_oracle_images import scene_hash
