@@ -152,7 +152,8 @@ def rmse_fit(kr, scene, gs, workload, spp, nrc=None, frames=16):
     i.e. 64 ... 1024 samples per pixel at 64 spp, every group of m frames
     disjoint from the others and RMSE^2(m) their mean. The reference's lens
     places sample k of a pixel at a fixed point of a ceil(sqrt(spp))^2 grid
-    (no jitter), so renders at other spp converge to other expectations:
+    (no jitter; src/core/misc/lens.hpp:19,61-65), so renders at other spp
+    converge to other expectations:
     only whole frames of the oracle's own spp share its expectation. The
     oracle image averages 1024 such frames, so its own noise sigma^2 / 1024
     is taken off b^2. Per oracle fixture (subset, crop, full), unclamped and
