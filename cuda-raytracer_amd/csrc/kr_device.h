@@ -649,6 +649,24 @@ __device__ __forceinline__ bool closest_walk(const DevScene &S, const RayBox &rb
     return false;
 }
 
+// The children an any-hit walk enters at one node (key = entry distance, +inf
+// for a missed child): the farthest goes to `cur`, the others onto the stack
+// in sorted order; false when at least one child is entered. Entering in slot
+// order without the sort costs more node steps than the sort saves.
+template <int W, int LDS_DEPTH>
+__device__ __forceinline__ bool occluded_children(float (&key)[W], uint32_t (&ref)[W], Stack<LDS_DEPTH> &st,
+                                                  uint32_t &cur) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) key[c] = key[c] != __builtin_inff() ? -key[c] : __builtin_inff();
+    sort_children<W>(key, ref);
+#pragma unroll
+    for (int c = W - 1; c >= 1; --c)
+        if (key[c] != __builtin_inff()) st.push(ref[c], -key[c]);
+    if (key[0] == __builtin_inff()) return true;
+    cur = ref[0];
+    return false;
+}
+
 // Any hit on the segment (o, d, tmax): Scene::intersect(Seg)
 // (src/core/meta/scene.hpp:57-67). Triangles block for 0 < t <= tmax,
 // spheres for a ray hit with t <= tmax (src/core/primitive/primitive.hpp:16-21).
@@ -759,14 +777,7 @@ __device__ __forceinline__ bool trace_occluded_steps(const DevScene &S, f3 o, f3
             float key[W];
             uint32_t ref[W];
             node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
-#pragma unroll
-            for (int c = 0; c < W; ++c) key[c] = key[c] != __builtin_inff() ? -key[c] : __builtin_inff();
-            sort_children<W>(key, ref);
-#pragma unroll
-            for (int c = W - 1; c >= 1; --c)
-                if (key[c] != __builtin_inff()) st.push(ref[c], -key[c]);
-            pop = key[0] == __builtin_inff();
-            if (!pop) cur = ref[0];
+            pop = occluded_children<W, LDS_DEPTH>(key, ref, st, cur);
         } else {
             uint32_t first = cur & kLeafFirstMask;
             uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
@@ -804,14 +815,7 @@ __device__ __forceinline__ int occluded_walk(const DevScene &S, const RayBox &rb
             float key[W];
             uint32_t ref[W];
             node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
-#pragma unroll
-            for (int c = 0; c < W; ++c) key[c] = key[c] != __builtin_inff() ? -key[c] : __builtin_inff();
-            sort_children<W>(key, ref);
-#pragma unroll
-            for (int c = W - 1; c >= 1; --c)
-                if (key[c] != __builtin_inff()) st.push(ref[c], -key[c]);
-            pop = key[0] == __builtin_inff();
-            if (!pop) cur = ref[0];
+            pop = occluded_children<W, LDS_DEPTH>(key, ref, st, cur);
         } else {
             uint32_t first = cur & kLeafFirstMask;
             uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
