@@ -539,6 +539,8 @@ def main():
     ap.add_argument("--workload", default="large", choices=["mesh", "spheres", "large", "room", "batch"])
     ap.add_argument("--bvh-width", type=int, default=0)
     ap.add_argument("--chunk-paths", type=int, default=0)
+    ap.add_argument("--leaf-size", type=int, default=0,
+                    help="most triangles per BVH leaf (0: the library's default)")
     ap.add_argument("--full-nodes", action="store_true",
                     help="full FP32 BVH child boxes instead of the default 8-bit quantised ones")
     ap.add_argument("--no-cache", action="store_true",
@@ -589,7 +591,7 @@ def main():
     log(f"[bench] building workload {args.workload}")
     scene = sg.WORKLOADS[args.workload]()
     dev = kr.Device(0)
-    gs = kr.GpuScene(dev, scene.to_desc(bvh_width=args.bvh_width,
+    gs = kr.GpuScene(dev, scene.to_desc(bvh_width=args.bvh_width, leaf_size=args.leaf_size,
                                         flags=kr.KR_SCENE_FULL_NODES if args.full_nodes else 0))
     info = gs.info()
     log(f"[bench] scene: {info.num_triangles} tris, {info.num_spheres} spheres, {info.num_lights} lights, "
@@ -700,6 +702,7 @@ def main():
             "scene_data": "seeded synthetic scene generated in-process; the plain steps run without a cache",
             "width": p.width, "height": p.height, "spp": p.spp, "max_bounce": p.max_bounce,
             "triangles": info.num_triangles, "bvh_width": info.bvh_width, "bvh_node_bytes": info.node_bytes,
+            "bvh_nodes": info.bvh_nodes, "bvh_leaves": info.bvh_leaves,
             "paths_per_step": paths_per_step,
             "mrays_per_s": (st.closest_rays + st.shadow_rays) / elapsed / 1e6,
             # the timed steps' frames accumulated (steps x spp samples per pixel)

@@ -243,7 +243,11 @@ bool build_wide_bvh(const float *tri_verts, uint32_t num_tris, uint32_t width, u
         err = "bvh width must be 2, 4 or 8";
         return false;
     }
-    if (leaf_size == 0) leaf_size = 4;
+    // Default: one triangle per leaf. The walk's leaf step then tests exactly
+    // one triangle in every lane, where leaves of up to 4 made a wave wait for
+    // its largest leaf; on the 1M-triangle frame 3 % faster (100k-triangle
+    // frame 1.5 %) for 1.7x the nodes.
+    if (leaf_size == 0) leaf_size = 1;
     if (leaf_size > kMaxLeafTris) {
         err = "leaf size must be at most 16";
         return false;

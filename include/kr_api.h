@@ -122,7 +122,7 @@ typedef struct kr_scene_desc {
     uint32_t num_shaders;
     uint64_t light_seed;      /* seeds the 1024-entry area-light sample tables */
     uint32_t bvh_width;       /* 2, 4 or 8; 0 = library default (4)            */
-    uint32_t leaf_size;       /* max triangles per leaf; 0 = default (4)       */
+    uint32_t leaf_size;       /* max triangles per leaf; 0 = default (1)       */
     uint32_t flags;           /* KR_SCENE_* bits                               */
     const struct kr_point_light *point_lights; /* "pointLight" assets, in file order */
     uint32_t num_point_lights;
