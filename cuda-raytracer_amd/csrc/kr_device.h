@@ -591,6 +591,36 @@ __device__ __forceinline__ bool closest_pop(Stack<LDS_DEPTH> &st, float t, uint3
     return false;
 }
 
+// One iteration of the closest-hit walk: the node step if `cur` is an inner
+// node, then the leaf if `cur` is (or the node step just entered) a leaf,
+// then the pop if the node entered nothing or the leaf was tested; true when
+// the walk is over. Taking an entered leaf in the node's iteration keeps the
+// lane's sequence of nodes and leaves, so the hit is the same, but the lanes
+// that descend and the lanes at a leaf share iterations: with one triangle
+// per leaf most node steps enter a leaf. KR_PAIR_STEPS 0: one node or one
+// leaf per iteration.
+#ifndef KR_PAIR_STEPS
+#define KR_PAIR_STEPS 1
+#endif
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ bool closest_iter(const DevScene &S, const RayBox &rb, const ShearRay &sr,
+                                             Stack<LDS_DEPTH> &st, uint32_t &cur, HitRec &h, uint32_t &nvis,
+                                             uint32_t &ntri) {
+    bool pop = false;
+    const bool at_node = !(cur & kLeafBit);
+    if (at_node) {
+        ++nvis;
+        uint32_t next = closest_node<W, LDS_DEPTH, Q>(S, rb, cur, h.t, st);
+        pop = next == kEmpty;
+        if (!pop) cur = next;
+    }
+    if (!pop && (cur & kLeafBit) && (KR_PAIR_STEPS || !at_node)) {
+        ntri += closest_leaf(S, sr, cur, h);
+        pop = true;
+    }
+    return pop && !closest_pop(st, h.t, cur);
+}
+
 template <int W, int LDS_DEPTH, bool Q = false>
 __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, Stack<LDS_DEPTH> &st,
                                                 uint64_t *counts = nullptr) {
@@ -599,24 +629,14 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
     RayBox rb = make_raybox(o, d);
     ShearRay sr = make_shear(o, d);
     uint32_t cur = 0;  // root is always an inner node
-    uint64_t nvis = 0, ntri = 0;
+    uint32_t nvis = 0, ntri = 0;
     st.reset();
-    // One node or one leaf per iteration ("if-if"): a lane that reaches a
-    // leaf tests it in the same iteration in which the wave's other lanes
-    // take their next node, instead of waiting at the leaf until every
-    // lane's descent is over, which is what a `continue` into the node
-    // branch compiles to.
-    while (true) {
-        bool pop = true;
-        if (!(cur & kLeafBit)) {
-            ++nvis;
-            uint32_t next = closest_node<W, LDS_DEPTH, Q>(S, rb, cur, h.t, st);
-            pop = next == kEmpty;
-            if (!pop) cur = next;
-        } else {
-            ntri += closest_leaf(S, sr, cur, h);
-        }
-        if (pop && !closest_pop(st, h.t, cur)) break;
+    // One walk step per iteration, or two when a node step enters a leaf
+    // (see closest_iter). A lane that reaches a leaf tests it in the same
+    // iteration in which the wave's other lanes take their next node,
+    // instead of waiting at the leaf until every lane's descent is over,
+    // which is what a `continue` into the node branch compiles to.
+    while (!closest_iter<W, LDS_DEPTH, Q>(S, rb, sr, st, cur, h, nvis, ntri)) {
     }
     if (counts) {
         counts[0] = nvis;
@@ -626,7 +646,7 @@ __device__ __forceinline__ HitRec trace_closest(const DevScene &S, f3 o, f3 d, S
 }
 
 // trace_closest's walk resumed from a state (the next ref `cur`, the hit so
-// far `h` and the stack), for at most `steps` steps of one node or one leaf;
+// far `h` and the stack), for at most `steps` iterations of closest_iter;
 // true when the walk is over. A walk run in such pieces visits the same nodes
 // and leaves in the same order as one uninterrupted walk, so its hit is the
 // same. Visited nodes and tested triangles are added to nvis and ntri.
@@ -634,18 +654,8 @@ template <int W, int LDS_DEPTH, bool Q>
 __device__ __forceinline__ bool closest_walk(const DevScene &S, const RayBox &rb, const ShearRay &sr,
                                              Stack<LDS_DEPTH> &st, uint32_t &cur, HitRec &h, uint32_t steps,
                                              uint32_t &nvis, uint32_t &ntri) {
-    for (uint32_t k = 0; k < steps; ++k) {
-        bool pop = true;
-        if (!(cur & kLeafBit)) {
-            ++nvis;
-            uint32_t next = closest_node<W, LDS_DEPTH, Q>(S, rb, cur, h.t, st);
-            pop = next == kEmpty;
-            if (!pop) cur = next;
-        } else {
-            ntri += closest_leaf(S, sr, cur, h);
-        }
-        if (pop && !closest_pop(st, h.t, cur)) return true;
-    }
+    for (uint32_t k = 0; k < steps; ++k)
+        if (closest_iter<W, LDS_DEPTH, Q>(S, rb, sr, st, cur, h, nvis, ntri)) return true;
     return false;
 }
 
@@ -665,6 +675,44 @@ __device__ __forceinline__ bool occluded_children(float (&key)[W], uint32_t (&re
     if (key[0] == __builtin_inff()) return true;
     cur = ref[0];
     return false;
+}
+
+// One iteration of the any-hit walk, paired like closest_iter: the node
+// step, then the leaf the node step entered (or `cur` if it is a leaf), then
+// the pop. Returns 1 when a triangle blocks the segment, 0 when the walk
+// ends without one, -1 when it goes on.
+template <int W, int LDS_DEPTH, bool Q>
+__device__ __forceinline__ int occluded_iter(const DevScene &S, const RayBox &rb, const ShearRay &sr, float limit,
+                                             float tmax, Stack<LDS_DEPTH> &st, uint32_t &cur) {
+    bool pop = false;
+    const bool at_node = !(cur & kLeafBit);
+    if (at_node) {
+        // sort key: minus the entry distance (farthest first), +inf for a
+        // child the segment misses (last)
+        float key[W];
+        uint32_t ref[W];
+        node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
+        pop = occluded_children<W, LDS_DEPTH>(key, ref, st, cur);
+    }
+    if (!pop && (cur & kLeafBit) && (KR_PAIR_STEPS || !at_node)) {
+        uint32_t first = cur & kLeafFirstMask;
+        uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const float4 *tp = S.tris + 3 * size_t(first + k);
+            float4 a = tp[0], b = tp[1], c = tp[2];
+            float tt, b1, b2;
+            if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, limit, tt, b1, b2) &&
+                tt <= tmax)
+                return 1;
+        }
+        pop = true;
+    }
+    if (pop) {
+        if (st.empty()) return 0;
+        float tn;
+        st.pop(cur, tn);
+    }
+    return -1;
 }
 
 // Any hit on the segment (o, d, tmax): Scene::intersect(Seg)
@@ -751,9 +799,9 @@ __device__ __forceinline__ bool trace_occluded(const DevScene &S, f3 o, f3 d, fl
     return false;
 }
 
-// The same any-hit walk one node or one leaf per iteration ("if-if"),
-// without postponed leaves, entering the children farthest first: a segment
-// runs from a surface towards a light, and an occluder found early ends it.
+// The same any-hit walk in iterations of occluded_iter, without postponed
+// leaves, entering the children farthest first: a segment runs from a
+// surface towards a light, and an occluder found early ends it.
 // Measured by a CPU count on the 1M-triangle frame's light segments:
 // farthest-first visits 8 % fewer nodes per 64-segment wave than the
 // children's slot order, nearest-first 1 % more.
@@ -769,38 +817,14 @@ __device__ __forceinline__ bool trace_occluded_steps(const DevScene &S, f3 o, f3
     float limit = tmax * 1.00000012f;
     st.reset();
     uint32_t cur = 0;
-    while (true) {
-        bool pop = true;
-        if (!(cur & kLeafBit)) {
-            // sort key: minus the entry distance (farthest first), +inf for a
-            // child the segment misses (last)
-            float key[W];
-            uint32_t ref[W];
-            node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
-            pop = occluded_children<W, LDS_DEPTH>(key, ref, st, cur);
-        } else {
-            uint32_t first = cur & kLeafFirstMask;
-            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 *tp = S.tris + 3 * size_t(first + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                float tt, b1, b2;
-                if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, limit, tt, b1, b2) &&
-                    tt <= tmax)
-                    return true;
-            }
-        }
-        if (pop) {
-            if (st.empty()) break;
-            float tn;
-            st.pop(cur, tn);
-        }
+    int r;
+    while ((r = occluded_iter<W, LDS_DEPTH, Q>(S, rb, sr, limit, tmax, st, cur)) < 0) {
     }
-    return false;
+    return r == 1;
 }
 
 // trace_occluded_steps' walk over the triangles resumed from a state (the
-// next ref `cur` and the stack), for at most `steps` steps: 1 when a
+// next ref `cur` and the stack), for at most `steps` iterations: 1 when a
 // triangle blocks the segment, 0 when the walk ends without one, -1 when the
 // steps ran out first. Pieces of the walk visit what the whole walk visits,
 // in the same order. (The spheres are tested once, before the walk.)
@@ -808,31 +832,8 @@ template <int W, int LDS_DEPTH, bool Q>
 __device__ __forceinline__ int occluded_walk(const DevScene &S, const RayBox &rb, const ShearRay &sr, float limit,
                                              float tmax, Stack<LDS_DEPTH> &st, uint32_t &cur, uint32_t steps) {
     for (uint32_t s = 0; s < steps; ++s) {
-        bool pop = true;
-        if (!(cur & kLeafBit)) {
-            // sort key: minus the entry distance (farthest first), +inf for a
-            // child the segment misses (last)
-            float key[W];
-            uint32_t ref[W];
-            node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
-            pop = occluded_children<W, LDS_DEPTH>(key, ref, st, cur);
-        } else {
-            uint32_t first = cur & kLeafFirstMask;
-            uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const float4 *tp = S.tris + 3 * size_t(first + k);
-                float4 a = tp[0], b = tp[1], c = tp[2];
-                float tt, b1, b2;
-                if (tri_hit(sr, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, f3{c.x, c.y, c.z}, limit, tt, b1, b2) &&
-                    tt <= tmax)
-                    return 1;
-            }
-        }
-        if (pop) {
-            if (st.empty()) return 0;
-            float tn;
-            st.pop(cur, tn);
-        }
+        const int r = occluded_iter<W, LDS_DEPTH, Q>(S, rb, sr, limit, tmax, st, cur);
+        if (r >= 0) return r;
     }
     return -1;
 }
