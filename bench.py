@@ -16,12 +16,15 @@ against the radiance cache: first in the mode the workload specifies (paths
 end at their second diffuse or rough vertex, 2^16 training samples per frame
 from 1/16 of paths, Adam lr 1e-2), then in the first-vertex mode, against
 both the clamped and the unclamped RMSE of the plain 64-spp frame.
-`roofline` is for the closest-hit traversal kernel (k_trace): algorithmic
-bytes per query = 44 B of queue I/O + node visits x node bytes + triangle
-tests x 48 B, with the visit counts measured by the same kernel on a 4-spp
-sample of the frame, divided by the kernel's average launch duration from
-HIP events recorded around every k_trace launch on the stream it runs on,
-over the timed steps. `cpu_baseline` is the oracle's FP32 restatement of the
+`roofline` is for the closest-hit traversal of one bounce's ray queue (a
+k_trace launch for camera rays, a k_trace_tail launch and its follow-ups for
+later bounces): algorithmic bytes per query = 44 B of queue I/O + node
+visits x node bytes + triangle tests x 48 B, with the visit counts measured
+by the same kernels on a 4-spp sample of the frame, divided by the mean
+duration of one queue's traversal from HIP events recorded around its
+launches on the stream they run on, over the timed steps (four streams
+share the GPU). `roofline.one_stream` repeats the measurement on frames
+whose chunks run on one stream, after the timed region. `cpu_baseline` is the oracle's FP32 restatement of the
 reference CPU path (CPUMulticore thread scheme, reference BVH and traversal,
 release flags) on a bounded sample of the same frame on this host.
 """
@@ -666,6 +669,29 @@ def main():
     kst = rend.stats()
     rend.set_profiling(0)
 
+    # the same traversal with the chunks on one stream, where a queue's
+    # launches have the GPU to themselves: one untimed frame, then up to 3
+    # frames with HIP events around the traversal launches (after the timed
+    # region)
+    one_stream = None
+    if not args.one_stream:
+        p1 = scene.params(seed=1, chunk_paths=args.chunk_paths, flags=p.flags | kr.KR_RENDER_ONE_STREAM)
+        r1 = kr.Render(gs, scene.camera_struct(), p1)
+        r1.set_profiling(kr.KR_PROFILE_TRACE_EVENTS)
+        r1.run(0, p.spp)
+        r1.sync()
+        r1.reset()
+        n1 = min(3, args.steps)
+        for _ in range(n1):
+            r1.run(0, p.spp)
+        r1.sync()
+        s1 = r1.stats()
+        ms1 = s1.trace_ms / max(1, s1.trace_launches)
+        a1 = (s1.closest_rays / max(1, s1.trace_launches)) * bytes_per_query / (ms1 * 1e-3) / 1e9 if ms1 > 0 else 0.0
+        one_stream = {"avg_launch_ms": ms1, "achieved": a1, "frac": a1 / HBM_PEAK_GBS, "frames": n1,
+                      "launches": int(s1.trace_launches)}
+        r1.free()
+
     cached = None
     if not args.no_cache and os.path.exists(os.path.join(ROOT, "tests", "golden", f"oracle_{args.workload}.npz")):
         rend.set_profiling(0)
@@ -742,15 +768,18 @@ def main():
             "launch_time": "mean duration of the traversal of one queue (its first launch to its last follow-up), "
                            "HIP events recorded around it on the stream it runs on over the timed steps (four "
                            "streams per frame share the GPU)",
+            # the same figures with the frame's chunks on one stream (frames
+            # after the timed region): the traversal's own speed
+            "one_stream": one_stream,
             # the HBM roofline is the reference line, not the limiter: the
-            # first k_trace_tail launch of a bounce moves 1.72 GB of HBM + MALL
-            # traffic against 8.97 GB of algorithmic bytes, issues VALU
-            # instructions in ~100 % of the SIMDs' cycles with 36 % of their
-            # lanes active, and keeps the texture address unit 82 % busy
-            # (rocprofv3 FETCH_SIZE / WRITE_SIZE, SQ and TA counters of this frame)
-            "limiter": "VALU issue (the first traversal launch of a bounce issues VALU in ~100 % of SIMD cycles, "
-                       "36 % of lanes active) with the vector memory address unit 82 % busy; ~70 % of the "
-                       "algorithmic bytes hit in L2",
+            # first k_trace_tail launch of a bounce moves 2.08 GB of HBM + MALL
+            # traffic against 8.94 GB of algorithmic bytes, issues VALU
+            # instructions in 87 % of the SIMDs' cycles with 41 % of their
+            # lanes active, and keeps the texture data unit 91 % busy
+            # (rocprofv3 FETCH_SIZE / WRITE_SIZE, SQ and TA / TD counters of this frame)
+            "limiter": "VALU issue and the vector memory data path (the first traversal launch of a bounce issues "
+                       "VALU in 87 % of SIMD cycles with 41 % of lanes active, the texture data unit 91 % busy); "
+                       "~68 % of the algorithmic bytes hit in L2",
         },
         "cpu_baseline": cpu,
     }
