@@ -597,24 +597,19 @@ __device__ __forceinline__ bool closest_pop(Stack<LDS_DEPTH> &st, float t, uint3
 // the walk is over. Taking an entered leaf in the node's iteration keeps the
 // lane's sequence of nodes and leaves, so the hit is the same, but the lanes
 // that descend and the lanes at a leaf share iterations: with one triangle
-// per leaf most node steps enter a leaf. KR_PAIR_STEPS 0: one node or one
-// leaf per iteration.
-#ifndef KR_PAIR_STEPS
-#define KR_PAIR_STEPS 1
-#endif
+// per leaf most node steps enter a leaf.
 template <int W, int LDS_DEPTH, bool Q>
 __device__ __forceinline__ bool closest_iter(const DevScene &S, const RayBox &rb, const ShearRay &sr,
                                              Stack<LDS_DEPTH> &st, uint32_t &cur, HitRec &h, uint32_t &nvis,
                                              uint32_t &ntri) {
     bool pop = false;
-    const bool at_node = !(cur & kLeafBit);
-    if (at_node) {
+    if (!(cur & kLeafBit)) {
         ++nvis;
         uint32_t next = closest_node<W, LDS_DEPTH, Q>(S, rb, cur, h.t, st);
         pop = next == kEmpty;
         if (!pop) cur = next;
     }
-    if (!pop && (cur & kLeafBit) && (KR_PAIR_STEPS || !at_node)) {
+    if (!pop && (cur & kLeafBit)) {
         ntri += closest_leaf(S, sr, cur, h);
         pop = true;
     }
@@ -685,8 +680,7 @@ template <int W, int LDS_DEPTH, bool Q>
 __device__ __forceinline__ int occluded_iter(const DevScene &S, const RayBox &rb, const ShearRay &sr, float limit,
                                              float tmax, Stack<LDS_DEPTH> &st, uint32_t &cur) {
     bool pop = false;
-    const bool at_node = !(cur & kLeafBit);
-    if (at_node) {
+    if (!(cur & kLeafBit)) {
         // sort key: minus the entry distance (farthest first), +inf for a
         // child the segment misses (last)
         float key[W];
@@ -694,7 +688,7 @@ __device__ __forceinline__ int occluded_iter(const DevScene &S, const RayBox &rb
         node_keys<W, Q>(S.nodes, cur, rb, limit, key, ref);
         pop = occluded_children<W, LDS_DEPTH>(key, ref, st, cur);
     }
-    if (!pop && (cur & kLeafBit) && (KR_PAIR_STEPS || !at_node)) {
+    if (!pop && (cur & kLeafBit)) {
         uint32_t first = cur & kLeafFirstMask;
         uint32_t cnt = ((cur >> kLeafCountShift) & 0xFu) + 1u;
         for (uint32_t k = 0; k < cnt; ++k) {
