@@ -857,7 +857,7 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, false, Q)) void k_trace_tail
 
 // Closest-hit traversal with lane refill (Aila and Laine, "Understanding the
 // Efficiency of Ray Traversal on GPUs", HPG 2009): a wave walks its rays one
-// step (a node or a leaf) per iteration and, once fewer than kRefillBelow of
+// closest_iter per iteration and, once fewer than kRefillBelow of
 // its lanes still walk, hands the finished rays to their material buckets and
 // gives every idle lane the next ray of the queue (one atomic per wave for
 // each). With
@@ -929,19 +929,14 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, false, Q)) void k_trace_refi
             if (__ballot(i != kMissCode) == 0ull) break;
         }
         if (walking) {
-            // one step of trace_closest's walk: a node, or a leaf, then the
-            // pop of the next subtree in front of the hit if needed
-            bool pop = true;
-            if (!(cur & kLeafBit)) {
-                if (COUNT) ++nvis;
-                uint32_t next = closest_node<W, kLdsDepth, Q>(S, rb, cur, h.t, st);
-                pop = next == kEmpty;
-                if (!pop) cur = next;
-            } else {
-                uint32_t nt = closest_leaf(S, sr, cur, h);
-                if (COUNT) ntri += nt;
+            // one iteration of trace_closest's walk (closest_iter)
+            uint32_t nv = 0, nt = 0;
+            const bool over = closest_iter<W, kLdsDepth, Q>(S, rb, sr, st, cur, h, nv, nt);
+            if (COUNT) {
+                nvis += nv;
+                ntri += nt;
             }
-            if (pop && !closest_pop(st, h.t, cur)) {
+            if (over) {
                 walking = false;
                 hit.t[i] = h.t;
                 hit.u[i] = h.u;
