@@ -424,7 +424,7 @@ struct kr_render {
     double *acc = nullptr;
     uint32_t *valid = nullptr;
     int grid = 0;
-    // tail rounds (k_trace_tail, k_shadow_tail): step budget of a launch,
+    // tail rounds (k_trace_tail, k_shadow_tail): iteration budget of a launch,
     // follow-up launches per queue (the last one finishes every walk), first
     // bounce whose closest-hit walks use them (shadow segments: every bounce)
     uint32_t tail_steps = 0, tail_rounds = 0, tail_from = 1;
@@ -763,7 +763,8 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_trace(DevS
 // runs until its slowest ray is done: on the 1M-triangle frame a bounce ray
 // takes 17 steps (one node or one leaf each) at the median and the slowest of
 // 64 about 48, so most lanes idle through the end of every wave. Here a walk
-// that has taken `steps` steps leaves its state in the tail queue `tout` and
+// that has run `steps` iterations of closest_iter (a node and the leaf it
+// enters, or a leaf) leaves its state in the tail queue `tout` and
 // the lane moves on; the next launch (FIRST = false) resumes the queued walks
 // packed into full waves from `tin`. A walk whose stack is deeper than the
 // queue keeps, or that finds the queue full, finishes in its launch, and so
@@ -1391,7 +1392,7 @@ __global__ __launch_bounds__(kBlock, trace_waves(W, BATCH, Q)) void k_shadow(Dev
 }
 
 // Shadow segments in tail rounds, as k_trace_tail does for closest hits: a
-// walk that has taken `steps` steps is queued in `tout` with its state, and
+// walk that has run `steps` iterations is queued in `tout` with its state, and
 // the next launch (FIRST = false) resumes the queued walks packed into full
 // waves. Single renders and one-job batch chunks only.
 template <int W, bool Q, bool FIRST>
@@ -2681,7 +2682,7 @@ int check_view(const kr_camera *cam, const kr_params *prm) {
     return KR_OK;
 }
 
-// Tail rounds (k_trace_tail): a launch's step budget, the follow-up launches
+// Tail rounds (k_trace_tail): a launch's iteration budget, the follow-up launches
 // per queue and the stack entries a queued walk keeps; KR_TAIL_STEPS,
 // KR_TAIL_ROUNDS, KR_TAIL_STACK and KR_TAIL_FROM override them.
 uint32_t env_u32(const char *name, uint32_t dflt) {
@@ -3057,8 +3058,8 @@ int run_chunk(kr_render *r, Pipe &pp, uint64_t g0, uint32_t P, const BatchView *
                                    pp.bucket[1], pp.bucket[2], pp.counters, r->stats, sort, pp.spill, pp.spill_lanes);
             };
             // tail rounds: the first launch walks every ray for up to tail_steps
-            // steps, each follow-up resumes the walks queued by the previous
-            // one, the last without a step limit
+            // iterations, each follow-up resumes the walks queued by the
+            // previous one, the last without a limit
             auto trace_tail = [&](auto kern, int round) {
                 const TailQ &tin = pp.tail[(round + 1) & 1], &tout = pp.tail[round & 1];
                 uint32_t steps = round < int(r->tail_rounds) ? r->tail_steps : ~0u;
