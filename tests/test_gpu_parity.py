@@ -121,6 +121,27 @@ def test_closest_hit_bvh_widths(device, mesh, width, nodes):
     gs.free()
 
 
+@pytest.mark.parametrize("leaf", [2, 4, 16])
+def test_leaf_sizes_give_identical_hits(device, mesh, leaf):
+    """Leaves of several triangles (kr_scene_desc.leaf_size) make another
+    tree, but a closest hit is the nearest triangle with exact ties to the
+    lowest index and the boxes are conservative, so the hits are the same
+    bit for bit as with the default one triangle per leaf."""
+    orc = ol.OracleScene(mesh, "f64", traversal=1)
+    o, d = _surface_rays(mesh, orc, 30000, 7)
+    tmax = np.random.default_rng(7).uniform(0.01, 8.0, o.shape[0]).astype(np.float32)
+    one = kr.GpuScene(device, mesh.to_desc())
+    many = kr.GpuScene(device, mesh.to_desc(leaf_size=leaf))
+    assert many.info().bvh_leaves < one.info().bvh_leaves
+    g1, gm = one.trace_closest(o, d), many.trace_closest(o, d)
+    for a, b in zip(g1, gm):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(one.trace_shadow(o, d, tmax), many.trace_shadow(o, d, tmax))
+    _compare_closest(gm, orc.trace_closest(o, d))
+    one.free()
+    many.free()
+
+
 @pytest.mark.parametrize("width,nodes", [(4, "full"), (4, "compressed"), (8, "compressed"), (2, "compressed")])
 def test_shadow_segments(device, mesh, spheres, width, nodes):
     flags = kr.KR_SCENE_FULL_NODES if nodes == "full" else 0
