@@ -48,7 +48,7 @@ def test_closest_hit_large_scene(device, large):
     """Same primitive and triangle on >= 99.99 % of rays, on camera rays of
     the 1080p frame at two sub-pixel strata, rays leaving surface points and
     random rays through the scene box. Hit t within 1e-5 relative to t itself
-    on >= 99.99 % of the agreeing camera hits. Rays leaving a surface 4e-3
+    on every agreeing camera hit. Rays leaving a surface 4e-3
     along their direction, and rays starting anywhere in the scene box, often
     hit at t << 1, where the FP32 resolution of world coordinates (~1e-6 at
     |x| ~ 10) bounds any FP32 intersection, the reference's Moller-Trumbore
@@ -86,7 +86,7 @@ def test_closest_hit_large_scene(device, large):
               f"{bad:.2e} (max {rel.max():.2e}), beyond 1e-5 of max(t, 1) on {bad1:.2e}", flush=True)
         if frac < 0.9999:
             failures.append((name, "agreement", frac))
-        if (bad if name == "camera" else bad1) > 1e-4:
+        if (bad if name == "camera" else bad1) > 0:
             failures.append((name, "t", bad, bad1))
     assert not failures, failures
     orc.free()
