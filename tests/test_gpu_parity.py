@@ -152,7 +152,7 @@ def test_shadow_segments(device, mesh, spheres, width, nodes):
         tmax = np.random.default_rng(seed).uniform(0.01, 8.0, o.shape[0]).astype(np.float32)
         g = gs.trace_shadow(o, d, tmax)
         c = orc.trace_shadow(o, d, tmax)
-        assert (g == c).mean() >= 0.9999, f"shadow agreement {(g == c).mean():.6f}"
+        assert (g == c).all(), f"shadow agreement {(g == c).mean():.6f}"
         gs.free()
 
 
