@@ -73,8 +73,12 @@ constexpr int kBlock = 256;
 #ifndef KR_SHADOW_STEPS
 #define KR_SHADOW_STEPS 1
 #endif
+// The plain single-render shading kernel asks for 6 waves per SIMD (80
+// VGPRs, no spill; its own choice was 90 VGPRs, 5 waves): the four-stream
+// 1M-triangle frame 71.5 -> 71.0 ms. The cache, profiler and batch
+// instantiations need more registers and keep the compiler's choice.
 #ifndef KR_SHADE_WAVES
-#define KR_SHADE_WAVES 1
+#define KR_SHADE_WAVES 6
 #endif
 constexpr int kLdsDepth = KR_LDS_DEPTH;
 // The wave hint applies to the BVH2 / BVH4 traversals of single renders
@@ -1029,7 +1033,7 @@ struct ShadeArgs {
 // profiler may record vertices (a.pv). The plain instantiation carries
 // neither's code or registers.
 template <bool BATCH, bool NRC, bool PROF>
-__global__ __launch_bounds__(kBlock, KR_SHADE_WAVES) void k_shade(ShadeArgs a) {
+__global__ __launch_bounds__(kBlock, (!BATCH && !NRC && !PROF) ? KR_SHADE_WAVES : 1) void k_shade(ShadeArgs a) {
     const bool nr_on = NRC && a.nr.enabled, pv_on = PROF && a.pv.enabled;
     uint32_t n0 = a.counters[CTR(C_BUCKET + 0)], n1 = a.counters[CTR(C_BUCKET + 1)], n2 = a.counters[CTR(C_BUCKET + 2)];
     uint32_t total = n0 + n1 + n2;
